@@ -1,0 +1,532 @@
+// dpe_capi.hip -- the C-ABI of libdpe_hip.so (include/dpe_hip.h).
+//
+// One dpe_ctx = one GPU, one HIP stream, a pool of device buffers that grows to
+// the largest problem seen (the reference's GpuWorkspace, DPE.cpp:775-837) and a
+// pool of hipEvents for per-kernel timing.  dpe_process restates the launch
+// schedule of DPE::RunPatchMatch (DPE.cu:3130-3262) over the kernels of
+// dpe_kernels.hip.  Host-side set-up follows DPE::InuputInitialization and
+// CudaSpaceInitialization (DPE.cpp:934-1185); every exit() there is a status.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dpe_state.h"
+
+struct dpe_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    struct Buf {
+        void *ptr = nullptr;
+        size_t bytes = 0;
+    };
+    std::vector<Buf> bufs;
+    std::vector<hipEvent_t> events;
+    dpe_stats stats{};
+    size_t reserved = 0;
+};
+
+static thread_local std::string g_create_error;
+
+namespace {
+
+using namespace dpe;
+
+enum BufId {
+    B_IMAGES, B_QUADS, B_GDEPTH, B_CAMS, B_VC, B_SPATIAL, B_PLANES, B_COSTS, B_SELECTED, B_VW, B_FIT, B_WEAK,
+    B_WEAKREL, B_NEAREST, B_NEIGHBOURS, B_EDGENEIGH, B_COMPLEX, B_LABELBOUND, B_RADIUS, B_NMAP, B_EDGE, B_EDGELOW,
+    B_LABEL, B_CANDPOS, B_COSTMAT, B_NSEL, B_RPLANE, B_RCOST, B_NOWPLANE, B_NOWDEPTH, B_NOWCOST, B_WNORM, B_POSTOP,
+    B_POSTBL, B_POSTDISP, B_POSTWN, B_POSTFLAGS, B_PCOST, B_COUNTERS, B_GIVEN_PLANES, B_GIVEN_COSTS, B_COUNT
+};
+
+struct Fail {
+    dpe_status st;
+};
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);                \
+            throw Fail{DPE_ERR_HIP};                                                     \
+        }                                                                                \
+    } while (0)
+
+void *buffer(dpe_ctx *ctx, int id, size_t bytes) {
+    if (ctx->bufs.size() < (size_t)B_COUNT) ctx->bufs.resize(B_COUNT);
+    auto &b = ctx->bufs[id];
+    if (bytes == 0) bytes = 16;
+    if (b.bytes < bytes) {
+        if (b.ptr) {
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            HIPCHK(hipFree(b.ptr));
+            ctx->reserved -= b.bytes;
+            b.ptr = nullptr;
+            b.bytes = 0;
+        }
+        HIPCHK(hipMalloc(&b.ptr, bytes));
+        b.bytes = bytes;
+        ctx->reserved += bytes;
+    }
+    return b.ptr;
+}
+
+dpe_status fail(dpe_ctx *ctx, dpe_status st, const std::string &msg) {
+    ctx->err = msg;
+    return st;
+}
+
+// Plane-independent half of ComputeHomography (DPE.cu:459-485) and the
+// camera-centre distance of DepthToWeak (DPE.cu:2644-2649), in host fp32 with
+// the reference's operation order.
+ViewConst view_const(const dpe_camera &ref, const dpe_camera &src) {
+    ViewConst v{};
+    float ref_C[3], src_C[3];
+    ref_C[0] = -(ref.R[0] * ref.t[0] + ref.R[3] * ref.t[1] + ref.R[6] * ref.t[2]);
+    ref_C[1] = -(ref.R[1] * ref.t[0] + ref.R[4] * ref.t[1] + ref.R[7] * ref.t[2]);
+    ref_C[2] = -(ref.R[2] * ref.t[0] + ref.R[5] * ref.t[1] + ref.R[8] * ref.t[2]);
+    src_C[0] = -(src.R[0] * src.t[0] + src.R[3] * src.t[1] + src.R[6] * src.t[2]);
+    src_C[1] = -(src.R[1] * src.t[0] + src.R[4] * src.t[1] + src.R[7] * src.t[2]);
+    src_C[2] = -(src.R[2] * src.t[0] + src.R[5] * src.t[1] + src.R[8] * src.t[2]);
+    v.Rr[0] = src.R[0] * ref.R[0] + src.R[1] * ref.R[1] + src.R[2] * ref.R[2];
+    v.Rr[1] = src.R[0] * ref.R[3] + src.R[1] * ref.R[4] + src.R[2] * ref.R[5];
+    v.Rr[2] = src.R[0] * ref.R[6] + src.R[1] * ref.R[7] + src.R[2] * ref.R[8];
+    v.Rr[3] = src.R[3] * ref.R[0] + src.R[4] * ref.R[1] + src.R[5] * ref.R[2];
+    v.Rr[4] = src.R[3] * ref.R[3] + src.R[4] * ref.R[4] + src.R[5] * ref.R[5];
+    v.Rr[5] = src.R[3] * ref.R[6] + src.R[4] * ref.R[7] + src.R[5] * ref.R[8];
+    v.Rr[6] = src.R[6] * ref.R[0] + src.R[7] * ref.R[1] + src.R[8] * ref.R[2];
+    v.Rr[7] = src.R[6] * ref.R[3] + src.R[7] * ref.R[4] + src.R[8] * ref.R[5];
+    v.Rr[8] = src.R[6] * ref.R[6] + src.R[7] * ref.R[7] + src.R[8] * ref.R[8];
+    float Cr[3];
+    Cr[0] = ref_C[0] - src_C[0];
+    Cr[1] = ref_C[1] - src_C[1];
+    Cr[2] = ref_C[2] - src_C[2];
+    v.tr[0] = src.R[0] * Cr[0] + src.R[1] * Cr[1] + src.R[2] * Cr[2];
+    v.tr[1] = src.R[3] * Cr[0] + src.R[4] * Cr[1] + src.R[5] * Cr[2];
+    v.tr[2] = src.R[6] * Cr[0] + src.R[7] * Cr[1] + src.R[8] * Cr[2];
+    for (int i = 0; i < 9; ++i) v.Ks[i] = src.K[i];
+    float dc[3];
+    dc[0] = ref.c[0] - src.c[0];
+    dc[1] = ref.c[1] - src.c[1];
+    dc[2] = ref.c[2] - src.c[2];
+    v.base = std::sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]);
+    v.width = src.width;
+    v.height = src.height;
+    return v;
+}
+
+struct Upload {
+    dpe_ctx *ctx;
+    bool device_in;
+    void to(void *dst, const void *src, size_t bytes) {
+        if (bytes == 0) return;
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, device_in ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                              ctx->stream));
+    }
+};
+
+dpe_status validate(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in) {
+    if (!ctx) return DPE_ERR_INVALID_ARGUMENT;
+    if (!prm || !in) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "null params or inputs");
+    if (in->num_images > DPE_MAX_IMAGES)
+        return fail(ctx, DPE_ERR_TOO_MANY_IMAGES, "Can't process so much images: " + std::to_string(in->num_images));
+    if (in->num_images < 2) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "need a reference and at least one source");
+    if (in->width <= 0 || in->height <= 0 || (long)in->width * in->height > (1L << 30))
+        return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad image size");
+    if (in->memory != DPE_MEM_HOST && in->memory != DPE_MEM_DEVICE)
+        return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad memory kind");
+    for (int i = 0; i < in->num_images; ++i) {
+        if (!in->images[i]) return fail(ctx, DPE_ERR_MISSING_INPUT, "image " + std::to_string(i) + " is null");
+        if (in->cameras[i].width != in->width || in->cameras[i].height != in->height)
+            return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "camera " + std::to_string(i) + " size differs from the images");
+        if (prm->geom_consistency && !in->geom_depths[i])
+            return fail(ctx, DPE_ERR_MISSING_INPUT, "geom_consistency needs every view's depth map");
+    }
+    if (prm->state < DPE_FIRST_INIT || prm->state > DPE_REFINE_ITER)
+        return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad run state");
+    if (prm->state != DPE_FIRST_INIT && (!in->prior_planes || !in->prior_selected))
+        return fail(ctx, DPE_ERR_MISSING_INPUT, "REFINE states need the prior planes and selected views");
+    if (prm->use_APD && !in->weak_info)
+        return fail(ctx, DPE_ERR_MISSING_INPUT, "Can't find weak info (use_APD)");
+    if ((prm->use_edge || prm->use_limit) && (!in->edges || !in->edges_low || in->low_width <= 0 || in->low_height <= 0))
+        return fail(ctx, DPE_ERR_MISSING_INPUT, "use_edge/use_limit need the edge maps");
+    if (prm->use_label && !in->labels) return fail(ctx, DPE_ERR_MISSING_INPUT, "use_label needs the label map");
+    if (prm->strong_increment <= 0 || prm->strong_radius < 0 || prm->weak_increment <= 0 || prm->max_iterations < 0)
+        return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad window parameters");
+    if (prm->rotate_time < 1 || prm->rotate_time > 4) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "rotate_time must be 1..4");
+    const int ntap = 2 * prm->strong_radius / prm->strong_increment + 1;
+    if (ntap > kMaxTaps) return fail(ctx, DPE_ERR_UNSUPPORTED, "strong window wider than 11 taps");
+    return DPE_OK;
+}
+
+// Common set-up of images, cameras and constants.  Returns the Dev with the
+// image/camera pointers filled.
+Dev setup_views(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in) {
+    Dev d{};
+    d.W = in->width;
+    d.H = in->height;
+    d.N = in->num_images;
+    d.NV = d.N - 1;
+    d.P = d.W * d.H;
+    d.half_rows = std::min(d.H, 2 * (((d.H / 2) + 15) / 16) * 16);
+    d.Wh = (d.W + 1) / 2;
+    d.Q = d.half_rows * d.Wh;
+    d.prm = *prm;
+    d.prm.num_images = d.N;
+    d.ntap = 2 * prm->strong_radius / prm->strong_increment + 1;
+    d.tap_off0 = -prm->strong_radius;
+    d.tap_inc = prm->strong_increment;
+    d.qs = (size_t)(d.W + 1) * (d.H + 1);
+    const size_t P = d.P;
+    Upload up{ctx, in->memory == DPE_MEM_DEVICE};
+    // images -> contiguous fp32 (reference) and quads of every view
+    float *imgs = (float *)buffer(ctx, B_IMAGES, sizeof(float) * P * d.N);
+    for (int i = 0; i < d.N; ++i) up.to(imgs + P * i, in->images[i], sizeof(float) * P);
+    float4 *quads = (float4 *)buffer(ctx, B_QUADS, sizeof(float4) * d.qs * d.N);
+    d.ref = imgs;
+    d.quads = quads;
+    std::vector<dpe_camera> cams(in->cameras, in->cameras + d.N);
+    std::vector<ViewConst> vcs(d.N);
+    for (int i = 0; i < d.N; ++i) vcs[i] = view_const(cams[0], cams[i]);
+    dpe_camera *dc = (dpe_camera *)buffer(ctx, B_CAMS, sizeof(dpe_camera) * d.N);
+    ViewConst *dv = (ViewConst *)buffer(ctx, B_VC, sizeof(ViewConst) * d.N);
+    HIPCHK(hipMemcpyAsync(dc, cams.data(), sizeof(dpe_camera) * d.N, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(dv, vcs.data(), sizeof(ViewConst) * d.N, hipMemcpyHostToDevice, ctx->stream));
+    d.cams = dc;
+    d.vc = dv;
+    // spatial exponents of the bilateral weight, -sqrt(i^2+j^2) / (2 ss^2) (DPE.cu:554-559)
+    std::vector<float> sp(d.ntap * d.ntap);
+    const float ss = prm->sigma_spatial;
+    for (int ki = 0; ki < d.ntap; ++ki)
+        for (int kj = 0; kj < d.ntap; ++kj) {
+            const float fi = (float)(d.tap_off0 + ki * d.tap_inc), fj = (float)(d.tap_off0 + kj * d.tap_inc);
+            const float spatial = std::sqrt(fi * fi + fj * fj);
+            sp[ki * d.ntap + kj] = -spatial / (2.0f * ss * ss);
+        }
+    float *dsp = (float *)buffer(ctx, B_SPATIAL, sizeof(float) * sp.size());
+    HIPCHK(hipMemcpyAsync(dsp, sp.data(), sizeof(float) * sp.size(), hipMemcpyHostToDevice, ctx->stream));
+    d.spatial = dsp;
+    // copies above are stream-ordered; the host vectors must outlive them
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < d.N; ++i) launch_make_quads(d, imgs + P * i, quads + d.qs * i, ctx->stream);
+    HIPCHK(hipGetLastError());
+    return d;
+}
+
+void copy_out(dpe_ctx *ctx, void *dst, const void *src, size_t bytes, bool device_out) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
+}
+
+struct EventTimer {
+    dpe_ctx *ctx;
+    int next = 0;
+    std::vector<std::pair<int, int>> ncc;
+    hipEvent_t get() {
+        if (next >= (int)ctx->events.size()) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            ctx->events.push_back(e);
+        }
+        return ctx->events[next++];
+    }
+    template <class F>
+    void ncc_launch(F f) {
+        const int a = next;
+        HIPCHK(hipEventRecord(get(), ctx->stream));
+        f();
+        HIPCHK(hipGetLastError());
+        const int b = next;
+        HIPCHK(hipEventRecord(get(), ctx->stream));
+        ncc.push_back({a, b});
+    }
+};
+
+dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in, dpe_outputs *out) {
+    dpe_status st = validate(ctx, prm, in);
+    if (st != DPE_OK) return st;
+    if (!out || !out->planes || !out->costs || !out->pixel_states || !out->selected_views)
+        return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "null output");
+    HIPCHK(hipSetDevice(ctx->device));
+    EventTimer tm{ctx};
+    hipEvent_t ev_start = tm.get();
+    Dev d = setup_views(ctx, prm, in);
+    const size_t P = d.P, Q = d.Q;
+    const int N = d.N, NV = d.NV;
+    Upload up{ctx, in->memory == DPE_MEM_DEVICE};
+    const bool dev_in = in->memory == DPE_MEM_DEVICE;
+    // geometric depths
+    if (prm->geom_consistency) {
+        float *g = (float *)buffer(ctx, B_GDEPTH, sizeof(float) * P * N);
+        for (int i = 0; i < N; ++i) up.to(g + P * i, in->geom_depths[i], sizeof(float) * P);
+        d.gdepth = g;
+    }
+    // pixel states and the weak index map (DPE.cpp:989-1030)
+    std::vector<uint8_t> weak_h(P, DPE_STRONG);
+    if (prm->use_APD) {
+        if (dev_in) {
+            HIPCHK(hipMemcpy(weak_h.data(), in->weak_info, P, hipMemcpyDeviceToHost));
+        } else {
+            std::memcpy(weak_h.data(), in->weak_info, P);
+        }
+    }
+    std::vector<int> nmap_h(P, 0);
+    int weak_count = 0;
+    for (size_t i = 0; i < P; ++i)
+        if (weak_h[i] == DPE_WEAK) nmap_h[i] = weak_count++;
+    d.weak = (uint8_t *)buffer(ctx, B_WEAK, P);
+    int *nmap = (int *)buffer(ctx, B_NMAP, sizeof(int) * P);
+    HIPCHK(hipMemcpyAsync(d.weak, weak_h.data(), P, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(nmap, nmap_h.data(), sizeof(int) * P, hipMemcpyHostToDevice, ctx->stream));
+    d.nmap = nmap;
+    // state buffers
+    d.planes = (float4 *)buffer(ctx, B_PLANES, sizeof(float4) * P);
+    d.costs = (float *)buffer(ctx, B_COSTS, sizeof(float) * P);
+    d.selected = (uint32_t *)buffer(ctx, B_SELECTED, sizeof(uint32_t) * P);
+    d.vw = (uint8_t *)buffer(ctx, B_VW, (size_t)kMaxImages * P);
+    d.fit = (float4 *)buffer(ctx, B_FIT, sizeof(float4) * P);
+    d.weak_rel = (uint8_t *)buffer(ctx, B_WEAKREL, P);
+    d.nearest = (short2 *)buffer(ctx, B_NEAREST, sizeof(short2) * P);
+    d.neighbours = (short2 *)buffer(ctx, B_NEIGHBOURS, sizeof(short2) * ((size_t)weak_count * kNeighbourNum + 1));
+    d.radius = (int *)buffer(ctx, B_RADIUS, sizeof(int) * P);
+    HIPCHK(hipMemsetAsync(d.fit, 0, sizeof(float4) * P, ctx->stream));
+    HIPCHK(hipMemsetAsync(d.vw, 0, (size_t)kMaxImages * P, ctx->stream));
+    HIPCHK(hipMemsetAsync(d.weak_rel, 0, P, ctx->stream));
+    {
+        std::vector<int> rad(P, prm->strong_radius);
+        HIPCHK(hipMemcpyAsync(d.radius, rad.data(), sizeof(int) * P, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    if (prm->state != DPE_FIRST_INIT) {
+        up.to(d.planes, in->prior_planes, sizeof(float4) * P);
+        up.to(d.selected, in->prior_selected, sizeof(uint32_t) * P);
+    } else {
+        HIPCHK(hipMemsetAsync(d.planes, 0, sizeof(float4) * P, ctx->stream));
+        HIPCHK(hipMemsetAsync(d.selected, 0, sizeof(uint32_t) * P, ctx->stream));
+    }
+    if (prm->use_edge || prm->use_limit) {
+        uint8_t *e = (uint8_t *)buffer(ctx, B_EDGE, P);
+        uint8_t *el = (uint8_t *)buffer(ctx, B_EDGELOW, (size_t)in->low_width * in->low_height);
+        up.to(e, in->edges, P);
+        up.to(el, in->edges_low, (size_t)in->low_width * in->low_height);
+        d.edge = e;
+        d.edge_low = el;
+        d.low_w = in->low_width;
+        d.low_h = in->low_height;
+    }
+    if (prm->use_edge) {
+        d.edge_neigh = (short2 *)buffer(ctx, B_EDGENEIGH, sizeof(short2) * 8 * P);
+        d.complexity = (float *)buffer(ctx, B_COMPLEX, sizeof(float) * P);
+        HIPCHK(hipMemsetAsync(d.complexity, 0, sizeof(float) * P, ctx->stream));
+    }
+    if (prm->use_label) {
+        int *lab = (int *)buffer(ctx, B_LABEL, sizeof(int) * P);
+        up.to(lab, in->labels, sizeof(int) * P);
+        d.label = lab;
+        d.label_bound = (short2 *)buffer(ctx, B_LABELBOUND, sizeof(short2) * ((size_t)weak_count * 8 + 1));
+        HIPCHK(hipMemsetAsync(d.label_bound, 0xFF, sizeof(short2) * ((size_t)weak_count * 8 + 1), ctx->stream));
+    }
+    // half-iteration and post-pass scratch
+    d.cand_pos = (int *)buffer(ctx, B_CANDPOS, sizeof(int) * kStrongSlots * Q);
+    d.cost_mat = (float *)buffer(ctx, B_COSTMAT, sizeof(float) * std::max((size_t)kStrongSlots * NV * Q, (size_t)NV * P));
+    d.nsel = (uint32_t *)buffer(ctx, B_NSEL, sizeof(uint32_t) * 4 * Q);
+    d.rplane = (float4 *)buffer(ctx, B_RPLANE, sizeof(float4) * kRefineSlots * Q);
+    d.rcost = (float *)buffer(ctx, B_RCOST, sizeof(float) * kRefineSlots * Q);
+    d.now_plane = (float4 *)buffer(ctx, B_NOWPLANE, sizeof(float4) * Q);
+    d.now_depth = (float *)buffer(ctx, B_NOWDEPTH, sizeof(float) * Q);
+    d.now_cost = (float *)buffer(ctx, B_NOWCOST, sizeof(float) * Q);
+    d.wnorm = (float *)buffer(ctx, B_WNORM, sizeof(float) * Q);
+    d.post_op = (float4 *)buffer(ctx, B_POSTOP, sizeof(float4) * P);
+    d.post_bl = (float *)buffer(ctx, B_POSTBL, sizeof(float) * P);
+    d.post_disp = (float *)buffer(ctx, B_POSTDISP, sizeof(float) * P);
+    d.post_wn = (float *)buffer(ctx, B_POSTWN, sizeof(float) * P);
+    d.post_flags = (uint8_t *)buffer(ctx, B_POSTFLAGS, P);
+    d.pcost = (float *)buffer(ctx, B_PCOST, sizeof(float) * kPostSlots * P);
+    d.counters = (unsigned long long *)buffer(ctx, B_COUNTERS, sizeof(unsigned long long) * 2);
+    HIPCHK(hipMemsetAsync(d.counters, 0, sizeof(unsigned long long) * 2, ctx->stream));
+    // GenNeighbours constants (DPE.cu:2151-2157): host double math
+    {
+        const float angle = 45.0f / prm->rotate_time;
+        d.gn_cos = (float)std::cos(angle * M_PI / 180.f);
+        d.gn_sin = (float)std::sin(angle * M_PI / 180.f);
+        d.gn_thr = (float)std::cos((angle / 2.0f) * M_PI / 180.0f);
+        d.gn_shift = std::max((int)(std::tan((angle / 2.0f) * M_PI / 180.0f) * 20), 1);
+    }
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipStreamSynchronize(s));
+    // ---- RunPatchMatch schedule (DPE.cu:3157-3234)
+    HIPCHK(hipEventRecord(ev_start, s));
+    launch_gen_edge_inform(d, s);
+    launch_find_nearest_strong(d, s);
+    launch_gen_neighbours(d, s);
+    launch_neighbour_update(d, s);
+    launch_random_init_planes(d, s);
+    tm.ncc_launch([&] { launch_ncc_current(d, s); });
+    launch_random_init_costs(d, s);
+    for (int it = 0; it < prm->max_iterations; ++it) {
+        for (int c = 0; c < 2; ++c) {
+            launch_strong_candidates(d, c, it, s);
+            tm.ncc_launch([&] { launch_ncc_strong(d, c, s); });
+            launch_strong_select(d, c, it, s);
+            tm.ncc_launch([&] { launch_ncc_refine(d, c, s); });
+            launch_strong_accept(d, c, s);
+        }
+        launch_ransac(d, it, s);
+        if (weak_count > 0) {
+            launch_weak_half(d, 0, it, s);
+            launch_weak_half(d, 1, it, s);
+        }
+    }
+    launch_depth_and_normal(d, s);
+    launch_filter(d, 0, s);
+    launch_filter(d, 1, s);
+    launch_post_prep(d, s);
+    tm.ncc_launch([&] { launch_ncc_post(d, s); });
+    launch_post_final(d, s);
+    HIPCHK(hipGetLastError());
+    hipEvent_t ev_stop = tm.get();
+    HIPCHK(hipEventRecord(ev_stop, s));
+    // ---- downloads (DPE.cu:3258-3260, plus the cost map)
+    const bool dev_out = dev_in;
+    copy_out(ctx, out->planes, d.planes, sizeof(float4) * P, dev_out);
+    copy_out(ctx, out->costs, d.costs, sizeof(float) * P, dev_out);
+    copy_out(ctx, out->pixel_states, d.weak, P, dev_out);
+    copy_out(ctx, out->selected_views, d.selected, sizeof(uint32_t) * P, dev_out);
+    unsigned long long cnt[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(cnt, d.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    dpe_stats stt{};
+    HIPCHK(hipEventElapsedTime(&stt.gpu_ms, ev_start, ev_stop));
+    for (auto &pr : tm.ncc) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->events[pr.first], ctx->events[pr.second]));
+        stt.ncc_ms += ms;
+    }
+    stt.ncc_launches = (int)tm.ncc.size();
+    stt.ncc_hyps = (int64_t)cnt[0];
+    stt.ncc_taps = (int64_t)cnt[1];
+    ctx->stats = stt;
+    return DPE_OK;
+}
+
+dpe_status ncc_cost_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in, int num_hyps, const float *planes,
+                         float *costs) {
+    dpe_params p = *prm;
+    p.use_APD = p.use_edge = p.use_limit = p.use_label = p.geom_consistency = 0;
+    p.state = DPE_FIRST_INIT;
+    dpe_status st = validate(ctx, &p, in);
+    if (st != DPE_OK) return st;
+    if (num_hyps < 1 || !planes || !costs) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad hypotheses");
+    HIPCHK(hipSetDevice(ctx->device));
+    EventTimer tm{ctx};
+    Dev d = setup_views(ctx, &p, in);
+    const size_t P = d.P;
+    const bool dev = in->memory == DPE_MEM_DEVICE;
+    const float4 *pl = (const float4 *)planes;
+    float *co = costs;
+    if (!dev) {
+        float4 *gp = (float4 *)buffer(ctx, B_GIVEN_PLANES, sizeof(float4) * P * num_hyps);
+        float *gc = (float *)buffer(ctx, B_GIVEN_COSTS, sizeof(float) * P * num_hyps * d.NV);
+        HIPCHK(hipMemcpyAsync(gp, planes, sizeof(float4) * P * num_hyps, hipMemcpyHostToDevice, ctx->stream));
+        pl = gp;
+        co = gc;
+    }
+    d.counters = (unsigned long long *)buffer(ctx, B_COUNTERS, sizeof(unsigned long long) * 2);
+    HIPCHK(hipMemsetAsync(d.counters, 0, sizeof(unsigned long long) * 2, ctx->stream));
+    tm.ncc_launch([&] { launch_ncc_given(d, num_hyps, pl, co, ctx->stream); });
+    if (!dev) HIPCHK(hipMemcpyAsync(costs, co, sizeof(float) * P * num_hyps * d.NV, hipMemcpyDeviceToHost, ctx->stream));
+    unsigned long long cnt[2];
+    HIPCHK(hipMemcpyAsync(cnt, d.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    dpe_stats stt{};
+    HIPCHK(hipEventElapsedTime(&stt.ncc_ms, ctx->events[tm.ncc[0].first], ctx->events[tm.ncc[0].second]));
+    stt.gpu_ms = stt.ncc_ms;
+    stt.ncc_launches = 1;
+    stt.ncc_hyps = (int64_t)cnt[0];
+    stt.ncc_taps = (int64_t)cnt[1];
+    ctx->stats = stt;
+    return DPE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dpe_abi_version(void) { return DPE_ABI_VERSION; }
+
+dpe_status dpe_create(int device, dpe_ctx **out) {
+    if (!out) {
+        g_create_error = "null output pointer";
+        return DPE_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        g_create_error = "no HIP device";
+        return DPE_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= n) {
+        g_create_error = "device index out of range";
+        return DPE_ERR_NO_DEVICE;
+    }
+    dpe_ctx *ctx = new dpe_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "cannot create a HIP stream";
+        delete ctx;
+        return DPE_ERR_HIP;
+    }
+    *out = ctx;
+    return DPE_OK;
+}
+
+void dpe_destroy(dpe_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (auto &b : ctx->bufs)
+        if (b.ptr) hipFree(b.ptr);
+    for (auto e : ctx->events) hipEventDestroy(e);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *dpe_last_error(const dpe_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+dpe_status dpe_process(dpe_ctx *ctx, const dpe_params *params, const dpe_inputs *in, dpe_outputs *out) {
+    if (!ctx) return DPE_ERR_INVALID_ARGUMENT;
+    try {
+        return process_impl(ctx, params, in, out);
+    } catch (const Fail &f) {
+        return f.st;
+    }
+}
+
+dpe_status dpe_last_stats(const dpe_ctx *ctx, dpe_stats *stats) {
+    if (!ctx || !stats) return DPE_ERR_INVALID_ARGUMENT;
+    *stats = ctx->stats;
+    return DPE_OK;
+}
+
+dpe_status dpe_ncc_cost(dpe_ctx *ctx, const dpe_params *params, const dpe_inputs *in, int num_hyps,
+                        const float *planes, float *costs) {
+    if (!ctx) return DPE_ERR_INVALID_ARGUMENT;
+    try {
+        return ncc_cost_impl(ctx, params, in, num_hyps, planes, costs);
+    } catch (const Fail &f) {
+        return f.st;
+    }
+}
+
+dpe_status dpe_fuse(dpe_ctx *ctx, const dpe_fuse_view *views, int num_views, const dpe_fuse_params *params,
+                    dpe_point *points, int64_t capacity, int64_t *num_points) {
+    if (!ctx) return DPE_ERR_INVALID_ARGUMENT;
+    (void)views; (void)num_views; (void)params; (void)points; (void)capacity; (void)num_points;
+    ctx->err = "dpe_fuse: not built yet";
+    return DPE_ERR_UNSUPPORTED;
+}
+
+}  // extern "C"

@@ -1,0 +1,1987 @@
+// dpe_kernels.hip -- gfx950 kernels of DPE-MVS's per-reference-image PatchMatch.
+//
+// Work decomposition (MI355X-first, not the reference's one-thread-per-pixel
+// monolith):
+//   * The bilateral NCC window (DPE.cu:696-782), the dominant cost, runs in two
+//     kernels that evaluate one (pixel, hypothesis) per thread over all source
+//     views: ncc_full writes the per-view cost matrix, ncc_weighted folds the
+//     views with the pixel's view weights.  Hypotheses of one slot (e.g. every
+//     pixel's "up far" candidate) share a grid row, so the 64 lanes of a wave
+//     gather neighbouring source footprints.  The reference-only half of each
+//     window (texels, bilateral weights, their sums) is computed once per thread
+//     into registers; each tap is then one 16-byte quad load + projection.
+//   * Control kernels (candidate search, view selection, acceptance) are
+//     per-pixel and touch O(slots) memory; they never evaluate a window.
+//   * The weak-texture path (GenNeighbours, RANSAC, weak propagation) keeps the
+//     reference's per-pixel structure: it runs on weak pixels only.
+// Every kernel cites the reference function it restates.
+#include "dpe_state.h"
+
+namespace dpe {
+
+enum { MODE_REFINE = 0, MODE_POST = 1 };
+
+__device__ __forceinline__ bool colour_pixel(const Dev &d, int colour, int q, int &x, int &y) {
+    if (colour < 0) {
+        if (q >= d.P) return false;
+        y = q / d.W;
+        x = q - y * d.W;
+        return true;
+    }
+    if (q >= d.Q) return false;
+    y = q / d.Wh;
+    const int k = q - y * d.Wh;
+    x = 2 * k + ((y + colour) & 1);
+    return y < d.half_rows && x < d.W;
+}
+
+__device__ __forceinline__ void count_add(unsigned long long *c, unsigned long long hyps, unsigned long long taps) {
+    if (!c) return;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        hyps += __shfl_down(hyps, off, 64);
+        taps += __shfl_down(taps, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (hyps) atomicAdd(c, hyps);
+        if (taps) atomicAdd(c + 1, taps);
+    }
+}
+
+// ------------------------------------------------------------------ images
+__global__ void k_make_quads(const float *__restrict__ img, float4 *__restrict__ quads, int W, int H) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = (W + 1) * (H + 1);
+    if (idx >= n) return;
+    const int iy = idx / (W + 1) - 1;
+    const int ix = idx % (W + 1) - 1;
+    quads[idx] = make_float4(texel(img, W, H, ix, iy), texel(img, W, H, ix + 1, iy), texel(img, W, H, ix, iy + 1),
+                             texel(img, W, H, ix + 1, iy + 1));
+}
+
+// ------------------------------------------------------------ NCC windows
+// Reference half of the strong window of pixel (x, y): bilateral weights
+// (ComputeBilateralWeight, DPE.cu:554-559) and the hypothesis-independent sums.
+template <int NT>
+struct Patch {
+    float w[NT * NT];
+    float wr[NT * NT];
+    float sum_ref, sum_ref_ref, wsum;
+};
+
+template <int NT>
+__device__ __forceinline__ void load_patch(const Dev &d, int x, int y, Patch<NT> &P) {
+    const float rc = texel(d.ref, d.W, d.H, x, y);
+    const float sc = d.prm.sigma_color;
+    const float cden = 2.0f * sc * sc;
+    float s_r = 0.0f, s_rr = 0.0f, s_w = 0.0f;
+#pragma unroll
+    for (int ki = 0; ki < NT; ++ki) {
+        float r_r = 0.0f, r_rr = 0.0f, r_w = 0.0f;
+#pragma unroll
+        for (int kj = 0; kj < NT; ++kj) {
+            const int k = ki * NT + kj;
+            const float ref = texel(d.ref, d.W, d.H, x + d.tap_off0 + ki * d.tap_inc, y + d.tap_off0 + kj * d.tap_inc);
+            const float w = d_expf(d.spatial[k] - fabsf(ref - rc) / cden);
+            const float wr = w * ref;
+            P.w[k] = w;
+            P.wr[k] = wr;
+            r_r = fmaf(w, ref, r_r);
+            r_rr = fmaf(wr, ref, r_rr);
+            r_w = r_w + w;
+        }
+        s_r += r_r;
+        s_rr += r_rr;
+        s_w += r_w;
+    }
+    P.sum_ref = s_r;
+    P.sum_ref_ref = s_rr;
+    P.wsum = s_w;
+}
+
+// ComputeBilateralNCCOld, DPE.cu:696-782, with the reference half precomputed.
+template <int NT>
+__device__ __forceinline__ float ncc_patch(const Dev &d, const Patch<NT> &P, int x, int y, int v, const float4 &pl,
+                                           unsigned long long &taps) {
+    const ViewConst &vc = d.vc[v];
+    float H[9];
+    homography(vc, d.cams[0], pl, H);
+    const float2 pt = project(H, x, y);
+    if (pt.x >= (float)vc.width || pt.x < 0.0f || pt.y >= (float)vc.height || pt.y < 0.0f) return 2.0f;
+    const float4 *__restrict__ q = d.quads + (size_t)v * d.qs;
+    float s_s = 0.0f, s_ss = 0.0f, s_rs = 0.0f;
+#pragma unroll
+    for (int ki = 0; ki < NT; ++ki) {
+        float r_s = 0.0f, r_ss = 0.0f, r_rs = 0.0f;
+        const int rx = x + d.tap_off0 + ki * d.tap_inc;
+#pragma unroll
+        for (int kj = 0; kj < NT; ++kj) {
+            const int k = ki * NT + kj;
+            const int ry = y + d.tap_off0 + kj * d.tap_inc;
+            const float2 sp = project(H, rx, ry);
+            const float src = bilinear_quad(q, d.W, d.H, sp.x, sp.y);
+            const float w = P.w[k];
+            r_s = fmaf(w, src, r_s);
+            r_ss = fmaf(w * src, src, r_ss);
+            r_rs = fmaf(P.wr[k], src, r_rs);
+        }
+        s_s += r_s;
+        s_ss += r_ss;
+        s_rs += r_rs;
+    }
+    taps += NT * NT;
+    return ncc_from_sums(P.sum_ref, P.sum_ref_ref, s_s, s_ss, s_rs, P.wsum);
+}
+
+// One bilateral window computed directly (any radius/increment): the shared
+// body of ComputeBilateralNCCNew (DPE.cu:613-672) and ComputeBilateralNCCOld.
+__device__ float ncc_window_direct(const Dev &d, int v, const float H[9], int cx, int cy, float rc, int radius,
+                                   int inc, unsigned long long &taps) {
+    const float4 *__restrict__ q = d.quads + (size_t)v * d.qs;
+    const float ss = d.prm.sigma_spatial, sc = d.prm.sigma_color;
+    float sum_ref = 0.0f, sum_ref_ref = 0.0f, sum_src = 0.0f, sum_src_src = 0.0f, sum_ref_src = 0.0f, wsum = 0.0f;
+    for (int i = -radius; i <= radius; i += inc) {
+        float r_ref = 0.0f, r_src = 0.0f, r_rr = 0.0f, r_ss = 0.0f, r_rs = 0.0f, r_w = 0.0f;
+        for (int j = -radius; j <= radius; j += inc) {
+            const int rx = cx + i, ry = cy + j;
+            const float ref = texel(d.ref, d.W, d.H, rx, ry);
+            const float2 sp = project(H, rx, ry);
+            const float src = bilinear_quad(q, d.W, d.H, sp.x, sp.y);
+            const float fi = (float)i, fj = (float)j;
+            const float spatial = sqrtf(fi * fi + fj * fj);
+            const float color = fabsf(ref - rc);
+            const float w = d_expf(-spatial / (2.0f * ss * ss) - color / (2.0f * sc * sc));
+            const float wr = w * ref;
+            r_ref = fmaf(w, ref, r_ref);
+            r_rr = fmaf(wr, ref, r_rr);
+            r_src = fmaf(w, src, r_src);
+            r_ss = fmaf(w * src, src, r_ss);
+            r_rs = fmaf(wr, src, r_rs);
+            r_w = r_w + w;
+            taps++;
+        }
+        sum_ref += r_ref;
+        sum_ref_ref += r_rr;
+        sum_src += r_src;
+        sum_src_src += r_ss;
+        sum_ref_src += r_rs;
+        wsum += r_w;
+    }
+    return ncc_from_sums(sum_ref, sum_ref_ref, sum_src, sum_src_src, sum_ref_src, wsum);
+}
+
+__device__ float ncc_old_direct(const Dev &d, int x, int y, int v, const float4 &pl, unsigned long long &hyps,
+                                unsigned long long &taps) {
+    const ViewConst &vc = d.vc[v];
+    float H[9];
+    homography(vc, d.cams[0], pl, H);
+    hyps++;
+    const float2 pt = project(H, x, y);
+    if (pt.x >= (float)vc.width || pt.x < 0.0f || pt.y >= (float)vc.height || pt.y < 0.0f) return 2.0f;
+    const float rc = texel(d.ref, d.W, d.H, x, y);
+    return ncc_window_direct(d, v, H, x, y, rc, d.prm.strong_radius, d.prm.strong_increment, taps);
+}
+
+__device__ __forceinline__ short2 neighbour_point(const Dev &d, int center, int k) {
+    return d.neighbours[(size_t)d.nmap[center] * kNeighbourNum + k];
+}
+
+// ComputeBilateralNCCNew, DPE.cu:561-694 (weak centre pixels)
+__device__ float ncc_new_direct(const Dev &d, int x, int y, int v, const float4 &pl, unsigned long long &hyps,
+                                unsigned long long &taps) {
+    const ViewConst &vc = d.vc[v];
+    const int center = x + y * d.W;
+    const float cost_max = 2.0f;
+    float H[9];
+    homography(vc, d.cams[0], pl, H);
+    hyps++;
+    const float2 pt = project(H, x, y);
+    if (pt.x >= (float)vc.width || pt.x < 0.0f || pt.y >= (float)vc.height || pt.y < 0.0f) return cost_max;
+    float cost = 0.0f;
+    if (d.weak[center] == DPE_WEAK) {
+        const float rc = texel(d.ref, d.W, d.H, x, y);
+        float center_cost = 0.0f, strong_cost = 0.0f;
+        int strong_count = 0;
+        for (int k = 0; k < kNeighbourNum; ++k) {
+            const short2 np = neighbour_point(d, center, k);
+            if (np.x == -1 || np.y == -1) continue;
+            const float2 nsp = project(H, np.x, np.y);
+            if (nsp.x < 0 || nsp.y < 0 || nsp.x >= (float)d.W || nsp.y >= (float)d.H) {
+                if (k != 0) {
+                    const uint32_t vi = d.selected[np.x + np.y * d.W];
+                    if (isSet(vi, v - 1)) {
+                        strong_cost += cost_max;
+                        strong_count++;
+                    }
+                    continue;
+                } else {
+                    return cost_max;
+                }
+            }
+            int radius = (k == 0 ? d.prm.strong_radius : d.prm.weak_radius);
+            int inc = (k == 0 ? d.prm.strong_increment : d.prm.weak_increment);
+            if (d.prm.use_radius && k == 0) {
+                radius = d.radius[center];
+                inc = max(2, (int)(2.0 * radius / 5.0));
+            }
+            const float tc = ncc_window_direct(d, v, H, np.x, np.y, rc, radius, inc, taps);
+            if (k == 0) {
+                center_cost = tc;
+            } else {
+                strong_cost += tc;
+                strong_count++;
+            }
+        }
+        if (strong_count == 0) {
+            cost = center_cost;
+        } else {
+            strong_cost /= strong_count;
+            strong_cost = fminf(strong_cost, cost_max);
+            cost = (float)(0.25 * center_cost + 0.75 * strong_cost);
+        }
+    }
+    return cost;
+}
+
+// Get3DPointonWorld_cu / ProjectonCamera_cu, DPE.cu:885-917
+__device__ __forceinline__ float3 point_world(float x, float y, float depth, const dpe_camera &cam) {
+    float3 X, T;
+    X.x = depth * (x - cam.K[2]) / cam.K[0];
+    X.y = depth * (y - cam.K[5]) / cam.K[4];
+    X.z = depth;
+    T.x = cam.R[0] * X.x + cam.R[3] * X.y + cam.R[6] * X.z;
+    T.y = cam.R[1] * X.x + cam.R[4] * X.y + cam.R[7] * X.z;
+    T.z = cam.R[2] * X.x + cam.R[5] * X.y + cam.R[8] * X.z;
+    X.x = T.x + cam.c[0];
+    X.y = T.y + cam.c[1];
+    X.z = T.z + cam.c[2];
+    return X;
+}
+__device__ __forceinline__ void project_camera(const float3 &X, const dpe_camera &cam, float2 &pt, float &depth) {
+    float3 t;
+    t.x = cam.R[0] * X.x + cam.R[1] * X.y + cam.R[2] * X.z + cam.t[0];
+    t.y = cam.R[3] * X.x + cam.R[4] * X.y + cam.R[5] * X.z + cam.t[1];
+    t.z = cam.R[6] * X.x + cam.R[7] * X.y + cam.R[8] * X.z + cam.t[2];
+    depth = cam.K[6] * t.x + cam.K[7] * t.y + cam.K[8] * t.z;
+    pt.x = (cam.K[0] * t.x + cam.K[1] * t.y + cam.K[2] * t.z) / depth;
+    pt.y = (cam.K[3] * t.x + cam.K[4] * t.y + cam.K[5] * t.z) / depth;
+}
+
+// ComputeGeomConsistencyCost, DPE.cu:919-957
+__device__ float geom_cost(const Dev &d, int x, int y, int v, const float4 &pl) {
+    const dpe_camera &ref = d.cams[0];
+    const dpe_camera &src = d.cams[v];
+    const float depth = depth_from_plane(ref, pl, x, y);
+    const float3 fwd = point_world((float)x, (float)y, depth, ref);
+    float2 sp;
+    float sd;
+    project_camera(fwd, src, sp, sd);
+    const int ix = f2i_clamp(sp.x, -1, d.W), iy = f2i_clamp(sp.y, -1, d.H);
+    const float src_depth = texel(d.gdepth + (size_t)v * d.P, d.W, d.H, ix, iy);
+    if (src_depth == 0.0f) return 3.0f;
+    const float3 s3 = point_world(sp.x, sp.y, src_depth, src);
+    float2 bp;
+    float rd;
+    project_camera(s3, ref, bp, rd);
+    const float dc = (float)x - bp.x, dr = (float)y - bp.y;
+    return fminf(3.0f, sqrtf(dc * dc + dr * dr));
+}
+
+// ---------------------------------------------------- NCC cost kernels (hot)
+// Per-view cost matrix out[(slot*NV + v-1)*Qn + q] of the plane of every
+// (pixel, slot): plane_src[cand_pos[slot*Qn+q]] if cand_pos is given (a -1
+// entry skips the slot), else plane_src[slot*Qn + q].
+template <int NT>
+__global__ __launch_bounds__(256) void k_ncc_full(Dev d, int colour, int Qn, const int *__restrict__ cand_pos,
+                                                   const float4 *__restrict__ plane_src, float *__restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int slot = blockIdx.y;
+    int x = 0, y = 0;
+    bool active = q < Qn && colour_pixel(d, colour, q, x, y);
+    float4 pl = make_float4(0, 0, 0, 0);
+    if (active) {
+        if (cand_pos) {
+            const int pos = cand_pos[(size_t)slot * Qn + q];
+            active = pos >= 0;
+            if (active) pl = plane_src[pos];
+        } else {
+            pl = plane_src[(size_t)slot * Qn + q];
+        }
+    }
+    unsigned long long hyps = 0, taps = 0;
+    if (active) {
+        if constexpr (NT > 0) {
+            Patch<NT> P;
+            load_patch<NT>(d, x, y, P);
+            for (int v = 1; v < d.N; ++v) {
+                out[((size_t)slot * d.NV + (v - 1)) * Qn + q] = ncc_patch<NT>(d, P, x, y, v, pl, taps);
+                hyps++;
+            }
+        } else {
+            for (int v = 1; v < d.N; ++v) {
+                unsigned long long h = 0;
+                out[((size_t)slot * d.NV + (v - 1)) * Qn + q] = ncc_old_direct(d, x, y, v, pl, h, taps);
+                hyps += h;
+            }
+        }
+    }
+    count_add(d.counters, hyps, taps);
+}
+
+// Weighted sums over views.
+//   MODE_REFINE: slot s of the refinement hypotheses (PlaneHypothesisRefinementStrong,
+//     DPE.cu:1101-1113): sum over views with weight > 0 of w*cost, / weight_norm.
+//   MODE_POST: DepthToWeak samples (DPE.cu:2667-2690) and LocalRefine
+//     (DPE.cu:2776-2835), all over the pixel's selected views.
+template <int NT, int MODE>
+__global__ __launch_bounds__(256) void k_ncc_weighted(Dev d, int colour, int Qn, float *__restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int slot = blockIdx.y;
+    int x = 0, y = 0;
+    bool active = q < Qn && colour_pixel(d, colour, q, x, y);
+    const int p = x + y * d.W;
+    float4 pl = make_float4(0, 0, 0, 0);
+    uint32_t sel = 0;
+    float result = 0.0f;
+    const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
+    float wn = 1.0f;
+    if (active) {
+        if (MODE == MODE_REFINE) {
+            active = d.cand_pos[q] >= 0;
+            if (active) {
+                pl = d.rplane[(size_t)slot * Qn + q];
+                wn = d.wnorm[q];
+            }
+        } else {
+            const uint8_t fl = d.post_flags[p];
+            const bool dtw = slot < 61, lr = slot >= 61;
+            active = (dtw && (fl & 1)) || (lr && (fl & 2));
+            if (active) {
+                const dpe_camera &cam = d.cams[0];
+                const float4 op = d.post_op[p];
+                const float od = op.w;
+                const float bl = d.post_bl[p];
+                const float disp = d.post_disp[p];
+                wn = d.post_wn[p];
+                sel = d.selected[p];
+                float pd = od;
+                if (slot != 61) {
+                    const int k = slot < 61 ? slot - 30 : slot - 67;
+                    pd = cam.K[0] * bl / (disp + (float)k);
+                    if (pd < d.prm.depth_min || pd > d.prm.depth_max) {
+                        active = false;
+                        out[(size_t)slot * Qn + q] = slot < 61 ? 2.0f : __builtin_inff();
+                    }
+                }
+                pl = op;
+                pl.w = distance_to_origin(cam, x, y, pd, op);
+            }
+        }
+    }
+    unsigned long long hyps = 0, taps = 0;
+    if (active) {
+        float acc = 0.0f;
+        const bool geom = d.prm.geom_consistency != 0;
+        const float gf = d.prm.geom_factor;
+        if constexpr (NT > 0) {
+            Patch<NT> P;
+            load_patch<NT>(d, x, y, P);
+            for (int v = 1; v < d.N; ++v) {
+                const float wv = (float)vw[v - 1];
+                if (MODE == MODE_REFINE) {
+                    if (vw[v - 1] > 0) {
+                        acc += wv * ncc_patch<NT>(d, P, x, y, v, pl, taps);
+                        hyps++;
+                    }
+                } else if (isSet(sel, v - 1)) {
+                    const float c = ncc_patch<NT>(d, P, x, y, v, pl, taps);
+                    hyps++;
+                    if (slot <= 61) {
+                        float tc = c;
+                        if (geom) tc += gf * geom_cost(d, x, y, v, pl);
+                        acc += tc * wv;
+                    } else {
+                        acc += c * wv;
+                        if (geom) acc += gf * geom_cost(d, x, y, v, pl) * wv;
+                    }
+                }
+            }
+        } else {
+            for (int v = 1; v < d.N; ++v) {
+                const float wv = (float)vw[v - 1];
+                if (MODE == MODE_REFINE) {
+                    if (vw[v - 1] > 0) acc += wv * ncc_old_direct(d, x, y, v, pl, hyps, taps);
+                } else if (isSet(sel, v - 1)) {
+                    const float c = ncc_old_direct(d, x, y, v, pl, hyps, taps);
+                    if (slot <= 61) {
+                        float tc = c;
+                        if (geom) tc += gf * geom_cost(d, x, y, v, pl);
+                        acc += tc * wv;
+                    } else {
+                        acc += c * wv;
+                        if (geom) acc += gf * geom_cost(d, x, y, v, pl) * wv;
+                    }
+                }
+            }
+        }
+        acc /= wn;
+        if (MODE == MODE_POST && slot < 61) acc = fminf(2.0f, acc);
+        result = acc;
+        out[(size_t)slot * Qn + q] = result;
+    }
+    count_add(d.counters, hyps, taps);
+}
+
+// ------------------------------------------------------ random initialisation
+// RandomInitialization, DPE.cu:1039-1067 (plane part)
+__device__ float4 random_normal(const dpe_camera &cam, int x, int y, Rng &rng, float depth) {
+    float4 n;
+    float q1 = 1.0f, q2 = 1.0f, s = 2.0f;
+    while (s >= 1.0f) {
+        q1 = 2.0f * rng.uniform() - 1.0f;
+        q2 = 2.0f * rng.uniform() - 1.0f;
+        s = q1 * q1 + q2 * q2;
+    }
+    const float sq = sqrtf(1.0f - s);
+    n.x = 2.0f * q1 * sq;
+    n.y = 2.0f * q2 * sq;
+    n.z = 1.0f - 2.0f * s;
+    n.w = 0;
+    const float4 vd = view_direction(cam, x, y, depth);
+    const float dot = n.x * vd.x + n.y * vd.y + n.z * vd.z;
+    if (dot > 0.0f) {
+        n.x = -n.x;
+        n.y = -n.y;
+        n.z = -n.z;
+    }
+    normalize3(&n);
+    return n;
+}
+
+// GeneratePerturbedNormal, DPE.cu:393-428
+__device__ float4 perturbed_normal(const dpe_camera &cam, int x, int y, const float4 &normal, Rng &rng,
+                                   float perturbation) {
+    const float4 vd = view_direction(cam, x, y, 1.0f);
+    const float a1 = (rng.uniform() - 0.5f) * perturbation;
+    const float a2 = (rng.uniform() - 0.5f) * perturbation;
+    const float a3 = (rng.uniform() - 0.5f) * perturbation;
+    float s1, c1, s2, c2, s3, c3;
+    d_sincosf(a1, &s1, &c1);
+    d_sincosf(a2, &s2, &c2);
+    d_sincosf(a3, &s3, &c3);
+    float R[9];
+    R[0] = c2 * c3;
+    R[1] = c3 * s1 * s2 - c1 * s3;
+    R[2] = s1 * s3 + c1 * c3 * s2;
+    R[3] = c2 * s3;
+    R[4] = c1 * c3 + s1 * s2 * s3;
+    R[5] = c1 * s2 * s3 - c3 * s1;
+    R[6] = -s2;
+    R[7] = c2 * s1;
+    R[8] = c1 * c2;
+    float4 np;
+    np.x = R[0] * normal.x + R[1] * normal.y + R[2] * normal.z;
+    np.y = R[3] * normal.x + R[4] * normal.y + R[5] * normal.z;
+    np.z = R[6] * normal.x + R[7] * normal.y + R[8] * normal.z;
+    np.w = normal.w;
+    if (np.x * vd.x + np.y * vd.y + np.z * vd.z >= 0.0f) np = normal;
+    normalize3(&np);
+    return np;
+}
+
+__global__ void k_random_init_planes(Dev d) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    const int x = p % d.W, y = p / d.W;
+    const dpe_camera &cam = d.cams[0];
+    if (d.prm.state == DPE_FIRST_INIT) {
+        Rng rng(d.prm.seed, p, PH_RANDOM_INIT, d.prm.call_index);
+        const float depth = rng.uniform() * (d.prm.depth_max - d.prm.depth_min) + d.prm.depth_min;
+        float4 pl = random_normal(cam, x, y, rng, depth);
+        pl.w = distance_to_origin(cam, x, y, depth, pl);
+        d.planes[p] = pl;
+    } else {
+        float4 pl = to_camera(cam, d.planes[p]);
+        const float depth = pl.w;
+        pl.w = distance_to_origin(cam, x, y, depth, pl);
+        d.planes[p] = pl;
+    }
+}
+
+// ComputeMultiViewInitialCostandSelectedViews (DPE.cu:784-830) /
+// ComputeMultiViewInitialCost (DPE.cu:832-861) from the slot-0 cost matrix.
+__global__ void k_random_init_costs(Dev d) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    const int NV = d.NV;
+    const float cost_max = 2.0f;
+    const float *cm = d.cost_mat;  // [NV][P]
+    if (d.prm.state == DPE_FIRST_INIT) {
+        float cv[32];
+        int num_valid = 0;
+        for (int i = 0; i < NV; ++i) {
+            const float c = cm[(size_t)i * d.P + p];
+            cv[i] = c;
+            if (c < cost_max) num_valid++;
+        }
+        int j;
+        for (int i = 1; i < NV; i++) {  // sort_small, DPE.cu:9-18
+            const float tmp = cv[i];
+            for (j = i; j >= 1 && tmp < cv[j - 1]; j--) cv[j] = cv[j - 1];
+            cv[j] = tmp;
+        }
+        uint32_t sel = 0;
+        const int top_k = min(num_valid, d.prm.top_k);
+        float cost = cost_max;
+        if (top_k > 0) {
+            float sum = 0.0f;
+            for (int i = 0; i < top_k; ++i) sum += cv[i];
+            const float thr = cv[top_k - 1];
+            for (int i = 0; i < NV; ++i)
+                if (cm[(size_t)i * d.P + p] <= thr) sel |= (1u << i);
+            cost = sum / top_k;
+        }
+        d.selected[p] = sel;
+        d.costs[p] = cost;
+    } else {
+        uint32_t sel = d.selected[p];
+        int count = 0;
+        float cost = 0.0f;
+        for (int i = 1; i < d.N; ++i) {
+            if (isSet(sel, i - 1)) {
+                const float c = cm[(size_t)(i - 1) * d.P + p];
+                if (c < cost_max) {
+                    count++;
+                    cost += c;
+                } else {
+                    sel &= (uint32_t)(0xFFFFFFFEu << (i - 1));  // unSetBit, DPE.cu:81-84
+                }
+            }
+        }
+        d.selected[p] = sel;
+        d.costs[p] = count == 0 ? cost_max : cost / count;
+    }
+}
+
+// ------------------------------------------------- strong half-iteration
+// Candidate search of CheckerboardPropagationStrong (DPE.cu:1239-1549).
+// Slots: 0 current plane, 1+d adaptive/near-far candidate d, 9+d fixed-step
+// candidate d (edge branch).  Also snapshots the four neighbours' selected
+// views for the view-selection priors (DPE.cu:1558-1570).
+__global__ void k_strong_candidates(Dev d, int colour, int iter) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.Q) return;
+    const int Q = d.Q;
+    const int nslot = d.prm.use_edge ? kStrongSlots : 9;
+    int *cp = d.cand_pos + q;
+    int x, y;
+    const bool valid = colour_pixel(d, colour, q, x, y);
+    const int W = d.W, H = d.H;
+    const int p = x + y * W;
+    if (!valid || d.weak[p] == DPE_WEAK) {
+        for (int s = 0; s < nslot; ++s) cp[(size_t)s * Q] = -1;
+        return;
+    }
+    cp[0] = p;
+    const float *costs = d.costs;
+    if (d.prm.use_edge) {
+        const int dirx[8] = {0, 0, -1, 1, -1, 1, -1, 1};
+        const int diry[8] = {-1, 1, 0, 0, -1, 1, 1, -1};
+        const short2 *en = d.edge_neigh + (size_t)p * 8;
+        const bool on_edge = d.edge[p] != 0;
+        const float max_edge_dist = max(H, W) / 30.0f;
+        const int min_step_len = 2;
+        for (int dd = 0; dd < 8; ++dd) {
+            const int dx = dirx[dd], dy = diry[dd];
+            const int sx = max(1, 5 - 2 * iter) * dx;
+            const int sy = max(1, 5 - 2 * iter) * dy;
+            const short2 ep = en[dd];
+            float dist = (float)sqrt((double)(ep.x - x) * (double)(ep.x - x) + (double)(ep.y - y) * (double)(ep.y - y));
+            if (dd >= 4) dist = (float)(dist / sqrt(2.0));
+            if (on_edge) {
+                dist = 11 * min_step_len;
+            } else if (ep.x == -1 || ep.y == -1 || dist > max_edge_dist) {
+                dist = max_edge_dist;
+                if (dd >= 4) dist = (float)(dist / sqrt(2.0));
+            }
+            const int step_num = min(max(11, (int)(1.0f * dist / min_step_len)), 22);
+            int step_len = max((int)(1.0f * dist / step_num), min_step_len);
+            if (dd < 4 && step_len % 2 == 1) step_len -= 1;
+            int fx = 0, fy = 0;
+            if (dd > 4) {
+                if (dd % 2) fx = dx; else fy = dy;
+            }
+            int best = -1;
+            float min_cost = 3.402823466e+38f;
+            for (int step = 0; step < step_num; ++step) {
+                const int tx = x + sx + step * step_len * dx + fx;
+                const int ty = y + sy + step * step_len * dy + fy;
+                if (!(tx >= 0 && ty >= 0 && tx < W && ty < H)) continue;
+                const int tc = tx + ty * W;
+                if (min_cost > costs[tc]) {
+                    best = tc;
+                    min_cost = costs[tc];
+                }
+            }
+            cp[(size_t)(1 + dd) * Q] = min_cost < 3.402823466e+38f ? best : -1;
+            int bestf = -1;
+            if (!on_edge) {
+                float mc = 3.402823466e+38f;
+                for (int step = 0; step < 11; ++step) {
+                    const int tx = x + sx + step * min_step_len * dx + fx;
+                    const int ty = y + sy + step * min_step_len * dy + fy;
+                    if (!(tx >= 0 && ty >= 0 && tx < W && ty < H)) continue;
+                    const int tc = tx + ty * W;
+                    if (mc > costs[tc]) {
+                        bestf = tc;
+                        mc = costs[tc];
+                    }
+                }
+                if (!(mc < 3.402823466e+38f)) bestf = -1;
+            }
+            cp[(size_t)(9 + dd) * Q] = bestf;
+        }
+    } else {
+        int pos[8];
+        for (int i = 0; i < 8; ++i) pos[i] = -1;
+        float costMin;
+        int costMinPoint;
+        const int center = p;
+        if (y > 2) {
+            const int up_far = center - 3 * W;
+            costMin = costs[up_far];
+            costMinPoint = up_far;
+            for (int i = 1; i < 11; ++i)
+                if (y > 2 + 2 * i) {
+                    const int pt = up_far - 2 * i * W;
+                    if (costs[pt] < costMin) { costMin = costs[pt]; costMinPoint = pt; }
+                }
+            pos[1] = costMinPoint;
+        }
+        if (y < H - 3) {
+            const int down_far = center + 3 * W;
+            costMin = costs[down_far];
+            costMinPoint = down_far;
+            for (int i = 1; i < 11; ++i)
+                if (y < H - 3 - 2 * i) {
+                    const int pt = down_far + 2 * i * W;
+                    if (costs[pt] < costMin) { costMin = costs[pt]; costMinPoint = pt; }
+                }
+            pos[3] = costMinPoint;
+        }
+        if (x > 2) {
+            const int left_far = center - 3;
+            costMin = costs[left_far];
+            costMinPoint = left_far;
+            for (int i = 1; i < 11; ++i)
+                if (x > 2 + 2 * i) {
+                    const int pt = left_far - 2 * i;
+                    if (costs[pt] < costMin) { costMin = costs[pt]; costMinPoint = pt; }
+                }
+            pos[5] = costMinPoint;
+        }
+        if (x < W - 3) {
+            const int right_far = center + 3;
+            costMin = costs[right_far];
+            costMinPoint = right_far;
+            for (int i = 1; i < 11; ++i)
+                if (x < W - 3 - 2 * i) {
+                    const int pt = right_far + 2 * i;
+                    if (costs[pt] < costMin) { costMin = costs[pt]; costMinPoint = pt; }
+                }
+            pos[7] = costMinPoint;
+        }
+        if (y > 0) {
+            const int up_near = center - W;
+            costMin = costs[up_near];
+            costMinPoint = up_near;
+            for (int i = 0; i < 3; ++i) {
+                if (y > 1 + i && x > i) {
+                    const int pt = up_near - (1 + i) * W - (1 + i);
+                    if (costs[pt] < costMin) { costMin = costs[pt]; costMinPoint = pt; }
+                }
+                if (y > 1 + i && x < W - 1 - i) {
+                    const int pt = up_near - (1 + i) * W + (1 + i);
+                    if (costs[pt] < costMin) { costMin = costs[pt]; costMinPoint = pt; }
+                }
+            }
+            pos[0] = costMinPoint;
+        }
+        if (y < H - 1) {
+            const int down_near = center + W;
+            costMin = costs[down_near];
+            costMinPoint = down_near;
+            for (int i = 0; i < 3; ++i) {
+                if (y < H - 2 - i && x > i) {
+                    const int pt = down_near + (1 + i) * W - (1 + i);
+                    if (costs[pt] < costMin) { costMin = costs[pt]; costMinPoint = pt; }
+                }
+                if (y < H - 2 - i && x < W - 1 - i) {
+                    const int pt = down_near + (1 + i) * W + (1 + i);
+                    if (costs[pt] < costMin) { costMin = costs[pt]; costMinPoint = pt; }
+                }
+            }
+            pos[2] = costMinPoint;
+        }
+        if (x > 0) {
+            const int left_near = center - 1;
+            costMin = costs[left_near];
+            costMinPoint = left_near;
+            for (int i = 0; i < 3; ++i) {
+                if (x > 1 + i && y > i) {
+                    const int pt = left_near - (1 + i) - (1 + i) * W;
+                    if (costs[pt] < costMin) { costMin = costs[pt]; costMinPoint = pt; }
+                }
+                if (x > 1 + i && y < H - 1 - i) {
+                    const int pt = left_near - (1 + i) + (1 + i) * W;
+                    if (costs[pt] < costMin) { costMin = costs[pt]; costMinPoint = pt; }
+                }
+            }
+            pos[4] = costMinPoint;
+        }
+        if (x < W - 1) {
+            const int right_near = center + 1;
+            costMin = costs[right_near];
+            costMinPoint = right_near;
+            for (int i = 0; i < 3; ++i) {
+                if (x < W - 2 - i && y > i) {
+                    const int pt = right_near + (1 + i) - (1 + i) * W;
+                    if (costs[pt] < costMin) { costMin = costs[pt]; costMinPoint = pt; }
+                }
+                if (x < W - 2 - i && y < H - 1 - i) {
+                    const int pt = right_near + (1 + i) + (1 + i) * W;
+                    if (costs[pt] < costMin) { costMin = costs[pt]; costMinPoint = pt; }
+                }
+            }
+            pos[6] = costMinPoint;
+        }
+        for (int i = 0; i < 8; ++i) cp[(size_t)(1 + i) * Q] = pos[i];
+    }
+    const long nb[4] = {(long)p - W, (long)p + W, (long)p - 1, (long)p + 1};
+    for (int i = 0; i < 4; ++i) d.nsel[(size_t)i * Q + q] = (nb[i] >= 0 && nb[i] < (long)d.P) ? d.selected[nb[i]] : 0u;
+}
+
+// Multi-hypothesis joint view selection (DPE.cu:1551-1631), shared by the
+// strong select kernel and the weak propagation.  `cost(i, v)` reads row i of
+// the 8 x NV candidate cost matrix.
+template <class CostFn>
+__device__ float joint_view_selection(const Dev &d, int iter, CostFn cost, const float *priors, Rng &rng,
+                                      uint8_t *vwl, uint32_t *tsel_out) {
+    const int NV = d.NV;
+    float probs[32];
+    const float thr = (float)(0.8 * (double)d_expf((float)(iter * iter) / (-90.0f)));
+    for (int i = 0; i < NV; i++) {
+        float count = 0;
+        int count_false = 0;
+        float tmpw = 0;
+        for (int j = 0; j < 8; j++) {
+            const float c = cost(j, i);
+            if (c < thr) {
+                tmpw += d_expf(c * c / (-0.18f));
+                count++;
+            }
+            if (c > 1.2f) count_false++;
+        }
+        float pr = 0.0f;
+        if (count > 2 && count_false < 3) pr = tmpw / count;
+        else if (count_false < 3) pr = d_expf(thr * thr / (-0.32f));
+        probs[i] = pr * priors[i];
+    }
+    float sum = 0.0f;  // TransformPDFToCDF, DPE.cu:297-311
+    for (int i = 0; i < NV; ++i) sum += probs[i];
+    const float inv = 1.0f / sum;
+    float cum = 0.0f;
+    for (int i = 0; i < NV; ++i) {
+        const float pr = probs[i] * inv;
+        cum += pr;
+        probs[i] = cum;
+    }
+    for (int i = 0; i < kMaxImages; ++i) vwl[i] = 0;
+    for (int s = 0; s < 15; ++s) {
+        const float rp = rng.uniform() - 1.19209290e-07f;
+        for (int id = 0; id < NV; ++id) {
+            if (probs[id] > rp) {
+                vwl[id] += 1;
+                break;
+            }
+        }
+    }
+    uint32_t tsel = 0;
+    float wn = 0;
+    for (int i = 0; i < NV; ++i) {
+        if (vwl[i] > 0) {
+            tsel |= (1u << i);
+            wn += vwl[i];
+        }
+    }
+    *tsel_out = tsel;
+    return wn;
+}
+
+// View selection, candidate adoption and refinement hypotheses of
+// CheckerboardPropagationStrong (DPE.cu:1551-1658, 1069-1099).
+__global__ void k_strong_select(Dev d, int colour, int iter) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.Q) return;
+    const int Q = d.Q;
+    if (d.cand_pos[q] < 0) return;
+    int x, y;
+    colour_pixel(d, colour, q, x, y);
+    const int p = x + y * d.W;
+    const int NV = d.NV;
+    const dpe_camera &cam = d.cams[0];
+    const float *cm = d.cost_mat;
+    // chosen slot of every candidate row (-1: no candidate, the row stays {2,0,..} / 0)
+    int row_slot[8];
+    for (int i = 0; i < 8; ++i) row_slot[i] = d.cand_pos[(size_t)(1 + i) * Q + q] >= 0 ? 1 + i : -1;
+    if (d.prm.use_edge && !d.edge[p]) {
+        const float good_thr = 0.8f * d_expf((float)(iter * iter) / (-90.0f));
+        const float bad_thr = 1.2f;
+        for (int i = 0; i < 8; ++i) {
+            if (d.cand_pos[(size_t)(9 + i) * Q + q] < 0) continue;
+            const bool had = row_slot[i] >= 0;
+            int good0 = 0, bad0 = 0, good1 = 0, bad1 = 0;
+            for (int j = 0; j < NV; ++j) {
+                const float v0 = had ? cm[((size_t)row_slot[i] * NV + j) * Q + q] : ((i == 0 && j == 0) ? 2.0f : 0.0f);
+                const float v1 = cm[((size_t)(9 + i) * NV + j) * Q + q];
+                if (v0 < good_thr) good0++;
+                if (v0 > bad_thr) bad0++;
+                if (v1 < good_thr) good1++;
+                if (v1 > bad_thr) bad1++;
+            }
+            if (!had || good1 > good0 || (good1 == good0 && bad1 < bad0)) row_slot[i] = 9 + i;
+        }
+    }
+    auto cost = [&](int i, int v) -> float {
+        const int s = row_slot[i];
+        if (s < 0) return (i == 0 && v == 0) ? 2.0f : 0.0f;
+        return cm[((size_t)s * NV + v) * Q + q];
+    };
+    float priors[32];
+    for (int j = 0; j < NV; ++j) priors[j] = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        if (row_slot[2 * i] >= 0) {
+            const uint32_t sv = d.nsel[(size_t)i * Q + q];
+            for (int j = 0; j < NV; ++j) priors[j] += isSet(sv, j) == 1 ? 0.9f : 0.1f;
+        }
+    }
+    Rng rng(d.prm.seed, p, ph_strong(iter), d.prm.call_index);
+    uint8_t vwl[kMaxImages];
+    uint32_t tsel = 0;
+    const float wn = joint_view_selection(d, iter, cost, priors, rng, vwl, &tsel);
+    float final_costs[8];
+    for (int i = 0; i < 8; ++i) {
+        float fc = 0.0f;
+        for (int j = 0; j < NV; ++j)
+            if (vwl[j] > 0) fc += vwl[j] * cost(i, j);
+        final_costs[i] = fc / wn;
+    }
+    int mi = 0;  // FindMinCostIndex, DPE.cu:50-61
+    {
+        float mc = final_costs[0];
+        for (int i = 1; i < 8; ++i)
+            if (final_costs[i] <= mc) { mc = final_costs[i]; mi = i; }
+    }
+    float cost_now = 0.0f;
+    for (int i = 0; i < NV; ++i) cost_now += vwl[i] * cm[((size_t)0 * NV + i) * Q + q];
+    cost_now /= wn;
+    d.costs[p] = cost_now;
+    const float4 cur = d.planes[p];
+    float depth_now = depth_from_plane(cam, cur, x, y);
+    float4 plane_now = cur;
+    if (row_slot[mi] >= 0) {
+        const float4 cand = d.planes[d.cand_pos[(size_t)row_slot[mi] * Q + q]];
+        const float db = depth_from_plane(cam, cand, x, y);
+        if (db >= d.prm.depth_min && db <= d.prm.depth_max && final_costs[mi] < cost_now) {
+            depth_now = db;
+            plane_now = cand;
+            cost_now = final_costs[mi];
+            d.selected[p] = tsel;
+        }
+    }
+    uint8_t *vwo = d.vw + (size_t)p * kMaxImages;
+    for (int i = 0; i < kMaxImages; ++i) vwo[i] = vwl[i];
+    // hypotheses of PlaneHypothesisRefinementStrong (DPE.cu:1080-1104)
+    const float dmin = d.prm.depth_min, dmax = d.prm.depth_max;
+    const float depth_rand = rng.uniform() * (dmax - dmin) + dmin;
+    const float4 plane_rand = random_normal(cam, x, y, rng, depth_now);
+    const float dminp = (1 - 0.02f) * depth_now;
+    const float dmaxp = (1 + 0.02f) * depth_now;
+    const float depth_pert = rng.uniform() * (dmaxp - dminp) + dminp;
+    const float4 plane_pert = perturbed_normal(cam, x, y, plane_now, rng, (float)(0.02f * 3.14159265358979323846));
+    const float depths[5] = {depth_rand, depth_now, depth_rand, depth_now, depth_pert};
+    const float4 normals[5] = {plane_now, plane_rand, plane_rand, plane_pert, plane_now};
+    for (int i = 0; i < 5; ++i) {
+        float4 tp = normals[i];
+        tp.w = distance_to_origin(cam, x, y, depths[i], tp);
+        d.rplane[(size_t)i * Q + q] = tp;
+    }
+    d.now_plane[q] = plane_now;
+    d.now_depth[q] = depth_now;
+    d.now_cost[q] = cost_now;
+    d.wnorm[q] = wn;
+}
+
+// Acceptance of the refinement hypotheses and the RunState write-back
+// (DPE.cu:1115-1121, 1660-1669).
+__global__ void k_strong_accept(Dev d, int colour) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.Q) return;
+    if (d.cand_pos[q] < 0) return;
+    const int Q = d.Q;
+    int x, y;
+    colour_pixel(d, colour, q, x, y);
+    const int p = x + y * d.W;
+    const dpe_camera &cam = d.cams[0];
+    float4 plane = d.now_plane[q];
+    float depth = d.now_depth[q];
+    float cost = d.now_cost[q];
+    for (int i = 0; i < 5; ++i) {
+        const float tc = d.rcost[(size_t)i * Q + q];
+        const float4 tp = d.rplane[(size_t)i * Q + q];
+        const float db = depth_from_plane(cam, tp, x, y);
+        if (db >= d.prm.depth_min && db <= d.prm.depth_max && tc < cost) {
+            depth = db;
+            plane = tp;
+            cost = tc;
+        }
+    }
+    if (d.prm.state == DPE_REFINE_INIT) {
+        if ((double)cost < (double)d.costs[p] - 0.1) {
+            d.costs[p] = cost;
+            d.planes[p] = plane;
+        }
+    } else {
+        d.costs[p] = cost;
+        d.planes[p] = plane;
+    }
+}
+
+// ------------------------------------------------------ weak-texture path
+__device__ __forceinline__ uint8_t edge_low_at(const Dev &d, long idx) {
+    if (idx < 0 || idx >= (long)d.low_w * d.low_h) return 0;
+    return d.edge_low[idx];
+}
+
+// BresenhamLine, DPE.cu:162-248 (reads outside the low-res map return 0)
+__device__ bool bresenham(const Dev &d, int Ax, int Ay, int Bx, int By) {
+    if (d.edge[Ax + Ay * d.W] || d.edge[Bx + By * d.W]) return false;
+    const float scale_x = 1.0f * d.low_w / (float)d.W;
+    const float scale_y = 1.0f * d.low_h / (float)d.H;
+    const int height = d.low_h, width = d.low_w;
+    const int max_step = d.prm.high_res_img ? (int)round(max(height, width) / 60.0) : max(height, width);
+    for (int pass = 0; pass < 2; ++pass) {
+        const int fx = pass == 0 ? Bx : Ax, fy = pass == 0 ? By : Ay;
+        const int tx = pass == 0 ? Ax : Bx, ty = pass == 0 ? Ay : By;
+        int x0 = min((int)roundf(fx * scale_x), width - 1);
+        int y0 = min((int)roundf(fy * scale_y), height - 1);
+        const int x1 = min((int)roundf(tx * scale_x), width - 1);
+        const int y1 = min((int)roundf(ty * scale_y), height - 1);
+        const int dx = abs(x1 - x0), sx = x0 < x1 ? 1 : -1;
+        const int dy = abs(y1 - y0), sy = y0 < y1 ? 1 : -1;
+        int erro = (dx > dy ? dx : dy) / 2;
+        int step = 0;
+        bool tagx = true, tagy = true;
+        while (tagx || tagy) {
+            if (x0 == x1) tagx = false;
+            if (y0 == y1) tagy = false;
+            const int e2 = erro;
+            if (e2 > -dx) {
+                erro -= dy;
+                x0 += sx;
+            }
+            if (e2 < dy) {
+                erro += dx;
+                y0 += sy;
+            }
+            if (edge_low_at(d, (long)x0 + (long)y0 * width)) return true;
+            step += 1;
+            if (step >= max_step) break;
+        }
+    }
+    return false;
+}
+
+// PointinTriangle, DPE.cu:139-160
+__device__ bool point_in_triangle(short2 A, short2 B, short2 C, int px, int py) {
+    const float abx = (float)(B.x - A.x), aby = (float)(B.y - A.y);
+    const float bcx = (float)(C.x - B.x), bcy = (float)(C.y - B.y);
+    const float cax = (float)(A.x - C.x), cay = (float)(A.y - C.y);
+    const float ab = sqrtf(abx * abx + aby * aby);
+    const float bc = sqrtf(bcx * bcx + bcy * bcy);
+    const float ca = sqrtf(cax * cax + cay * cay);
+    if (ab <= 2 || bc <= 2 || ca <= 2) return false;
+    if (!(ab + bc > ca && bc + ca > ab && ab + ca > bc)) return false;
+    const float pax = (float)(A.x - px), pay = (float)(A.y - py);
+    const float pbx = (float)(B.x - px), pby = (float)(B.y - py);
+    const float pcx = (float)(C.x - px), pcy = (float)(C.y - py);
+    const float t1 = pax * pby - pay * pbx;
+    const float t2 = pbx * pcy - pby * pcx;
+    const float t3 = pcx * pay - pcy * pax;
+    return t1 * t2 >= 0 && t1 * t3 >= 0;
+}
+
+// GenEdgeInform, DPE.cu:2487-2595
+__global__ void k_gen_edge_inform(Dev d) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    const int W = d.W, H = d.H;
+    const int x = p % W, y = p / W;
+    const int dirx[8] = {0, 0, -1, 1, -1, 1, -1, 1};
+    const int diry[8] = {-1, 1, 0, 0, -1, 1, 1, -1};
+    if (d.prm.use_edge) {
+        short2 *en = d.edge_neigh + (size_t)p * 8;
+        for (int i = 0; i < 8; i++) {
+            short2 r = make_short2(-1, -1);
+            int nx = x + dirx[i], ny = y + diry[i];
+            while (true) {
+                if (nx < 0 || nx >= W || ny < 0 || ny >= H) break;
+                if (d.edge[nx + ny * W]) {
+                    r = make_short2((short)nx, (short)ny);
+                    break;
+                }
+                nx += dirx[i];
+                ny += diry[i];
+            }
+            en[i] = r;
+        }
+        const int radius = d.prm.strong_radius;
+        int edge_pix = 0, tot_pix = 0;
+        for (int i = -radius; i <= radius; i++)
+            for (int j = -radius; j <= radius; j++) {
+                const int nx = x + i, ny = y + j;
+                if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
+                if (d.edge[ny * W + nx]) edge_pix++;
+                tot_pix++;
+            }
+        float density = 1.0f * edge_pix / tot_pix;
+        if (d.prm.use_label) {
+            int bound_pix = 0;
+            for (int i = -radius; i <= radius; i++)
+                for (int j = -radius; j <= radius; j++) {
+                    const int nx = x + i, ny = y + j;
+                    if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
+                    if (d.label[ny * W + nx] == 0) bound_pix++;
+                }
+            density = fmaxf(density, (float)(bound_pix / tot_pix));
+        }
+        d.complexity[p] = (float)(1.0f / (1.0f + d_exp(-25.0 * ((double)density - 0.35))));
+    }
+    if (d.prm.use_label && d.weak[p] == DPE_WEAK) {
+        short2 *lb = d.label_bound + (size_t)d.nmap[p] * 8;
+        const int cl = d.label[p];
+        if (cl > 0) {
+            for (int i = 0; i < 8; i++) {
+                int nx = x + dirx[i], ny = y + diry[i];
+                int lx = -1, ly = -1;
+                while (true) {
+                    if (nx < 0 || nx >= W || ny < 0 || ny >= H) break;
+                    const int nl = d.label[nx + ny * W];
+                    if (nl == cl) {
+                        lx = nx;
+                        ly = ny;
+                    } else if (nl == -1) {
+                        break;
+                    }
+                    nx += dirx[i];
+                    ny += diry[i];
+                }
+                lb[i] = make_short2((short)lx, (short)ly);
+            }
+        }
+    }
+}
+
+// FindNearestStrongPoint, DPE.cu:2859-2893
+__global__ void k_find_nearest_strong(Dev d) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    const int W = d.W, H = d.H;
+    const int x = p % W, y = p / W;
+    d.nearest[p] = make_short2(-1, -1);
+    if (d.weak[p] != DPE_WEAK) return;
+    for (int r = 0; r <= 100; ++r)
+        for (int i = -r; i <= r; ++i)
+            for (int j = -r; j <= r; ++j) {
+                if (abs(i) != r && abs(j) != r) continue;
+                const int nx = x + i, ny = y + j;
+                if (nx < 0 || ny < 0 || nx >= W || ny >= H) continue;
+                if (d.weak[nx + ny * W] == DPE_STRONG) {
+                    d.nearest[p] = make_short2((short)nx, (short)ny);
+                    return;
+                }
+            }
+}
+
+// GenNeighbours, DPE.cu:2107-2467
+__global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    if (d.weak[p] != DPE_WEAK) return;
+    const int W = d.W, H = d.H;
+    const int px = p % W, py = p / W;
+    constexpr int max_pt_num = 64;
+    const int min_margin = 6;
+    const float depth_diff = d.prm.depth_max - d.prm.depth_min;
+    const dpe_camera &cam = d.cams[0];
+    short2 *neighbours = d.neighbours + (size_t)d.nmap[p] * kNeighbourNum;
+    Rng rng(d.prm.seed, p, PH_GEN_NEIGHBOURS, d.prm.call_index);
+    for (int i = 0; i < kNeighbourNum; ++i) neighbours[i] = make_short2(-1, -1);
+    neighbours[0] = make_short2((short)px, (short)py);
+    short2 strong_points[max_pt_num];
+    bool dir_valid[max_pt_num];
+    for (int i = 0; i < max_pt_num; ++i) {
+        strong_points[i] = make_short2(-1, -1);
+        dir_valid[i] = false;
+    }
+    int odi = -1;
+    int strong_point_size = 0;
+    const int rotate_time = d.prm.rotate_time;
+    const float ransac_threshold = d.prm.ransac_threshold * depth_diff;
+    bool edge_limit = false;
+    if (d.prm.use_limit) {
+        edge_limit = true;
+        if (d.prm.use_edge) {
+            const float cv = d.complexity[p];
+            const float rp = rng.uniform() - 1.19209290e-07f;
+            if (rp < cv) edge_limit = false;
+            else d.complexity[p] = fmaxf(0.99f, cv);
+        }
+    }
+    for (int odx = -1; odx <= 1; ++odx) {
+        for (int ody = -1; ody <= 1; ++ody) {
+            if (odx == 0 && ody == 0) continue;
+            float2 od = make_float2((float)odx, (float)ody);
+            normalize2(&od);
+            odi++;
+            for (int rot = 0; rot < rotate_time; ++rot) {
+                const int di = odi * 4 + rot;
+                for (int radius = 2; radius <= DPE_MAX_SEARCH_RADIUS; radius = min(radius * 2, radius + 25)) {
+                    const float tx = (float)px + od.x * (float)radius;
+                    const float ty = (float)py + od.y * (float)radius;
+                    if (tx < 0 || ty < 0 || tx >= (float)W || ty >= (float)H) break;
+                    for (int ri = 0; ri < 4; ++ri) {
+                        const uint32_t s1 = rng.next();
+                        const uint32_t m1 = rng.next();
+                        const int rxs = (int)(((s1 % 2 == 0) ? 1u : 0xFFFFFFFFu) * m1 % (uint32_t)d.gn_shift);
+                        const uint32_t s2 = rng.next();
+                        const uint32_t m2 = rng.next();
+                        const int rys = (int)(((s2 % 2 == 0) ? 1u : 0xFFFFFFFFu) * m2 % (uint32_t)d.gn_shift);
+                        float2 dv = make_float2(od.x * 20 + (float)rxs, od.y * 20 + (float)rys);
+                        normalize2(&dv);
+                        short2 np = make_short2((short)(int)((float)px + dv.x * (float)radius),
+                                                (short)(int)((float)py + dv.y * (float)radius));
+                        if (np.x < min_margin || np.y < min_margin || np.x >= W - min_margin || np.y >= H - min_margin)
+                            continue;
+                        int nc = np.x + np.y * W;
+                        if (d.weak[nc] != DPE_STRONG) {
+                            np = d.nearest[nc];
+                            if (np.x == -1 || np.y == -1) continue;
+                            nc = np.x + np.y * W;
+                        }
+                        float2 td = make_float2((float)(np.x - px), (float)(np.y - py));
+                        normalize2(&td);
+                        const float ca = td.x * od.x + td.y * od.y;
+                        if (ca > d.gn_thr && (!edge_limit || !bresenham(d, px, py, np.x, np.y))) {
+                            strong_points[di] = np;
+                            dir_valid[di] = true;
+                            strong_point_size++;
+                            break;
+                        }
+                    }
+                    if (dir_valid[di]) break;
+                }
+                float2 rd;
+                rd.x = od.x * d.gn_cos - od.y * d.gn_sin;
+                rd.y = od.x * d.gn_sin + od.y * d.gn_cos;
+                normalize2(&rd);
+                od = rd;
+            }
+        }
+    }
+    int extend_index = 31;
+    if (d.prm.use_label && d.label[p] > 0) {
+        const int dirx[8] = {0, 0, -1, 1, -1, 1, -1, 1};
+        const int diry[8] = {-1, 1, 0, 0, -1, 1, 1, -1};
+        const short2 *lb = d.label_bound + (size_t)d.nmap[p] * 8;
+        float bound_dist[8];
+        int dir_step[8];
+        for (int i = 0; i < 8; ++i) {
+            bound_dist[i] = 0;
+            dir_step[i] = 0;
+        }
+        for (int i = 0; i < 8; ++i) {
+            const short2 bp = lb[i];
+            float dist = 0.0f;
+            if (bp.x != -1 && bp.y != -1) {
+                dist = (float)sqrt((double)(px - bp.x) * (double)(px - bp.x) + (double)(py - bp.y) * (double)(py - bp.y));
+                if (i >= 4) dist = (float)(dist / sqrt(2.0));
+            }
+            bound_dist[i] = dist;
+            if (i % 2 == 1) {
+                const float opposite = bound_dist[i - 1];
+                const float ratio = 2 * rotate_time * dist / (dist + opposite);
+                const int rint_ = (ratio != ratio) ? 0 : f2i_clamp(ratio, -1000000, 1000000);
+                const int step = min(1, max(2 * rotate_time - 1, rint_));
+                dir_step[i - 1] = 2 * rotate_time - step;
+                dir_step[i] = step;
+            }
+        }
+        for (int i = 0; i < 8; ++i) {
+            const float dist = bound_dist[i];
+            const int gap = dir_step[i] + 1;
+            const int step_len = max(1, (int)floor(1.0 * dist / gap));
+            for (int step = 1; step <= dir_step[i]; ++step) {
+                short2 np = make_short2((short)(px + step * step_len * dirx[i]), (short)(py + step * step_len * diry[i]));
+                if (np.x < min_margin || np.y < min_margin || np.x >= W - min_margin || np.y >= H - min_margin) continue;
+                int nc = np.x + np.y * W;
+                if (d.weak[nc] != DPE_STRONG) {
+                    np = d.nearest[nc];
+                    if (np.x == -1 || np.y == -1) continue;
+                    nc = np.x + np.y * W;
+                }
+                if (d.label[nc] != 0 && d.label[nc] != d.label[p]) continue;
+                if (extend_index >= max_pt_num - 1) continue;
+                extend_index++;
+                strong_points[extend_index] = np;
+                dir_valid[extend_index] = true;
+                strong_point_size++;
+            }
+        }
+    }
+    if (strong_point_size <= 3) {
+        d.weak_rel[p] = 0;
+        return;
+    }
+    float4 best_plane = make_float4(0, 0, 0, 0);
+    bool has_valid_plane = false;
+    short2 sv[max_pt_num];
+    float3 s3[max_pt_num];
+    float3 sn[max_pt_num];
+    int vc = 0;
+    float X[3];
+    get3d(cam, px, py, d.planes[p].w, X);
+    const float cpz = X[2];
+    for (int i = 0; i < max_pt_num; ++i) {
+        sv[i] = make_short2(-1, -1);
+        if (dir_valid[i]) {
+            const short2 sp = strong_points[i];
+            const int sc = sp.x + sp.y * W;
+            sv[vc] = sp;
+            get3d(cam, sp.x, sp.y, d.planes[sc].w, X);
+            s3[vc] = make_float3(X[0], X[1], X[2]);
+            const float4 n4 = to_camera(cam, d.planes[sc]);
+            sn[vc] = make_float3(n4.x, n4.y, n4.z);
+            vc++;
+        }
+    }
+    {
+        int iteration = 50, max_iter = d.prm.high_res_img ? 200 : 125, max_count = 3;
+        float min_cost = 3.402823466e+38f;
+        float residuals[max_pt_num];
+        for (int i = 0; i < max_pt_num; ++i) residuals[i] = 0.0f;
+        float temp_thr = ransac_threshold;
+        bool has_consist = false;
+        bool must_in_triangle = (d.prm.use_label && d.label[p] > 0 && edge_limit) ? false : true;
+        while (iteration > 0 && max_iter > 0) {
+            max_iter--;
+            const int a = (int)(rng.next() % (uint32_t)vc);
+            const int b = (int)(rng.next() % (uint32_t)vc);
+            const int c = (int)(rng.next() % (uint32_t)vc);
+            if (a == b || b == c || a == c) continue;
+            if (must_in_triangle && !point_in_triangle(sv[a], sv[b], sv[c], px, py)) continue;
+            if (edge_limit) {
+                if (bresenham(d, sv[a].x, sv[a].y, sv[b].x, sv[b].y) || bresenham(d, sv[b].x, sv[b].y, sv[c].x, sv[c].y) ||
+                    bresenham(d, sv[c].x, sv[c].y, sv[a].x, sv[a].y))
+                    continue;
+            }
+            bool normal_consistency = false;
+            if (d.prm.geom_consistency && edge_limit) {
+                normal_consistency = true;
+                const float ab = sn[a].x * sn[b].x + sn[a].y * sn[b].y + sn[a].z * sn[b].z;
+                const float ac = sn[a].x * sn[c].x + sn[a].y * sn[c].y + sn[a].z * sn[c].z;
+                const float bc = sn[b].x * sn[c].x + sn[b].y * sn[c].y + sn[b].z * sn[c].z;
+                if ((double)ab < 0.8660254 || (double)ac < 0.8660254 || (double)bc < 0.8660254) normal_consistency = false;
+                if (has_consist && !normal_consistency) continue;
+            }
+            iteration--;
+            const float3 A = s3[a], B = s3[b], Cc = s3[c];
+            const float3 AC = make_float3(A.x - Cc.x, A.y - Cc.y, A.z - Cc.z);
+            const float3 BC = make_float3(B.x - Cc.x, B.y - Cc.y, B.z - Cc.z);
+            float4 cr;
+            cr.x = AC.y * BC.z - BC.y * AC.z;
+            cr.y = -(AC.x * BC.z - BC.x * AC.z);
+            cr.z = AC.x * BC.y - BC.x * AC.y;
+            if ((cr.x == 0 && cr.y == 0 && cr.z == 0) || cr.x != cr.x || cr.y != cr.y || cr.z != cr.z) continue;
+            normalize3(&cr);
+            cr.w = -(cr.x * A.x + cr.y * A.y + cr.z * A.z);
+            int temp_count = 0;
+            float strong_dist = 0.0f;
+            for (int si = 0; si < vc; ++si) {
+                const float fx = ((float)sv[si].x - cam.K[2]) / cam.K[0];
+                const float fy = ((float)sv[si].y - cam.K[5]) / cam.K[4];
+                const float fit_depth = -cr.w / (cr.x * fx + cr.y * fy + cr.z);
+                const float distance = fabsf(fit_depth - s3[si].z);
+                residuals[si] = distance;
+                if (distance < temp_thr) {
+                    temp_count++;
+                    strong_dist += distance;
+                }
+            }
+            if (temp_count < 6) continue;
+            if (temp_count > max_count) {
+                if (!must_in_triangle && point_in_triangle(sv[a], sv[b], sv[c], px, py)) must_in_triangle = true;
+                if (!has_consist && normal_consistency) has_consist = true;
+                const float fx = ((float)px - cam.K[2]) / cam.K[0];
+                const float fy = ((float)py - cam.K[5]) / cam.K[4];
+                const float fit_depth = -cr.w / (cr.x * fx + cr.y * fy + cr.z);
+                const float cd = fabsf(fit_depth - cpz);
+                best_plane = cr;
+                max_count = temp_count;
+                strong_dist /= temp_count;
+                min_cost = cd;
+                has_valid_plane = true;
+                if ((double)temp_thr > (d.prm.high_res_img ? 0.05 : 0.005)) {
+                    int j;
+                    for (int i = 1; i < vc; i++) {
+                        const float tmp = residuals[i];
+                        for (j = i; j >= 1 && tmp < residuals[j - 1]; j--) residuals[j] = residuals[j - 1];
+                        residuals[j] = tmp;
+                    }
+                    if (temp_thr < residuals[kNeighbourNum]) continue;
+                    temp_thr = (float)((double)residuals[kNeighbourNum] - 1e-6);
+                    temp_count = 0;
+                    for (int i = 0; i < vc; ++i) {
+                        if (residuals[i] < temp_thr) temp_count++;
+                        else break;
+                    }
+                    max_count = temp_count;
+                }
+            } else if (temp_count == max_count) {
+                if (!must_in_triangle && point_in_triangle(sv[a], sv[b], sv[c], px, py)) must_in_triangle = true;
+                const float fx = ((float)px - cam.K[2]) / cam.K[0];
+                const float fy = ((float)py - cam.K[5]) / cam.K[4];
+                const float fit_depth = -cr.w / (cr.x * fx + cr.y * fy + cr.z);
+                const float cd = fabsf(fit_depth - cpz);
+                if (cd < min_cost) {
+                    best_plane = cr;
+                    max_count = temp_count;
+                    strong_dist /= temp_count;
+                    min_cost = cd;
+                }
+            }
+        }
+    }
+    if (!has_valid_plane) {
+        d.weak_rel[p] = 0;
+        return;
+    }
+    float weight[max_pt_num];
+    for (int i = 0; i < vc; ++i) {
+        const float fx = ((float)sv[i].x - cam.K[2]) / cam.K[0];
+        const float fy = ((float)sv[i].y - cam.K[5]) / cam.K[4];
+        const float fit_depth = -best_plane.w / (best_plane.x * fx + best_plane.y * fy + best_plane.z);
+        const float distance = fabsf(fit_depth - s3[i].z);
+        if (distance >= ransac_threshold) {
+            sv[i] = make_short2(-1, -1);
+            weight[i] = 3.402823466e+38f;
+            continue;
+        }
+        weight[i] = distance;
+    }
+    {
+        int j;  // sort_small_weighted, DPE.cu:20-33
+        for (int i = 1; i < vc; i++) {
+            const short2 tmp = sv[i];
+            const float tw = weight[i];
+            for (j = i; j >= 1 && tw < weight[j - 1]; j--) {
+                sv[j] = sv[j - 1];
+                weight[j] = weight[j - 1];
+            }
+            sv[j] = tmp;
+            weight[j] = tw;
+        }
+    }
+    for (int i = 1; i < kNeighbourNum; ++i) neighbours[i] = sv[i - 1];
+    d.weak_rel[p] = 1;
+}
+
+// NeigbourUpdate, DPE.cu:2469-2485
+__global__ void k_neighbour_update(Dev d) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    if (d.weak[p] == DPE_WEAK && d.weak_rel[p] != 1) d.weak[p] = DPE_UNKNOWN;
+}
+
+// RANSACToGetFitPlane, DPE.cu:2895-3128
+__global__ void k_ransac(Dev d, int iter) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    const int W = d.W;
+    const int px = p % W, py = p / W;
+    if (d.weak[p] != DPE_WEAK) {
+        d.fit[p] = d.planes[p];
+        return;
+    }
+    Rng rng(d.prm.seed, p, ph_ransac(iter), d.prm.call_index);
+    const dpe_camera &cam = d.cams[0];
+    bool edge_limit = false;
+    if (d.prm.use_limit) {
+        edge_limit = true;
+        if (d.prm.use_edge) {
+            const float cv = d.complexity[p];
+            const float rp = rng.uniform() - 1.19209290e-07f;
+            if (rp < cv) edge_limit = false;
+        }
+    }
+    short2 sp[kNeighbourNum - 1];
+    float3 s3[kNeighbourNum - 1];
+    float3 sn[kNeighbourNum - 1];
+    int count = 0;
+    float X[3];
+    for (int i = 1; i < kNeighbourNum; ++i) {
+        const short2 tp = neighbour_point(d, p, i);
+        if (tp.x == -1 || tp.y == -1) continue;
+        sp[count] = tp;
+        const int tc = tp.x + tp.y * W;
+        const float4 pl = d.planes[tc];
+        const float depth = depth_from_plane(cam, pl, tp.x, tp.y);
+        get3d(cam, tp.x, tp.y, depth, X);
+        s3[count] = make_float3(X[0], X[1], X[2]);
+        sn[count] = make_float3(pl.x, pl.y, pl.z);
+        count++;
+    }
+    if (count < 3) {
+        d.fit[p] = d.planes[p];
+        return;
+    }
+    int iteration = 50;
+    int ua = -1, ub = -1, uc = -1;
+    float min_cost = 3.402823466e+38f;
+    float4 best = make_float4(0, 0, 0, 0);
+    bool has_best = false, has_strong = false;
+    bool must_in_triangle = (d.prm.use_label && d.label[p] > 0 && edge_limit) ? false : true;
+    while (iteration--) {
+        const int a = (int)(rng.next() % (uint32_t)count);
+        const int b = (int)(rng.next() % (uint32_t)count);
+        const int c = (int)(rng.next() % (uint32_t)count);
+        if (a == b || b == c || a == c) continue;
+        bool is_strong = false;
+        if (d.prm.geom_consistency && edge_limit) {
+            is_strong = true;
+            const float ab = sn[a].x * sn[b].x + sn[a].y * sn[b].y + sn[a].z * sn[b].z;
+            const float ac = sn[a].x * sn[c].x + sn[a].y * sn[c].y + sn[a].z * sn[c].z;
+            const float bc = sn[b].x * sn[c].x + sn[b].y * sn[c].y + sn[b].z * sn[c].z;
+            if ((double)ab < 0.8660254 || (double)ac < 0.8660254 || (double)bc < 0.8660254) is_strong = false;
+            if (has_strong && !is_strong) continue;
+        }
+        if (must_in_triangle && !point_in_triangle(sp[a], sp[b], sp[c], px, py)) continue;
+        if (edge_limit) {
+            if (bresenham(d, sp[a].x, sp[a].y, sp[b].x, sp[b].y) || bresenham(d, sp[b].x, sp[b].y, sp[c].x, sp[c].y) ||
+                bresenham(d, sp[c].x, sp[c].y, sp[a].x, sp[a].y))
+                continue;
+        }
+        const float3 A = s3[a], B = s3[b], Cc = s3[c];
+        const float3 AC = make_float3(A.x - Cc.x, A.y - Cc.y, A.z - Cc.z);
+        const float3 BC = make_float3(B.x - Cc.x, B.y - Cc.y, B.z - Cc.z);
+        float4 cr;
+        cr.x = AC.y * BC.z - BC.y * AC.z;
+        cr.y = -(AC.x * BC.z - BC.x * AC.z);
+        cr.z = AC.x * BC.y - BC.x * AC.y;
+        if ((cr.x == 0 && cr.y == 0 && cr.z == 0) || cr.x != cr.x || cr.y != cr.y || cr.z != cr.z) continue;
+        normalize3(&cr);
+        cr.w = -(cr.x * A.x + cr.y * A.y + cr.z * A.z);
+        if (!has_strong && is_strong) has_strong = true;
+        float tcost = 0.0f;
+        for (int si = 0; si < count; ++si) {
+            if (si == a || si == b || si == c) continue;
+            const float fx = ((float)sp[si].x - cam.K[2]) / cam.K[0];
+            const float fy = ((float)sp[si].y - cam.K[5]) / cam.K[4];
+            const float fit_depth = -cr.w / (cr.x * fx + cr.y * fy + cr.z);
+            tcost += fabsf(fit_depth - s3[si].z);
+        }
+        if (tcost < min_cost) {
+            if (!must_in_triangle && point_in_triangle(sp[a], sp[b], sp[c], px, py)) must_in_triangle = true;
+            min_cost = tcost;
+            best = cr;
+            has_best = true;
+            ua = a;
+            ub = b;
+            uc = c;
+        }
+    }
+    if (has_best) {
+        const float depth = depth_from_plane(cam, d.planes[p], px, py);
+        const float4 vd = view_direction(cam, px, py, depth);
+        const float dot = best.x * vd.x + best.y * vd.y + best.z * vd.z;
+        if (dot > 0) {
+            best.x = -best.x;
+            best.y = -best.y;
+            best.z = -best.z;
+            best.w = -best.w;
+        }
+        d.fit[p] = best;
+        if (d.prm.use_radius) {
+            if (must_in_triangle) {
+                const short2 A = sp[ua], B = sp[ub], Cc = sp[uc];
+                const float a = sqrtf((float)((A.x - B.x) * (A.x - B.x) + (A.y - B.y) * (A.y - B.y)));
+                const float b = sqrtf((float)((B.x - Cc.x) * (B.x - Cc.x) + (B.y - Cc.y) * (B.y - Cc.y)));
+                const float c = sqrtf((float)((Cc.x - A.x) * (Cc.x - A.x) + (Cc.y - A.y) * (Cc.y - A.y)));
+                const float pp = (float)((a + b + c) / 2.0);
+                const float S = sqrtf(pp * (pp - a) * (pp - b) * (pp - c));
+                // (int)floor of NaN (degenerate triangle) is defined as 0 on both paths
+                const double rr = floor(sqrtf(S) / 2.0);
+                int radius = (rr != rr) ? 0 : (int)fmax(-1e9, fmin(1e9, rr));
+                const float A_dis = sqrtf((float)((A.x - px) * (A.x - px) + (A.y - py) * (A.y - py)));
+                const float B_dis = sqrtf((float)((B.x - px) * (B.x - px) + (B.y - py) * (B.y - py)));
+                const float C_dis = sqrtf((float)((Cc.x - px) * (Cc.x - px) + (Cc.y - py) * (Cc.y - py)));
+                const float min_dis = fminf(fminf(A_dis, B_dis), C_dis);
+                if (2.5 * min_dis < radius) radius = (int)min_dis;
+                if (edge_limit) {
+                    if (d.prm.use_edge) {
+                        float med = 3.402823466e+38f;
+                        const short2 *en = d.edge_neigh + (size_t)p * 8;
+                        for (int k = 0; k < 8; ++k) {
+                            const short2 ep = en[k];
+                            if (ep.x == -1 || ep.y == -1) continue;
+                            med = fminf(med, sqrtf((float)((ep.x - px) * (ep.x - px) + (ep.y - py) * (ep.y - py))));
+                        }
+                        if (med < radius) radius = (int)med;
+                    }
+                    if (d.prm.use_label && d.label[p] > 0) {
+                        float mbd = 3.402823466e+38f;
+                        const short2 *lb = d.label_bound + (size_t)d.nmap[p] * 8;
+                        for (int k = 0; k < 8; ++k) {
+                            const short2 bp = lb[k];
+                            if (bp.x == -1 || bp.y == -1) continue;
+                            const float dist = (float)sqrt((double)(px - bp.x) * (double)(px - bp.x) +
+                                                           (double)(py - bp.y) * (double)(py - bp.y));
+                            mbd = fminf(mbd, dist);
+                        }
+                        if (mbd < radius) radius = (int)mbd;
+                    }
+                }
+                while ((radius << 1) % 5 != 0) radius--;
+                if (!edge_limit)
+                    d.radius[p] = radius > d.prm.strong_radius ? 0 : d.prm.strong_radius;
+                else
+                    d.radius[p] = radius > d.prm.strong_radius ? radius : d.prm.strong_radius;
+            } else {
+                d.radius[p] = d.prm.strong_radius;
+            }
+        }
+    } else {
+        d.fit[p] = make_float4(0, 0, 0, 0);
+        if (d.prm.use_radius) d.radius[p] = d.prm.strong_radius;
+    }
+}
+
+// CheckerboardPropagationWeak + PlaneHypothesisRefinementWeak, DPE.cu:1672-1866, 1124-1216
+__global__ __launch_bounds__(256) void k_weak_update(Dev d, int colour, int iter) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    int x, y;
+    if (!colour_pixel(d, colour, q, x, y)) return;
+    const int W = d.W;
+    const int p = x + y * W;
+    if (d.weak[p] != DPE_WEAK) return;
+    const int NV = d.NV;
+    const dpe_camera &cam = d.cams[0];
+    const bool geom = d.prm.geom_consistency != 0;
+    const float gf = d.prm.geom_factor;
+    unsigned long long hyps = 0, taps = 0;
+    float cost_array[8][32];
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 32; ++j) cost_array[i][j] = 0.0f;
+    cost_array[0][0] = 2.0f;
+    bool flag[8];
+    int positions[8];
+    float4 new_plane[8];
+    for (int i = 0; i < 8; ++i) {
+        flag[i] = false;
+        positions[i] = 0;
+        new_plane[i] = make_float4(0, 0, 0, 0);
+        const short2 np = neighbour_point(d, p, i + 1);
+        if (np.x == -1 || np.y == -1 || d.weak[np.x + np.y * W] != DPE_STRONG) continue;
+        positions[i] = np.x + np.y * W;
+        flag[i] = true;
+        const float4 pl = d.planes[positions[i]];
+        for (int v = 1; v < d.N; ++v) cost_array[i][v - 1] = ncc_new_direct(d, x, y, v, pl, hyps, taps);
+        new_plane[i] = pl;
+    }
+    float priors[32];
+    for (int j = 0; j < NV; ++j) priors[j] = 0.0f;
+    for (int i = 0; i < 8; ++i) {
+        const short2 np = neighbour_point(d, p, i + 1);
+        if (np.x == -1 || np.y == -1) continue;
+        const uint32_t sv = d.selected[np.x + np.y * W];
+        for (int j = 0; j < NV; ++j) priors[j] += isSet(sv, j) == 1 ? 0.9f : 0.1f;
+    }
+    Rng rng(d.prm.seed, p, ph_weak(iter), d.prm.call_index);
+    uint8_t vwl[kMaxImages];
+    uint32_t tsel = 0;
+    auto cost = [&](int i, int v) -> float { return cost_array[i][v]; };
+    const float wn = joint_view_selection(d, iter, cost, priors, rng, vwl, &tsel);
+    float final_costs[8];
+    for (int i = 0; i < 8; ++i) {
+        float fc = 0.0f;
+        for (int j = 0; j < NV; ++j) {
+            if (vwl[j] > 0) {
+                if (geom) {
+                    if (flag[i])
+                        fc += vwl[j] * (cost_array[i][j] + gf * geom_cost(d, x, y, j + 1, d.planes[positions[i]]));
+                    else
+                        fc += vwl[j] * (cost_array[i][j] + gf * 3.0f);
+                } else {
+                    fc += vwl[j] * cost_array[i][j];
+                }
+            }
+        }
+        final_costs[i] = fc / wn;
+    }
+    int mi = 0;
+    {
+        float mc = final_costs[0];
+        for (int i = 1; i < 8; ++i)
+            if (final_costs[i] <= mc) { mc = final_costs[i]; mi = i; }
+    }
+    const float4 cur = d.planes[p];
+    float cost_now = 0.0f;
+    for (int v = 1; v < d.N; ++v) {
+        const float c = ncc_new_direct(d, x, y, v, cur, hyps, taps);
+        if (geom) cost_now += vwl[v - 1] * (c + gf * geom_cost(d, x, y, v, cur));
+        else cost_now += vwl[v - 1] * c;
+    }
+    cost_now /= wn;
+    d.costs[p] = cost_now;
+    float depth_now = depth_from_plane(cam, cur, x, y);
+    float4 plane_now = cur;
+    if (flag[mi]) {
+        const float db = depth_from_plane(cam, new_plane[mi], x, y);
+        if (db >= d.prm.depth_min && db <= d.prm.depth_max && final_costs[mi] < cost_now) {
+            depth_now = db;
+            plane_now = new_plane[mi];
+            cost_now = final_costs[mi];
+            d.selected[p] = tsel;
+        }
+    }
+    uint8_t *vwo = d.vw + (size_t)p * kMaxImages;
+    for (int i = 0; i < kMaxImages; ++i) vwo[i] = vwl[i];
+    // PlaneHypothesisRefinementWeak
+    const float dmin = d.prm.depth_min, dmax = d.prm.depth_max;
+    auto weighted = [&](const float4 &tp) -> float {
+        float tc = 0.0f;
+        for (int v = 1; v < d.N; ++v) {
+            if (vwl[v - 1] > 0) {
+                const float c = ncc_new_direct(d, x, y, v, tp, hyps, taps);
+                if (geom) tc += vwl[v - 1] * (c + gf * geom_cost(d, x, y, v, tp));
+                else tc += vwl[v - 1] * c;
+            }
+        }
+        return tc / wn;
+    };
+    bool do_random = true;
+    {
+        const float4 fit = d.fit[p];
+        if (fit.x == 0 && fit.y == 0 && fit.z == 0) {
+            do_random = false;
+        } else {
+            const float tc = weighted(fit);
+            const float db = depth_from_plane(cam, fit, x, y);
+            if (db >= dmin && db <= dmax && tc < cost_now) {
+                depth_now = db;
+                plane_now = fit;
+                cost_now = tc;
+            }
+        }
+    }
+    if (do_random) {
+        const float depth_rand = rng.uniform() * (dmax - dmin) + dmin;
+        const float4 plane_rand = random_normal(cam, x, y, rng, depth_now);
+        const float dminp = (1 - 0.02f) * depth_now;
+        const float dmaxp = (1 + 0.02f) * depth_now;
+        const float depth_pert = rng.uniform() * (dmaxp - dminp) + dminp;
+        const float4 plane_pert = perturbed_normal(cam, x, y, plane_now, rng, (float)(0.02f * 3.14159265358979323846));
+        const float depths[5] = {depth_rand, depth_now, depth_rand, depth_now, depth_pert};
+        const float4 normals[5] = {plane_now, plane_rand, plane_rand, plane_pert, plane_now};
+        for (int i = 0; i < 5; ++i) {
+            float4 tp = normals[i];
+            tp.w = distance_to_origin(cam, x, y, depths[i], tp);
+            const float tc = weighted(tp);
+            const float db = depth_from_plane(cam, tp, x, y);
+            if (db >= dmin && db <= dmax && tc < cost_now) {
+                depth_now = db;
+                plane_now = tp;
+                cost_now = tc;
+            }
+        }
+    }
+    float4 final_plane = cur;
+    if (d.prm.state == DPE_REFINE_INIT) {
+        if ((double)cost_now < (double)d.costs[p] - 0.1) {
+            d.costs[p] = cost_now;
+            d.planes[p] = plane_now;
+            final_plane = plane_now;
+        }
+    } else {
+        d.costs[p] = cost_now;
+        d.planes[p] = plane_now;
+        final_plane = plane_now;
+    }
+    // final cost with the Old NCC (DPE.cu:1849-1865)
+    float c2 = 0.0f;
+    for (int v = 1; v < d.N; ++v) c2 += vwl[v - 1] * ncc_old_direct(d, x, y, v, final_plane, hyps, taps);
+    d.costs[p] = c2 / wn;
+}
+
+// ------------------------------------------------------------ post-pass
+// GetDepthandNormal, DPE.cu:1944-1959
+__global__ void k_depth_and_normal(Dev d) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    const int x = p % d.W, y = p / d.W;
+    float4 pl = d.planes[p];
+    pl.w = depth_from_plane(d.cams[0], pl, x, y);
+    d.planes[p] = to_world(d.cams[0], pl);
+}
+
+// CheckerboardFilterStrong, DPE.cu:1961-2105
+__global__ void k_filter(Dev d, int colour) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    int x, y;
+    if (!colour_pixel(d, colour, q, x, y)) return;
+    const int W = d.W, H = d.H;
+    const int center = x + y * W;
+    const uint8_t *wk = d.weak;
+    if (wk[center] == DPE_WEAK) return;
+    float4 *pl = d.planes;
+    if (d.costs[center] < 0.001f) return;
+    float f[21];
+    int n = 0;
+    f[n++] = pl[center].w;
+    const int left = center - 1, leftleft = center - 3;
+    const int up = center - W, upup = center - 3 * W;
+    const int down = center + W, downdown = center + 3 * W;
+    const int right = center + 1, rightright = center + 3;
+    if (y > 0 && wk[up] == DPE_STRONG) f[n++] = pl[up].w;
+    if (y > 2 && wk[upup] == DPE_STRONG) f[n++] = pl[upup].w;
+    if (y > 4 && wk[upup - W * 2] == DPE_STRONG) f[n++] = pl[upup - W * 2].w;
+    if (y < H - 1 && wk[down] == DPE_STRONG) f[n++] = pl[down].w;
+    if (y < H - 3 && wk[downdown] == DPE_STRONG) f[n++] = pl[downdown].w;
+    if (y < H - 5 && wk[downdown + W * 2] == DPE_STRONG) f[n++] = pl[downdown + W * 2].w;
+    if (x > 0 && wk[left] == DPE_STRONG) f[n++] = pl[left].w;
+    if (x > 2 && wk[leftleft] == DPE_STRONG) f[n++] = pl[leftleft].w;
+    if (x > 4 && wk[leftleft - 2] == DPE_STRONG) f[n++] = pl[leftleft - 2].w;
+    if (x < W - 1 && wk[right] == DPE_STRONG) f[n++] = pl[right].w;
+    if (x < W - 3 && wk[rightright] == DPE_STRONG) f[n++] = pl[rightright].w;
+    if (x < W - 5 && wk[rightright + 2] == DPE_STRONG) f[n++] = pl[rightright + 2].w;
+    if (y > 0 && x < W - 2 && wk[up + 2] == DPE_STRONG) f[n++] = pl[up + 2].w;
+    if (y < H - 1 && x < W - 2 && wk[down + 2] == DPE_STRONG) f[n++] = pl[down + 2].w;
+    if (y > 0 && x > 1 && wk[up - 2] == DPE_STRONG) f[n++] = pl[up - 2].w;
+    if (y < H - 1 && x > 1 && wk[down - 2] == DPE_STRONG) f[n++] = pl[down - 2].w;
+    if (x > 0 && y > 2 && wk[left - W * 2] == DPE_STRONG) f[n++] = pl[left - W * 2].w;
+    if (x < W - 1 && y > 2 && wk[right - W * 2] == DPE_STRONG) f[n++] = pl[right - W * 2].w;
+    if (x > 0 && y < H - 2 && wk[left + W * 2] == DPE_STRONG) f[n++] = pl[left + W * 2].w;
+    if (x < W - 1 && y < H - 2 && wk[right + W * 2] == DPE_STRONG) f[n++] = pl[right + W * 2].w;
+    int j;
+    for (int i = 1; i < n; i++) {
+        const float tmp = f[i];
+        for (j = i; j >= 1 && tmp < f[j - 1]; j--) f[j] = f[j - 1];
+        f[j] = tmp;
+    }
+    const int m = n / 2;
+    pl[center].w = (n % 2 == 0) ? (f[m - 1] + f[m]) / 2 : f[m];
+}
+
+// Shared set-up of DepthToWeak (DPE.cu:2605-2661) and LocalRefine
+// (DPE.cu:2761-2808): camera-frame plane, baseline, disparity, weight norm.
+__global__ void k_post_prep(Dev d) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    const int W = d.W, H = d.H;
+    const int x = p % W, y = p / W;
+    const dpe_camera &cam = d.cams[0];
+    const uint32_t sel = d.selected[p];
+    const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
+    const float4 op = to_camera(cam, d.planes[p]);
+    const float od = op.w;
+    float bl = 0.0f, wn = 0.0f;
+    int valid = 0;
+    for (int s = 1; s < d.N; ++s) {
+        if (isSet(sel, s - 1)) {
+            wn += vw[s - 1];
+            bl += d.vc[s].base;
+            valid++;
+        }
+    }
+    uint8_t flags = 0;
+    const bool margin = x < 6 || y < 6 || x >= W - 6 || y >= H - 6;
+    if (margin || od == 0 || valid == 0) {
+        d.weak[p] = DPE_UNKNOWN;  // DepthToWeak's early exits
+    } else {
+        flags |= 1;
+    }
+    if (!(od == 0) && !(wn == 0 || valid == 0)) flags |= 2;
+    d.post_flags[p] = flags;
+    if (valid > 0) bl /= valid;
+    d.post_op[p] = op;
+    d.post_bl[p] = bl;
+    d.post_disp[p] = cam.K[0] * bl / od;
+    d.post_wn[p] = wn;
+}
+
+// Peak analysis of DepthToWeak (DPE.cu:2699-2750) and LocalRefine's update
+// (DPE.cu:2811-2838).
+__global__ void k_post_final(Dev d) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    const uint8_t flags = d.post_flags[p];
+    const size_t P = d.P;
+    if (flags & 1) {
+        const int radius = 30, n = 61;
+        int peak_count = 0, min_peak = 0;
+        float min_cost = 2.0f;
+        uint64_t peaks = 0;
+        float prev = d.pcost[1 * P + p], cur = d.pcost[2 * P + p];
+        for (int i = 2; i < n - 2; ++i) {
+            const float nxt = d.pcost[(size_t)(i + 1) * P + p];
+            if (prev > cur && nxt > cur) {
+                peaks |= (1ull << i);
+                peak_count++;
+                if (cur < min_cost) {
+                    min_peak = i;
+                    min_cost = cur;
+                }
+            }
+            prev = cur;
+            cur = nxt;
+        }
+        const float pmin = d.pcost[(size_t)min_peak * P + p];
+        uint8_t st;
+        if (abs(min_peak - radius) > d.prm.weak_peak_radius || pmin > 0.5f) {
+            st = DPE_WEAK;
+        } else if (peak_count == 1) {
+            st = pmin <= 0.15f ? DPE_STRONG : DPE_WEAK;
+        } else {
+            float var = 0.0f;
+            for (int i = 2; i < n - 2; ++i) {
+                if (((peaks >> i) & 1) && i != min_peak) {
+                    const float dist = d.pcost[(size_t)i * P + p] - min_cost;
+                    var += dist * dist;
+                }
+            }
+            var = sqrtf(var);
+            var /= (peak_count - 1);
+            st = var > 0.2f ? DPE_STRONG : DPE_WEAK;
+        }
+        d.weak[p] = st;
+    }
+    if (flags & 2) {
+        const float cost_now = d.pcost[61 * P + p];
+        float min_cost = 2.0f;
+        const dpe_camera &cam = d.cams[0];
+        const float bl = d.post_bl[p], disp = d.post_disp[p];
+        float best_depth = d.post_op[p].w;
+        for (int k = -5; k <= 5; ++k) {
+            const float tc = d.pcost[(size_t)(67 + k) * P + p];
+            if (tc < min_cost) {
+                min_cost = tc;
+                best_depth = cam.K[0] * bl / (disp + (float)k);
+            }
+        }
+        if ((double)(cost_now - min_cost) > 0.1) d.planes[p].w = best_depth;
+    }
+}
+
+// ------------------------------------------------------------ launchers
+static inline dim3 grid1(int n, int b = 256) { return dim3((n + b - 1) / b); }
+
+void launch_make_quads(const Dev &d, const float *img, float4 *quads, hipStream_t s) {
+    const int n = (d.W + 1) * (d.H + 1);
+    hipLaunchKernelGGL(k_make_quads, grid1(n), dim3(256), 0, s, img, quads, d.W, d.H);
+}
+void launch_gen_edge_inform(const Dev &d, hipStream_t s) {
+    hipLaunchKernelGGL(k_gen_edge_inform, grid1(d.P), dim3(256), 0, s, d);
+}
+void launch_find_nearest_strong(const Dev &d, hipStream_t s) {
+    hipLaunchKernelGGL(k_find_nearest_strong, grid1(d.P), dim3(256), 0, s, d);
+}
+void launch_gen_neighbours(const Dev &d, hipStream_t s) {
+    hipLaunchKernelGGL(k_gen_neighbours, grid1(d.P, 64), dim3(64), 0, s, d);
+}
+void launch_neighbour_update(const Dev &d, hipStream_t s) {
+    hipLaunchKernelGGL(k_neighbour_update, grid1(d.P), dim3(256), 0, s, d);
+}
+void launch_random_init_planes(const Dev &d, hipStream_t s) {
+    hipLaunchKernelGGL(k_random_init_planes, grid1(d.P), dim3(256), 0, s, d);
+}
+
+template <class F6, class F0>
+static void ncc_dispatch(const Dev &d, F6 f6, F0 f0) {
+    if (d.ntap == 6) f6();
+    else f0();
+}
+
+void launch_ncc_current(const Dev &d, hipStream_t s) {
+    dim3 g((d.P + 255) / 256, 1);
+    ncc_dispatch(
+        d, [&] { hipLaunchKernelGGL(k_ncc_full<6>, g, dim3(256), 0, s, d, -1, d.P, nullptr, d.planes, d.cost_mat); },
+        [&] { hipLaunchKernelGGL(k_ncc_full<0>, g, dim3(256), 0, s, d, -1, d.P, nullptr, d.planes, d.cost_mat); });
+}
+void launch_random_init_costs(const Dev &d, hipStream_t s) {
+    hipLaunchKernelGGL(k_random_init_costs, grid1(d.P), dim3(256), 0, s, d);
+}
+void launch_strong_candidates(const Dev &d, int colour, int iter, hipStream_t s) {
+    hipLaunchKernelGGL(k_strong_candidates, grid1(d.Q), dim3(256), 0, s, d, colour, iter);
+}
+void launch_ncc_strong(const Dev &d, int colour, hipStream_t s) {
+    const int nslot = d.prm.use_edge ? kStrongSlots : 9;
+    dim3 g((d.Q + 255) / 256, nslot);
+    ncc_dispatch(
+        d,
+        [&] { hipLaunchKernelGGL(k_ncc_full<6>, g, dim3(256), 0, s, d, colour, d.Q, d.cand_pos, d.planes, d.cost_mat); },
+        [&] { hipLaunchKernelGGL(k_ncc_full<0>, g, dim3(256), 0, s, d, colour, d.Q, d.cand_pos, d.planes, d.cost_mat); });
+}
+void launch_strong_select(const Dev &d, int colour, int iter, hipStream_t s) {
+    hipLaunchKernelGGL(k_strong_select, grid1(d.Q, 64), dim3(64), 0, s, d, colour, iter);
+}
+void launch_ncc_refine(const Dev &d, int colour, hipStream_t s) {
+    dim3 g((d.Q + 255) / 256, kRefineSlots);
+    ncc_dispatch(
+        d, [&] { hipLaunchKernelGGL((k_ncc_weighted<6, MODE_REFINE>), g, dim3(256), 0, s, d, colour, d.Q, d.rcost); },
+        [&] { hipLaunchKernelGGL((k_ncc_weighted<0, MODE_REFINE>), g, dim3(256), 0, s, d, colour, d.Q, d.rcost); });
+}
+void launch_strong_accept(const Dev &d, int colour, hipStream_t s) {
+    hipLaunchKernelGGL(k_strong_accept, grid1(d.Q), dim3(256), 0, s, d, colour);
+}
+void launch_ransac(const Dev &d, int iter, hipStream_t s) {
+    hipLaunchKernelGGL(k_ransac, grid1(d.P, 64), dim3(64), 0, s, d, iter);
+}
+void launch_weak_half(const Dev &d, int colour, int iter, hipStream_t s) {
+    hipLaunchKernelGGL(k_weak_update, grid1(d.Q, 64), dim3(64), 0, s, d, colour, iter);
+}
+void launch_depth_and_normal(const Dev &d, hipStream_t s) {
+    hipLaunchKernelGGL(k_depth_and_normal, grid1(d.P), dim3(256), 0, s, d);
+}
+void launch_filter(const Dev &d, int colour, hipStream_t s) {
+    hipLaunchKernelGGL(k_filter, grid1(d.Q), dim3(256), 0, s, d, colour);
+}
+void launch_post_prep(const Dev &d, hipStream_t s) {
+    hipLaunchKernelGGL(k_post_prep, grid1(d.P), dim3(256), 0, s, d);
+}
+void launch_ncc_post(const Dev &d, hipStream_t s) {
+    dim3 g((d.P + 255) / 256, kPostSlots);
+    ncc_dispatch(
+        d, [&] { hipLaunchKernelGGL((k_ncc_weighted<6, MODE_POST>), g, dim3(256), 0, s, d, -1, d.P, d.pcost); },
+        [&] { hipLaunchKernelGGL((k_ncc_weighted<0, MODE_POST>), g, dim3(256), 0, s, d, -1, d.P, d.pcost); });
+}
+void launch_post_final(const Dev &d, hipStream_t s) {
+    hipLaunchKernelGGL(k_post_final, grid1(d.P), dim3(256), 0, s, d);
+}
+void launch_ncc_given(const Dev &d, int num_hyps, const float4 *planes, float *costs, hipStream_t s) {
+    dim3 g((d.P + 255) / 256, num_hyps);
+    ncc_dispatch(
+        d, [&] { hipLaunchKernelGGL(k_ncc_full<6>, g, dim3(256), 0, s, d, -1, d.P, nullptr, planes, costs); },
+        [&] { hipLaunchKernelGGL(k_ncc_full<0>, g, dim3(256), 0, s, d, -1, d.P, nullptr, planes, costs); });
+}
+
+}  // namespace dpe

@@ -1,0 +1,100 @@
+// dpe_state.h -- device buffers of one process call and the launch interface
+// between the C-ABI (dpe_capi.hip) and the kernels (dpe_kernels.hip).
+//
+// HBM layout (P = width*height, Q = pixels of one checkerboard colour as the
+// reference's HALF launch covers them, DPE.cu:3148-3155):
+//   quads      float4[N][(H+1)(W+1)]   bilinear quads of every view (16 B/tap)
+//   ref        float[P]                reference image (patch texels)
+//   gdepth     float[N][P]             geometric-consistency depth maps
+//   planes     float4[P]               (n, d) camera frame during iterations
+//   costs, selected, view_weight (32 B/px), weak, fit planes, ... as DataPassHelper
+//   cand_pos   int[17][Q]              candidate pixel of every hypothesis slot
+//   cost_mat   float[17][NV][Q]        per-view NCC cost of every slot
+//   rplane     float4[5][Q], rcost float[5][Q]   refinement hypotheses
+#pragma once
+#include "dpe_device.h"
+
+namespace dpe {
+
+constexpr int kStrongSlots = 17;  // 0 = current plane, 1..8 candidates, 9..16 fixed-step candidates
+constexpr int kRefineSlots = 5;
+constexpr int kPostSlots = 73;    // 0..60 DepthToWeak samples, 61 LocalRefine now, 62..72 LocalRefine samples
+
+struct Dev {
+    int W, H, N, NV, P;
+    int half_rows, Wh, Q;   // colour indexing
+    int ntap, tap_off0, tap_inc;  // strong window: taps per dimension, first offset, increment
+    size_t qs;              // quad stride per view
+    dpe_params prm;
+    const dpe_camera *cams;
+    const ViewConst *vc;
+    const float *spatial;   // [ntap*ntap] spatial weight exponents
+    const float *ref;
+    const float4 *quads;
+    const float *gdepth;
+    float4 *planes;
+    float *costs;
+    uint32_t *selected;
+    uint8_t *vw;
+    float4 *fit;
+    uint8_t *weak;
+    uint8_t *weak_rel;
+    short2 *nearest;
+    short2 *neighbours;
+    short2 *edge_neigh;
+    float *complexity;
+    short2 *label_bound;
+    int *radius;
+    const int *nmap;
+    const uint8_t *edge;
+    const uint8_t *edge_low;
+    int low_w, low_h;
+    const int *label;
+    // strong half-iteration scratch
+    int *cand_pos;
+    float *cost_mat;
+    uint32_t *nsel;
+    float4 *rplane;
+    float *rcost;
+    float4 *now_plane;
+    float *now_depth;
+    float *now_cost;
+    float *wnorm;
+    // post-pass scratch (DepthToWeak / LocalRefine)
+    float4 *post_op;
+    float *post_bl;
+    float *post_disp;
+    float *post_wn;
+    uint8_t *post_flags;   // bit0 DepthToWeak evaluates, bit1 LocalRefine evaluates
+    float *pcost;          // [kPostSlots][P]
+    // GenNeighbours constants (DPE.cu:2151-2157, host double math)
+    float gn_cos, gn_sin, gn_thr;
+    int gn_shift;
+    unsigned long long *counters;  // [0] NCC hyps, [1] taps
+};
+
+// Launch interface (dpe_kernels.hip); every function enqueues on `stream`.
+// Kernels named ncc_* evaluate bilateral NCC windows (the timed hot path).
+void launch_make_quads(const Dev &d, const float *img, float4 *quads, hipStream_t s);
+void launch_gen_edge_inform(const Dev &d, hipStream_t s);
+void launch_find_nearest_strong(const Dev &d, hipStream_t s);
+void launch_gen_neighbours(const Dev &d, hipStream_t s);
+void launch_neighbour_update(const Dev &d, hipStream_t s);
+void launch_random_init_planes(const Dev &d, hipStream_t s);
+void launch_ncc_current(const Dev &d, hipStream_t s);          // slot 0 cost vector of every pixel
+void launch_random_init_costs(const Dev &d, hipStream_t s);
+void launch_strong_candidates(const Dev &d, int colour, int iter, hipStream_t s);
+void launch_ncc_strong(const Dev &d, int colour, hipStream_t s);
+void launch_strong_select(const Dev &d, int colour, int iter, hipStream_t s);
+void launch_ncc_refine(const Dev &d, int colour, hipStream_t s);
+void launch_strong_accept(const Dev &d, int colour, hipStream_t s);
+void launch_ransac(const Dev &d, int iter, hipStream_t s);
+void launch_weak_half(const Dev &d, int colour, int iter, hipStream_t s);
+void launch_depth_and_normal(const Dev &d, hipStream_t s);
+void launch_filter(const Dev &d, int colour, hipStream_t s);
+void launch_post_prep(const Dev &d, hipStream_t s);
+void launch_ncc_post(const Dev &d, hipStream_t s);
+void launch_post_final(const Dev &d, hipStream_t s);
+void launch_ncc_given(const Dev &d, int num_hyps, const float4 *planes, float *costs, hipStream_t s);
+
+}  // namespace dpe

@@ -1,0 +1,195 @@
+/*
+ * dpe_hip.h -- C-ABI of libdpe_hip.so, the MI355X (gfx950) implementation of
+ * DPE-MVS's per-reference-image PatchMatch depth/normal estimation and
+ * depth-map fusion.
+ *
+ * The reference has no FFI: its boundary is the C++ class `DPE` (DPE.h:140-243)
+ * driven by `ProcessProblem` (main.cpp:241-319) and the free function
+ * `RunFusion` (DPE.h:95-96, DPE.cpp:1387-1548).  Every entry point below names
+ * the reference interface it replaces.  All structs are plain C; every pointer
+ * is host memory unless `memory == DPE_MEM_DEVICE`, in which case it is a
+ * device pointer on the context's GPU.  No call exits the process: every
+ * failure is a status code plus a message from dpe_last_error().
+ */
+#ifndef DPE_HIP_H_
+#define DPE_HIP_H_
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPE_ABI_VERSION 1
+/* main.h:41-44 */
+#define DPE_MAX_IMAGES 32
+#define DPE_NEIGHBOUR_NUM 9
+#define DPE_MAX_SEARCH_RADIUS 4096
+
+typedef enum dpe_status {
+    DPE_OK = 0,
+    DPE_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad size, bad enum */
+    DPE_ERR_TOO_MANY_IMAGES = 2,  /* > 32 images per problem: DPE.cpp:944-947 */
+    DPE_ERR_MISSING_INPUT = 3,    /* an input the params require is null: DPE.cpp:995-998 */
+    DPE_ERR_HIP = 4,              /* HIP runtime error: replaces CudaSafeCall's exit, DPE.cpp:646-679 */
+    DPE_ERR_NO_DEVICE = 5,
+    DPE_ERR_UNSUPPORTED = 6
+} dpe_status;
+
+/* RunState, main.h:69-73 */
+typedef enum dpe_run_state { DPE_FIRST_INIT = 0, DPE_REFINE_INIT = 1, DPE_REFINE_ITER = 2 } dpe_run_state;
+/* PixelState, main.h:75-79 */
+typedef enum dpe_pixel_state { DPE_WEAK = 0, DPE_STRONG = 1, DPE_UNKNOWN = 2 } dpe_pixel_state;
+
+typedef enum dpe_memory { DPE_MEM_HOST = 0, DPE_MEM_DEVICE = 1 } dpe_memory;
+
+/* Camera, main.h:53-62: 112 bytes, byte-identical.  K and R row-major,
+ * R maps world to camera, c = -R^T t. */
+typedef struct dpe_camera {
+    float K[9];
+    float R[9];
+    float t[3];
+    float c[3];
+    int32_t height;
+    int32_t width;
+    float depth_min;
+    float depth_max;
+} dpe_camera;
+
+/* PatchMatchParams, main.h:81-109 (bools widened to int32), plus the
+ * RNG seed and the index of this process call (both select the counter-based
+ * random streams; the reference seeds curand from clock64(), DPE.cu:1036). */
+typedef struct dpe_params {
+    int32_t max_iterations;   /* 3 */
+    int32_t num_images;       /* ref + sources; set from the inputs */
+    float sigma_spatial;      /* 5 */
+    float sigma_color;        /* 3 */
+    int32_t top_k;            /* 4 */
+    float depth_min;          /* camera depth_min * 0.6 (DPE.cpp:953) */
+    float depth_max;          /* camera depth_max * 1.2 (DPE.cpp:954) */
+    int32_t geom_consistency;
+    int32_t strong_radius;    /* 5 */
+    int32_t strong_increment; /* 2 */
+    int32_t weak_radius;      /* 5 */
+    int32_t weak_increment;   /* 5 */
+    int32_t use_APD;
+    int32_t use_edge;
+    int32_t use_limit;
+    int32_t use_label;
+    int32_t use_radius;
+    int32_t high_res_img;
+    int32_t max_scale_size;
+    int32_t scale_size;
+    int32_t weak_peak_radius; /* 2 */
+    int32_t rotate_time;      /* 4 */
+    float ransac_threshold;   /* 0.005 */
+    float geom_factor;        /* 0.2 */
+    int32_t state;            /* dpe_run_state */
+    int32_t call_index;       /* which process call this is (RNG stream) */
+    uint64_t seed;            /* RNG key */
+} dpe_params;
+
+/* Inputs of one per-reference-image process call: what
+ * DPE::InuputInitialization / SupportInitialization gather (DPE.cpp:934-1069,
+ * 1187-1214) before CudaSpaceInitialization uploads them (DPE.cpp:1071-1185). */
+typedef struct dpe_inputs {
+    int32_t width;
+    int32_t height;
+    int32_t num_images;                     /* ref first, then sources; 2..32 */
+    int32_t memory;                         /* dpe_memory of every pointer below */
+    const float *images[DPE_MAX_IMAGES];    /* width*height fp32 gray, row-major */
+    dpe_camera cameras[DPE_MAX_IMAGES];     /* already scaled to width x height */
+    const float *geom_depths[DPE_MAX_IMAGES]; /* per view depth maps; geom_consistency only */
+    const uint8_t *weak_info;               /* pixel states; NULL => all STRONG (use_APD off) */
+    const float *prior_planes;              /* float4 per pixel: world normal xyz, depth w; not FIRST_INIT */
+    const uint32_t *prior_selected;         /* selected-view bits; not FIRST_INIT */
+    const uint8_t *edges;                   /* edge map (use_edge or use_limit) */
+    const uint8_t *edges_low;               /* edge map at the coarsest scale */
+    int32_t low_width;
+    int32_t low_height;
+    const int32_t *labels;                  /* label map (use_label): 0 edge, -1 small, >0 region */
+} dpe_inputs;
+
+/* Outputs: what RunPatchMatch downloads (DPE.cu:3258-3260) plus the cost map. */
+typedef struct dpe_outputs {
+    float *planes;            /* float4 per pixel: world normal xyz, depth w */
+    float *costs;             /* per pixel final cost */
+    uint8_t *pixel_states;    /* dpe_pixel_state per pixel */
+    uint32_t *selected_views; /* bit i = source view i+1 selected */
+} dpe_outputs;
+
+/* Per-kernel timing of the last dpe_process call (hipEvent, milliseconds), and
+ * exact counts of NCC plane hypotheses and taps evaluated on the GPU. */
+typedef struct dpe_stats {
+    float gpu_ms;             /* whole call: GetLastGpuTimeMs (DPE.cpp:1286) */
+    float ncc_ms;             /* all launches of the multi-view NCC cost kernel */
+    int64_t ncc_hyps;         /* (pixel, plane, source view) NCC windows */
+    int64_t ncc_taps;         /* bilinear source samples behind them */
+    int32_t ncc_launches;
+} dpe_stats;
+
+typedef struct dpe_ctx dpe_ctx;
+
+/* Context = one GPU, one HIP stream and a device memory pool (GpuWorkspace,
+ * DPE.h:67-84).  One context per host thread.  Replaces cudaSetDevice
+ * (main.cpp:343) and the GpuWorkspace constructor. */
+dpe_status dpe_create(int device, dpe_ctx **out);
+void dpe_destroy(dpe_ctx *ctx);
+/* Last error message of this context (or of dpe_create when ctx is NULL). */
+const char *dpe_last_error(const dpe_ctx *ctx);
+int dpe_abi_version(void);
+
+/* One per-image process call.  Replaces the DPE object's
+ * InuputInitialization..RunPatchMatch sequence (main.cpp:249-262, DPE.cu:3130-3262);
+ * `params` arrive with depth_min/max already scaled as DPE.cpp:953-954 does. */
+dpe_status dpe_process(dpe_ctx *ctx, const dpe_params *params,
+                       const dpe_inputs *in, dpe_outputs *out);
+dpe_status dpe_last_stats(const dpe_ctx *ctx, dpe_stats *stats);
+
+/* Multi-view bilateral NCC cost vectors, the device function
+ * ComputeMultiViewCostVectorOld (DPE.cu:874-883) exposed for tests and
+ * measurement: for every pixel p and hypothesis h, planes[(h*H*W + p)*4..+3]
+ * is a camera-frame plane (n, d) and costs[(h*(num_images-1) + v)*H*W + p] the
+ * cost against source view v+1.  num_hyps >= 1. */
+dpe_status dpe_ncc_cost(dpe_ctx *ctx, const dpe_params *params, const dpe_inputs *in,
+                        int num_hyps, const float *planes, float *costs);
+
+/* Fusion (RunFusion, DPE.cpp:1387-1548).  One entry per reference image in
+ * pair order; src_index lists indices into the same array. */
+typedef struct dpe_fuse_view {
+    int32_t width;
+    int32_t height;
+    dpe_camera camera;           /* at width x height */
+    const float *depth;          /* 0 = no depth */
+    const float *normal;         /* 3 floats per pixel, world frame */
+    const uint8_t *pixel_states;
+    const uint8_t *color;        /* 3 bytes per pixel, BGR */
+    int32_t num_src;
+    int32_t src_index[DPE_MAX_IMAGES];
+} dpe_fuse_view;
+
+typedef struct dpe_fuse_params {
+    float max_reproj_px;      /* 2.0 (DPE.cpp:1508) */
+    float max_rel_depth;      /* 0.01 */
+    float max_angle_rad;      /* 0.174533 */
+    float strong_factor;      /* 0.3 (DPE.cpp:1518) */
+    float weak_factor;        /* 0.45 */
+    int32_t min_consistent;   /* 1 in the reference (DPE.cpp:1519) */
+} dpe_fuse_params;
+
+/* Points: xyz float + bgr uchar, the PLY vertex of ExportPointCloud
+ * (DPE.cpp:532-572).  Call with points==NULL to get the count only. */
+typedef struct dpe_point {
+    float x, y, z;
+    uint8_t b, g, r, pad;
+} dpe_point;
+
+dpe_status dpe_fuse(dpe_ctx *ctx, const dpe_fuse_view *views, int num_views,
+                    const dpe_fuse_params *params, dpe_point *points, int64_t capacity,
+                    int64_t *num_points);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DPE_HIP_H_ */

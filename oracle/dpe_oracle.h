@@ -1,0 +1,30 @@
+// dpe_oracle.h -- C entry points of the CPU oracle (liboracle_dpe.so).
+// TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg; never by the product.
+#ifndef DPE_ORACLE_H_
+#define DPE_ORACLE_H_
+#include "../include/dpe_hip.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* One process call (DPE.cu:3130-3262); counters[0..1] = NCC hyps, taps. */
+int oracle_process(const dpe_params *params, const dpe_inputs *in, dpe_outputs *out, int nthreads,
+                   int64_t *counters);
+/* ComputeMultiViewCostVectorOld for given camera-frame planes (layout of dpe_ncc_cost). */
+int oracle_ncc_cost(const dpe_params *params, const dpe_inputs *in, int num_hyps, const float *planes,
+                    float *costs, int nthreads);
+/* RunFusion, DPE.cpp:1387-1548. */
+int oracle_fuse(const dpe_fuse_view *views, int num_views, const dpe_fuse_params *fp, dpe_point *points,
+                int64_t capacity, int64_t *num_points);
+float oracle_expf(float x);
+double oracle_exp(double x);
+float oracle_acosf(float x);
+void oracle_sincosf(float x, float *s, float *c);
+void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float oracle_bilinear(const float *im, int W, int H, float u, float v);
+void oracle_homography(const dpe_camera *ref, const dpe_camera *src, const float *plane, float *H);
+int oracle_point_in_triangle(const int16_t *A, const int16_t *B, const int16_t *C, int px, int py);
+#ifdef __cplusplus
+}
+#endif
+#endif

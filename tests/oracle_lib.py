@@ -1,0 +1,58 @@
+"""Loader of the CPU oracle (oracle/liboracle_dpe.so) for tests and the bench's
+CPU baseline.  The oracle is the checker, never the thing measured or shipped."""
+import ctypes as C
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle_dpe.so")
+
+
+def build_oracle():
+    subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
+    return ORACLE_LIB
+
+
+def load_oracle():
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "dpe-mvs-fast_amd"))
+    from dpe_amd import abi
+    if not os.path.exists(ORACLE_LIB):
+        build_oracle()
+    lib = C.CDLL(ORACLE_LIB)
+    P = C.POINTER
+    lib.oracle_process.argtypes = [P(abi.Params), P(abi.Inputs), P(abi.Outputs), C.c_int, P(C.c_int64)]
+    lib.oracle_process.restype = C.c_int
+    lib.oracle_ncc_cost.argtypes = [P(abi.Params), P(abi.Inputs), C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+    lib.oracle_ncc_cost.restype = C.c_int
+    lib.oracle_fuse.argtypes = [P(abi.FuseView), C.c_int, P(abi.FuseParams), C.c_void_p, C.c_int64, P(C.c_int64)]
+    lib.oracle_fuse.restype = C.c_int
+    lib.oracle_expf.argtypes = [C.c_float]
+    lib.oracle_expf.restype = C.c_float
+    lib.oracle_exp.argtypes = [C.c_double]
+    lib.oracle_exp.restype = C.c_double
+    lib.oracle_acosf.argtypes = [C.c_float]
+    lib.oracle_acosf.restype = C.c_float
+    lib.oracle_sincosf.argtypes = [C.c_float, P(C.c_float), P(C.c_float)]
+    lib.oracle_sincosf.restype = None
+    lib.oracle_philox.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+    lib.oracle_philox.restype = None
+    lib.oracle_bilinear.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float]
+    lib.oracle_bilinear.restype = C.c_float
+    lib.oracle_homography.argtypes = [P(abi.Camera), P(abi.Camera), C.c_void_p, C.c_void_p]
+    lib.oracle_homography.restype = None
+    lib.oracle_point_in_triangle.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+    lib.oracle_point_in_triangle.restype = C.c_int
+    return lib
+
+
+def run_oracle(lib, pin, nthreads=8):
+    """One oracle process call on ProblemInputs; returns (ProblemOutputs, (hyps, taps))."""
+    from dpe_amd import host
+    out = host.ProblemOutputs(pin.width, pin.height)
+    cnt = (C.c_int64 * 2)()
+    st = lib.oracle_process(C.byref(pin.params), C.byref(pin.struct), C.byref(out.struct), nthreads, cnt)
+    if st != 0:
+        raise RuntimeError(f"oracle_process failed with status {st}")
+    return out, (cnt[0], cnt[1])

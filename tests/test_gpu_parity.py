@@ -1,0 +1,143 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+Bar (north_star): >= 99.5 % of pixels with |dd|/d < 1e-3 and normals within
+0.5 degrees.  Both sides follow the same numerics contract, so most outputs are
+expected to be bit-identical; the thresholds are the stated tolerance.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from dpe_amd import abi, host, scene
+import oracle_lib
+from parity_util import depth_agreement, normal_agreement
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = host.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return oracle_lib.load_oracle()
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return scene.config1()
+
+
+def _ncc_inputs(s, ref, srcs):
+    ids = [ref] + list(srcs)
+    imgs = [s.views[i].image.astype(np.float32) for i in ids]
+    cams = host.scene_cameras(s, ids)
+    prm = host.call_params(s.depth_min, s.depth_max, use_APD=0, use_edge=0, use_limit=0, use_label=0)
+    return host.ProblemInputs(imgs, cams, prm)
+
+
+def _gt_planes(s, ref, noise, rng):
+    """Camera-frame planes (n, d) from the ground truth of the reference view, optionally perturbed."""
+    v = s.views[ref]
+    H, W = v.depth.shape
+    n = v.normal @ v.R.T  # world -> camera
+    depth = v.depth * (1 + noise * rng.standard_normal(v.depth.shape))
+    ys, xs = np.mgrid[0:H, 0:W]
+    K = v.K
+    X = np.stack([depth * (xs - K[0, 2]) / K[0, 0], depth * (ys - K[1, 2]) / K[1, 1], depth], -1)
+    d = -np.sum(n * X, -1)
+    return np.concatenate([n, d[..., None]], -1).astype(np.float32)
+
+
+def _random_planes(s, ref, rng):
+    v = s.views[ref]
+    H, W = v.depth.shape
+    n = rng.standard_normal((H, W, 3))
+    n[..., 2] = -np.abs(n[..., 2]) - 0.5
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    depth = rng.uniform(1.2, 7.2, (H, W))
+    ys, xs = np.mgrid[0:H, 0:W]
+    K = v.K
+    X = np.stack([depth * (xs - K[0, 2]) / K[0, 0], depth * (ys - K[1, 2]) / K[1, 1], depth], -1)
+    d = -np.sum(n * X, -1)
+    return np.concatenate([n, d[..., None]], -1).astype(np.float32)
+
+
+def test_ncc_cost_vectors_match_oracle(ctx, orc, c1):
+    """ComputeMultiViewCostVectorOld on GT, perturbed-GT and random planes: bitwise."""
+    ref = c1.meta["ref_mid"]
+    pin = _ncc_inputs(c1, ref, c1.pairs[ref])
+    rng = np.random.default_rng(7)
+    planes = np.stack([_gt_planes(c1, ref, 0.0, rng), _gt_planes(c1, ref, 0.01, rng), _random_planes(c1, ref, rng)])
+    nh, H, W = planes.shape[0], c1.height, c1.width
+    NV = len(c1.pairs[ref])
+    g = np.zeros((nh, NV, H, W), np.float32)
+    o = np.zeros_like(g)
+    ctx.check(ctx.lib.dpe_ncc_cost(ctx.h, C.byref(pin.params), C.byref(pin.struct), nh,
+                                   planes.ctypes.data, g.ctypes.data))
+    assert orc.oracle_ncc_cost(C.byref(pin.params), C.byref(pin.struct), nh, planes.ctypes.data,
+                               o.ctypes.data, 8) == 0
+    exact = float((g == o).mean())
+    close = float((np.abs(g - o) <= 1e-5).mean())
+    print(f"ncc bitwise {exact:.6f} within 1e-5 {close:.6f}")
+    assert close >= 0.9999
+    # ground-truth planes give near-zero cost on textured pixels
+    gt_cost = g[0].mean(0)
+    assert np.median(gt_cost) < 0.2
+    st = ctx.stats()
+    assert st.ncc_hyps == nh * NV * H * W
+
+
+def _compare(res_g, res_o, label):
+    da = depth_agreement(res_g["depth"], res_o["depth"])
+    na = normal_agreement(res_g["normal"], res_o["normal"])
+    sa = float((res_g["pixel_states"] == res_o["pixel_states"]).mean())
+    va = float((res_g["selected_views"] == res_o["selected_views"]).mean())
+    exact = float((res_g["depth"] == res_o["depth"]).mean())
+    print(f"{label}: depth {da:.5f} (bitwise {exact:.5f}) normal {na:.5f} states {sa:.5f} views {va:.5f}")
+    return da, na, sa, va
+
+
+def test_config1_first_init_matches_oracle(ctx, orc, c1):
+    """Config 1: one FIRST_INIT call, 3 checkerboard iterations, FP32."""
+    ref = c1.meta["ref_mid"]
+    pin = host.first_call_inputs(c1, ref, c1.pairs[ref], max_iterations=3)
+    out_g = host.run_hip(ctx, pin)
+    out_o, _ = oracle_lib.run_oracle(orc, pin, 8)
+    dmin, dmax = pin.params.depth_min, pin.params.depth_max
+    res_g = host.assemble(out_g, dmin, dmax)
+    res_o = host.assemble(out_o, dmin, dmax)
+    da, na, sa, va = _compare(res_g, res_o, "config1")
+    assert da >= 0.995
+    assert na >= 0.995
+    assert sa >= 0.99
+
+
+def test_config1b_weak_path_matches_oracle(ctx, orc, c1):
+    """Config 1b: a REFINE_ITER call with APD, edge sampling, the weak path and
+    geometric consistency on, fed by a FIRST_INIT call of every view."""
+    store_g, store_o = host.StateStore(), host.StateStore()
+    ref = c1.meta["ref_mid"]
+    for i in range(len(c1.views)):
+        pin = host.first_call_inputs(c1, i, c1.pairs[i], max_iterations=3, call_index=0)
+        dmin, dmax = pin.params.depth_min, pin.params.depth_max
+        store_g.put(i, host.assemble(host.run_hip(ctx, pin), dmin, dmax))
+        store_o.put(i, host.assemble(oracle_lib.run_oracle(orc, pin, 8)[0], dmin, dmax))
+    # the refine call consumes the oracle's first-call state on both sides
+    pin = host.refine_call_inputs(c1, ref, c1.pairs[ref], store_o, max_iterations=3, call_index=1)
+    weak = int((pin.weak_info == abi.WEAK).sum())
+    out_g = host.run_hip(ctx, pin)
+    out_o, _ = oracle_lib.run_oracle(orc, pin, 8)
+    dmin, dmax = pin.params.depth_min, pin.params.depth_max
+    res_g = host.assemble(out_g, dmin, dmax)
+    res_o = host.assemble(out_o, dmin, dmax)
+    print(f"weak pixels in the refine call: {weak}")
+    assert weak > 0
+    da, na, sa, va = _compare(res_g, res_o, "config1b")
+    assert da >= 0.995
+    assert na >= 0.995
