@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "dpe_state.h"
+#include "dpe_fuse.h"
 
 struct dpe_ctx {
     int device = 0;
@@ -40,7 +41,9 @@ enum BufId {
     B_IMAGES, B_QUADS, B_GDEPTH, B_CAMS, B_VC, B_SPATIAL, B_PLANES, B_COSTS, B_SELECTED, B_VW, B_FIT, B_WEAK,
     B_WEAKREL, B_NEAREST, B_NEIGHBOURS, B_EDGENEIGH, B_COMPLEX, B_LABELBOUND, B_RADIUS, B_NMAP, B_EDGE, B_EDGELOW,
     B_LABEL, B_CANDPOS, B_COSTMAT, B_NSEL, B_RPLANE, B_RCOST, B_NOWPLANE, B_NOWDEPTH, B_NOWCOST, B_WNORM, B_POSTOP,
-    B_POSTBL, B_POSTDISP, B_POSTWN, B_POSTFLAGS, B_PCOST, B_COUNTERS, B_GIVEN_PLANES, B_GIVEN_COSTS, B_COUNT
+    B_POSTBL, B_POSTDISP, B_POSTWN, B_POSTFLAGS, B_PCOST, B_COUNTERS, B_GIVEN_PLANES, B_GIVEN_COSTS,
+    B_F_VIEWS, B_F_DEPTH, B_F_NORMAL, B_F_STATES, B_F_COLOR, B_F_MASK, B_F_CLAIM, B_F_USED0, B_F_USED1, B_F_ACC0,
+    B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_COUNT
 };
 
 struct Fail {
@@ -451,6 +454,117 @@ dpe_status ncc_cost_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *
     return DPE_OK;
 }
 
+dpe_status fuse_impl(dpe_ctx *ctx, const dpe_fuse_view *views, int nv, const dpe_fuse_params *fp, dpe_point *points,
+                     int64_t capacity, int64_t *num_points) {
+    if (!views || nv <= 0 || !fp || !num_points) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad fusion arguments");
+    if (fp->memory != DPE_MEM_HOST && fp->memory != DPE_MEM_DEVICE)
+        return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad memory kind");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const bool dev = fp->memory == DPE_MEM_DEVICE;
+    std::vector<FuseViewDev> fv(nv);
+    long total = 0;
+    int maxP = 0;
+    for (int i = 0; i < nv; ++i) {
+        const dpe_fuse_view &v = views[i];
+        if (v.width <= 0 || v.height <= 0 || !v.depth || !v.normal || !v.pixel_states || !v.color)
+            return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "fusion view " + std::to_string(i) + " is incomplete");
+        if (v.num_src < 0 || v.num_src > DPE_MAX_IMAGES)
+            return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "fusion view " + std::to_string(i) + ": bad source count");
+        fv[i].width = v.width;
+        fv[i].height = v.height;
+        fv[i].offset = total;
+        fv[i].cam = v.camera;
+        fv[i].num_src = v.num_src;
+        for (int j = 0; j < v.num_src; ++j) {
+            if (v.src_index[j] < 0 || v.src_index[j] >= nv || v.src_index[j] == i)
+                return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "fusion view " + std::to_string(i) + ": bad source index");
+            fv[i].src[j] = v.src_index[j];
+        }
+        total += (long)v.width * v.height;
+        maxP = std::max(maxP, v.width * v.height);
+    }
+    FuseDev f{};
+    FuseViewDev *dv = (FuseViewDev *)buffer(ctx, B_F_VIEWS, sizeof(FuseViewDev) * nv);
+    float *depth = (float *)buffer(ctx, B_F_DEPTH, sizeof(float) * total);
+    float *normal = (float *)buffer(ctx, B_F_NORMAL, sizeof(float) * 3 * total);
+    uint8_t *states = (uint8_t *)buffer(ctx, B_F_STATES, total);
+    uint8_t *color = (uint8_t *)buffer(ctx, B_F_COLOR, 3 * total);
+    uint8_t *mask = (uint8_t *)buffer(ctx, B_F_MASK, total);
+    int *claim = (int *)buffer(ctx, B_F_CLAIM, sizeof(int) * total);
+    int *used[2] = {(int *)buffer(ctx, B_F_USED0, sizeof(int) * (size_t)maxP * DPE_MAX_IMAGES),
+                    (int *)buffer(ctx, B_F_USED1, sizeof(int) * (size_t)maxP * DPE_MAX_IMAGES)};
+    uint8_t *acc[2] = {(uint8_t *)buffer(ctx, B_F_ACC0, maxP), (uint8_t *)buffer(ctx, B_F_ACC1, maxP)};
+    int *flags = (int *)buffer(ctx, B_F_FLAGS, sizeof(int) * maxP);
+    int *index = (int *)buffer(ctx, B_F_INDEX, sizeof(int) * maxP);
+    const size_t temp_bytes = fuse_scan_temp_bytes(maxP);
+    void *temp = buffer(ctx, B_F_TEMP, temp_bytes);
+    int *changed = (int *)buffer(ctx, B_F_CHANGED, sizeof(int));
+    HIPCHK(hipMemcpyAsync(dv, fv.data(), sizeof(FuseViewDev) * nv, hipMemcpyHostToDevice, s));
+    const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    for (int i = 0; i < nv; ++i) {
+        const size_t P = (size_t)fv[i].width * fv[i].height;
+        HIPCHK(hipMemcpyAsync(depth + fv[i].offset, views[i].depth, sizeof(float) * P, kind, s));
+        HIPCHK(hipMemcpyAsync(normal + 3 * fv[i].offset, views[i].normal, sizeof(float) * 3 * P, kind, s));
+        HIPCHK(hipMemcpyAsync(states + fv[i].offset, views[i].pixel_states, P, kind, s));
+        HIPCHK(hipMemcpyAsync(color + 3 * fv[i].offset, views[i].color, 3 * P, kind, s));
+    }
+    HIPCHK(hipMemsetAsync(mask, 0, total, s));
+    f.views = dv;
+    f.depth = depth;
+    f.normal = normal;
+    f.states = states;
+    f.color = color;
+    f.mask = mask;
+    f.claim = claim;
+    f.max_reproj = fp->max_reproj_px;
+    f.max_rel = fp->max_rel_depth;
+    f.max_angle = fp->max_angle_rad;
+    f.strong_factor = fp->strong_factor;
+    f.weak_factor = fp->weak_factor;
+    f.min_consistent = fp->min_consistent;
+    // points of one view go to a staging buffer, then to the caller
+    dpe_point *stage = (dpe_point *)buffer(ctx, B_F_POINTS, sizeof(dpe_point) * (size_t)maxP);
+    int64_t count = 0;
+    for (int i = 0; i < nv; ++i) {
+        const int P = fv[i].width * fv[i].height;
+        const int ns = fv[i].num_src;
+        HIPCHK(hipMemsetAsync(acc[0], 0, P, s));
+        HIPCHK(hipMemsetAsync(used[0], 0xFF, sizeof(int) * (size_t)P * std::max(ns, 1), s));
+        launch_fuse_fill_claims(f, i, ns, s);
+        int cur = 0;
+        for (int round = 0;; ++round) {
+            if (round > 4096) return fail(ctx, DPE_ERR_HIP, "fusion fixpoint did not converge");
+            HIPCHK(hipMemsetAsync(changed, 0, sizeof(int), s));
+            launch_fuse_eval(f, i, P, used[cur], acc[cur], used[1 - cur], acc[1 - cur], changed, s);
+            cur = 1 - cur;
+            launch_fuse_fill_claims(f, i, ns, s);
+            launch_fuse_claim(f, i, P, used[cur], acc[cur], s);
+            HIPCHK(hipGetLastError());
+            int ch = 0;
+            HIPCHK(hipMemcpyAsync(&ch, changed, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (!ch) break;
+        }
+        launch_fuse_compact(f, i, P, used[cur], acc[cur], flags, index, temp, temp_bytes, stage, 0, s);
+        HIPCHK(hipGetLastError());
+        int last_index = 0, last_flag = 0;
+        HIPCHK(hipMemcpyAsync(&last_index, index + P - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&last_flag, flags + P - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const int64_t n = (int64_t)last_index + last_flag;
+        if (points && count < capacity && n > 0) {
+            const int64_t m = std::min<int64_t>(n, capacity - count);
+            HIPCHK(hipMemcpyAsync(points + count, stage, sizeof(dpe_point) * m,
+                                  dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
+        count += n;
+    }
+    *num_points = count;
+    return DPE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -524,9 +638,11 @@ dpe_status dpe_ncc_cost(dpe_ctx *ctx, const dpe_params *params, const dpe_inputs
 dpe_status dpe_fuse(dpe_ctx *ctx, const dpe_fuse_view *views, int num_views, const dpe_fuse_params *params,
                     dpe_point *points, int64_t capacity, int64_t *num_points) {
     if (!ctx) return DPE_ERR_INVALID_ARGUMENT;
-    (void)views; (void)num_views; (void)params; (void)points; (void)capacity; (void)num_points;
-    ctx->err = "dpe_fuse: not built yet";
-    return DPE_ERR_UNSUPPORTED;
+    try {
+        return fuse_impl(ctx, views, num_views, params, points, capacity, num_points);
+    } catch (const Fail &f) {
+        return f.st;
+    }
 }
 
 }  // extern "C"

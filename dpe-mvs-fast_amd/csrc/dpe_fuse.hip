@@ -1,0 +1,202 @@
+// dpe_fuse.hip -- depth-map fusion (RunFusion, DPE.cpp:1387-1548) on gfx950.
+//
+// The reference fuses on one CPU thread: reference views in pair order, pixels in
+// raster order, and every accepted point masks the source pixels it used, which
+// later pixels then skip (DPE.cpp:1474, 1497, 1528).  Within one reference view a
+// pixel's result depends only on the masks set by earlier views and by earlier
+// pixels of the same view, i.e. on pixels with a smaller raster index.  The GPU
+// evaluates all pixels of a view in parallel and iterates to the fixpoint of
+//     state_k = eval_k(masks of earlier views + claims of accepted pixels j < k),
+// where a claim is the smallest raster index of an accepted pixel that used a
+// source pixel.  Dependencies only point to smaller indices, so the iteration
+// reaches the unique fixpoint -- the sequential result -- after at most one more
+// round than the longest dependency chain (a few rounds in practice).  Accepted
+// pixels are then compacted in raster order with a prefix sum, which reproduces
+// the reference's point order.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "dpe_fuse.h"
+
+namespace dpe {
+
+// Get3DPointonWorld (host), DPE.cpp:1337-1361: the centre comes from R and t.
+__device__ __forceinline__ float3 fuse_point_world(int x, int y, float depth, const dpe_camera &cam) {
+    float3 X, T, Cc;
+    X.x = depth * ((float)x - cam.K[2]) / cam.K[0];
+    X.y = depth * ((float)y - cam.K[5]) / cam.K[4];
+    X.z = depth;
+    T.x = cam.R[0] * X.x + cam.R[3] * X.y + cam.R[6] * X.z;
+    T.y = cam.R[1] * X.x + cam.R[4] * X.y + cam.R[7] * X.z;
+    T.z = cam.R[2] * X.x + cam.R[5] * X.y + cam.R[8] * X.z;
+    Cc.x = -(cam.R[0] * cam.t[0] + cam.R[3] * cam.t[1] + cam.R[6] * cam.t[2]);
+    Cc.y = -(cam.R[1] * cam.t[0] + cam.R[4] * cam.t[1] + cam.R[7] * cam.t[2]);
+    Cc.z = -(cam.R[2] * cam.t[0] + cam.R[5] * cam.t[1] + cam.R[8] * cam.t[2]);
+    X.x = T.x + Cc.x;
+    X.y = T.y + Cc.y;
+    X.z = T.z + Cc.z;
+    return X;
+}
+
+// ProjectCamera, DPE.cpp:1363-1373
+__device__ __forceinline__ void fuse_project(const float3 &X, const dpe_camera &cam, float2 &pt, float &depth) {
+    float3 t;
+    t.x = cam.R[0] * X.x + cam.R[1] * X.y + cam.R[2] * X.z + cam.t[0];
+    t.y = cam.R[3] * X.x + cam.R[4] * X.y + cam.R[5] * X.z + cam.t[1];
+    t.z = cam.R[6] * X.x + cam.R[7] * X.y + cam.R[8] * X.z + cam.t[2];
+    depth = cam.K[6] * t.x + cam.K[7] * t.y + cam.K[8] * t.z;
+    pt.x = (cam.K[0] * t.x + cam.K[1] * t.y + cam.K[2] * t.z) / depth;
+    pt.y = (cam.K[3] * t.x + cam.K[4] * t.y + cam.K[5] * t.z) / depth;
+}
+
+// int(v + 0.5f) (DPE.cpp:1491-1492); NaN and huge values map to -1.
+__device__ __forceinline__ int fuse_round(float v) {
+    const float r = v + 0.5f;
+    if (!(r > -1.0e9f && r < 1.0e9f)) return -1;
+    return (int)r;
+}
+
+// One pixel of one reference view (DPE.cpp:1475-1541 up to the point output).
+__global__ void k_fuse_eval(FuseDev f, int vi, const int *__restrict__ used_old, const uint8_t *__restrict__ acc_old,
+                            int *__restrict__ used_new, uint8_t *__restrict__ acc_new, int *__restrict__ changed) {
+    const FuseViewDev &rv = f.views[vi];
+    const int P = rv.width * rv.height;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int r = p / rv.width, c = p - r * rv.width;
+    const long rp = rv.offset + p;
+    uint8_t accepted = 0;
+    int used[kMaxImages];
+    for (int j = 0; j < rv.num_src; ++j) used[j] = -1;
+    const float ref_depth = f.depth[rp];
+    if (f.mask[rp] != 1 && ref_depth > 0.0f) {
+        const float rn0 = f.normal[3 * rp], rn1 = f.normal[3 * rp + 1], rn2 = f.normal[3 * rp + 2];
+        const float3 X = fuse_point_world(c, r, ref_depth, rv.cam);
+        int num_consistent = 0;
+        float dyn = 0.0f;
+        for (int j = 0; j < rv.num_src; ++j) {
+            const FuseViewDev &sv = f.views[rv.src[j]];
+            float2 pt;
+            float proj_depth;
+            fuse_project(X, sv.cam, pt, proj_depth);
+            const int src_r = fuse_round(pt.y);
+            const int src_c = fuse_round(pt.x);
+            if (src_c >= 0 && src_c < sv.width && src_r >= 0 && src_r < sv.height) {
+                const long sp = sv.offset + (long)src_r * sv.width + src_c;
+                if (f.mask[sp] == 1 || f.claim[sp] < p) continue;
+                const float src_depth = f.depth[sp];
+                if (src_depth <= 0.0f) continue;
+                const float3 tX = fuse_point_world(src_c, src_r, src_depth, sv.cam);
+                float2 tp;
+                fuse_project(tX, rv.cam, tp, proj_depth);
+                const float dx = (float)c - tp.x, dy = (float)r - tp.y;
+                const float reproj = (float)sqrt((double)dx * (double)dx + (double)dy * (double)dy);
+                const float rel = fabsf(proj_depth - ref_depth) / ref_depth;
+                const float dot = rn0 * f.normal[3 * sp] + rn1 * f.normal[3 * sp + 1] + rn2 * f.normal[3 * sp + 2];
+                float angle = d_acosf(dot);
+                if (angle != angle) angle = 0.0f;  // GetAngle, DPE.cpp:1379-1382
+                if (reproj < f.max_reproj && rel < f.max_rel && angle < f.max_angle) {
+                    used[j] = (int)(sp - sv.offset);
+                    const float idx = reproj + 200 * rel + angle * 10;
+                    dyn += d_expf(-idx);
+                    num_consistent++;
+                }
+            }
+        }
+        const float factor = f.states[rp] == DPE_WEAK ? f.weak_factor : f.strong_factor;
+        accepted = (num_consistent >= f.min_consistent && num_consistent >= 1 && dyn > factor * num_consistent) ? 1 : 0;
+    }
+    bool diff = acc_old[p] != accepted;
+    acc_new[p] = accepted;
+    for (int j = 0; j < rv.num_src; ++j) {
+        const int u = accepted ? used[j] : -1;
+        diff = diff || used_old[(size_t)j * P + p] != u;
+        used_new[(size_t)j * P + p] = u;
+    }
+    if (diff) atomicOr(changed, 1);
+}
+
+__global__ void k_fuse_fill_claims(FuseDev f, int vi) {
+    const FuseViewDev &rv = f.views[vi];
+    const int k = blockIdx.y;
+    const FuseViewDev &sv = f.views[rv.src[k]];
+    const int P = sv.width * sv.height;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) f.claim[sv.offset + p] = 0x7FFFFFFF;
+}
+
+__global__ void k_fuse_claim(FuseDev f, int vi, const int *__restrict__ used, const uint8_t *__restrict__ acc) {
+    const FuseViewDev &rv = f.views[vi];
+    const int P = rv.width * rv.height;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || !acc[p]) return;
+    for (int j = 0; j < rv.num_src; ++j) {
+        const int u = used[(size_t)j * P + p];
+        if (u >= 0) atomicMin(&f.claim[f.views[rv.src[j]].offset + u], p);
+    }
+}
+
+// Point output and mask update of the accepted pixels (DPE.cpp:1520-1541).
+__global__ void k_fuse_commit(FuseDev f, int vi, const int *__restrict__ used, const uint8_t *__restrict__ acc,
+                              const int *__restrict__ index, dpe_point *__restrict__ out, long base) {
+    const FuseViewDev &rv = f.views[vi];
+    const int P = rv.width * rv.height;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || !acc[p]) return;
+    const int r = p / rv.width, c = p - r * rv.width;
+    const long rp = rv.offset + p;
+    const float3 X = fuse_point_world(c, r, f.depth[rp], rv.cam);
+    float col[3] = {(float)f.color[3 * rp], (float)f.color[3 * rp + 1], (float)f.color[3 * rp + 2]};
+    int n = 0;
+    for (int j = 0; j < rv.num_src; ++j) {
+        const int u = used[(size_t)j * P + p];
+        if (u < 0) continue;
+        const long sp = f.views[rv.src[j]].offset + u;
+        f.mask[sp] = 1;
+        col[0] += f.color[3 * sp];
+        col[1] += f.color[3 * sp + 1];
+        col[2] += f.color[3 * sp + 2];
+        n++;
+    }
+    col[0] /= (n + 1);
+    col[1] /= (n + 1);
+    col[2] /= (n + 1);
+    dpe_point o;
+    o.x = X.x;
+    o.y = X.y;
+    o.z = X.z;
+    o.b = (uint8_t)col[0];
+    o.g = (uint8_t)col[1];
+    o.r = (uint8_t)col[2];
+    o.pad = 0;
+    out[base + index[p]] = o;
+}
+
+__global__ void k_fuse_flags(const uint8_t *__restrict__ acc, int *__restrict__ flags, int P) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < P) flags[p] = acc[p];
+}
+
+void launch_fuse_eval(const FuseDev &f, int vi, int P, const int *used_old, const uint8_t *acc_old, int *used_new,
+                      uint8_t *acc_new, int *changed, hipStream_t s) {
+    hipLaunchKernelGGL(k_fuse_eval, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, used_old, acc_old, used_new, acc_new,
+                       changed);
+}
+void launch_fuse_fill_claims(const FuseDev &f, int vi, int num_src, hipStream_t s) {
+    if (num_src > 0) hipLaunchKernelGGL(k_fuse_fill_claims, dim3(1024, num_src), dim3(256), 0, s, f, vi);
+}
+void launch_fuse_claim(const FuseDev &f, int vi, int P, const int *used, const uint8_t *acc, hipStream_t s) {
+    hipLaunchKernelGGL(k_fuse_claim, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, used, acc);
+}
+size_t fuse_scan_temp_bytes(int P) {
+    size_t bytes = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (int *)nullptr, (int *)nullptr, P);
+    return bytes;
+}
+void launch_fuse_compact(const FuseDev &f, int vi, int P, const int *used, const uint8_t *acc, int *flags, int *index,
+                         void *temp, size_t temp_bytes, dpe_point *out, long base, hipStream_t s) {
+    hipLaunchKernelGGL(k_fuse_flags, dim3((P + 255) / 256), dim3(256), 0, s, acc, flags, P);
+    hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, flags, index, P, s);
+    hipLaunchKernelGGL(k_fuse_commit, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, used, acc, index, out, base);
+}
+
+}  // namespace dpe

@@ -63,7 +63,8 @@ class FuseView(C.Structure):
 
 class FuseParams(C.Structure):
     _fields_ = [("max_reproj_px", C.c_float), ("max_rel_depth", C.c_float), ("max_angle_rad", C.c_float),
-                ("strong_factor", C.c_float), ("weak_factor", C.c_float), ("min_consistent", C.c_int32)]
+                ("strong_factor", C.c_float), ("weak_factor", C.c_float), ("min_consistent", C.c_int32),
+                ("memory", C.c_int32)]
 
 
 class Point(C.Structure):
@@ -90,7 +91,7 @@ def default_params(**kw):
 def fuse_defaults(**kw):
     """RunFusion thresholds (DPE.cpp:1508, 1518-1519)."""
     f = FuseParams(max_reproj_px=2.0, max_rel_depth=0.01, max_angle_rad=0.174533, strong_factor=0.3,
-                   weak_factor=0.45, min_consistent=1)
+                   weak_factor=0.45, min_consistent=1, memory=MEM_HOST)
     for k, v in kw.items():
         setattr(f, k, v)
     return f

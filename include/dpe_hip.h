@@ -176,6 +176,7 @@ typedef struct dpe_fuse_params {
     float strong_factor;      /* 0.3 (DPE.cpp:1518) */
     float weak_factor;        /* 0.45 */
     int32_t min_consistent;   /* 1 in the reference (DPE.cpp:1519) */
+    int32_t memory;           /* dpe_memory of the view pointers and of `points` */
 } dpe_fuse_params;
 
 /* Points: xyz float + bgr uchar, the PLY vertex of ExportPointCloud

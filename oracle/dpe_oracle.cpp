@@ -2128,6 +2128,13 @@ F3 FuseGet3DPointonWorld(int x, int y, float depth, const dpe_camera &cam) {
     return X;
 }
 
+// int(v + 0.5f) of RunFusion (DPE.cpp:1491-1492); NaN and huge values map to -1
+// (outside every image).
+inline int fuse_round(float v) {
+    const float r = v + 0.5f;
+    if (!(r > -1.0e9f && r < 1.0e9f)) return -1;
+    return (int)r;
+}
 }  // namespace
 }  // namespace oracle
 
@@ -2210,8 +2217,8 @@ int oracle_fuse(const dpe_fuse_view *views, int num_views, const dpe_fuse_params
                     F2 pt;
                     float proj_depth;
                     ProjectonCamera_cu(PointX, sv.camera, pt, proj_depth);
-                    const int src_r = (int)(pt.y + 0.5f);
-                    const int src_c = (int)(pt.x + 0.5f);
+                    const int src_r = fuse_round(pt.y);
+                    const int src_c = fuse_round(pt.x);
                     if (src_c >= 0 && src_c < sv.width && src_r >= 0 && src_r < sv.height) {
                         const size_t sp = (size_t)src_r * sv.width + src_c;
                         if (masks[si][sp] == 1) continue;
@@ -2221,8 +2228,9 @@ int oracle_fuse(const dpe_fuse_view *views, int num_views, const dpe_fuse_params
                         const F3 tX = FuseGet3DPointonWorld(src_c, src_r, src_depth, sv.camera);
                         F2 tp;
                         ProjectonCamera_cu(tX, rv.camera, tp, proj_depth);
+                        // sqrt(pow(c - x, 2) + pow(r - y, 2)) evaluates in double (DPE.cpp:1506)
                         const float dx = (float)c - tp.x, dy = (float)r - tp.y;
-                        const float reproj = std::sqrt(dx * dx + dy * dy);
+                        const float reproj = (float)std::sqrt((double)dx * (double)dx + (double)dy * (double)dy);
                         const float rel = std::fabs(proj_depth - ref_depth) / ref_depth;
                         const float dot = rn[0] * sn[0] + rn[1] * sn[1] + rn[2] * sn[2];
                         const float angle = o_acosf(dot);

@@ -1,0 +1,51 @@
+"""Fusion inputs for tests: dpe_fuse_view structs over host arrays."""
+import ctypes as C
+
+import numpy as np
+
+from dpe_amd import abi, host
+
+
+class FuseSet:
+    def __init__(self, scene, depths, normals, states, pairs=None):
+        self.keep = []
+        n = len(scene.views)
+        self.views = (abi.FuseView * n)()
+        cams = host.scene_cameras(scene, list(range(n)))
+        pairs = pairs or scene.pairs
+        for i, v in enumerate(scene.views):
+            d = np.ascontiguousarray(depths[i], np.float32)
+            nm = np.ascontiguousarray(normals[i], np.float32)
+            st = np.ascontiguousarray(states[i], np.uint8)
+            col = np.ascontiguousarray(np.repeat(v.image[..., None], 3, -1), np.uint8)
+            self.keep += [d, nm, st, col]
+            fv = self.views[i]
+            fv.width, fv.height = scene.width, scene.height
+            fv.camera = cams[i]
+            fv.depth = d.ctypes.data
+            fv.normal = nm.ctypes.data
+            fv.pixel_states = st.ctypes.data
+            fv.color = col.ctypes.data
+            fv.num_src = len(pairs[i])
+            for j, s in enumerate(pairs[i]):
+                fv.src_index[j] = s
+        self.n = n
+
+
+def fuse_oracle(orc, fs, fp):
+    cnt = C.c_int64()
+    assert orc.oracle_fuse(fs.views, fs.n, C.byref(fp), None, 0, C.byref(cnt)) == 0
+    pts = (abi.Point * max(cnt.value, 1))()
+    assert orc.oracle_fuse(fs.views, fs.n, C.byref(fp), pts, cnt.value, C.byref(cnt)) == 0
+    return np.frombuffer(pts, dtype=np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("b", "u1"), ("g", "u1"),
+                                              ("r", "u1"), ("pad", "u1")]))[:cnt.value]
+
+
+def fuse_hip(ctx, fs, fp):
+    cnt = C.c_int64()
+    ctx.check(ctx.lib.dpe_fuse(ctx.h, fs.views, fs.n, C.byref(fp), None, 0, C.byref(cnt)))
+    pts = (abi.Point * max(cnt.value, 1))()
+    ctx.check(ctx.lib.dpe_fuse(ctx.h, fs.views, fs.n, C.byref(fp), C.cast(pts, C.c_void_p), cnt.value,
+                               C.byref(cnt)))
+    return np.frombuffer(pts, dtype=np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("b", "u1"), ("g", "u1"),
+                                              ("r", "u1"), ("pad", "u1")]))[:cnt.value]

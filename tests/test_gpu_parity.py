@@ -86,9 +86,9 @@ def test_ncc_cost_vectors_match_oracle(ctx, orc, c1):
     close = float((np.abs(g - o) <= 1e-5).mean())
     print(f"ncc bitwise {exact:.6f} within 1e-5 {close:.6f}")
     assert close >= 0.9999
-    # ground-truth planes give near-zero cost on textured pixels
-    gt_cost = g[0].mean(0)
-    assert np.median(gt_cost) < 0.2
+    # ground-truth planes give near-zero cost in the best source view
+    gt_cost = g[0].min(0)
+    assert np.median(gt_cost[~c1.views[ref].weak]) < 0.15
     st = ctx.stats()
     assert st.ncc_hyps == nh * NV * H * W
 
@@ -141,3 +141,26 @@ def test_config1b_weak_path_matches_oracle(ctx, orc, c1):
     da, na, sa, va = _compare(res_g, res_o, "config1b")
     assert da >= 0.995
     assert na >= 0.995
+
+
+def test_fusion_matches_oracle(ctx, orc, c1):
+    """RunFusion on ground-truth and on perturbed depth maps: the GPU fixpoint
+    reproduces the sequential mask semantics, so point counts and points match."""
+    from fuse_util import FuseSet, fuse_hip, fuse_oracle
+    rng = np.random.default_rng(9)
+    n = len(c1.views)
+    states = [np.where(v.weak, abi.WEAK, abi.STRONG).astype(np.uint8) for v in c1.views]
+    normals = [v.normal for v in c1.views]
+    for noise in (0.0, 0.004):
+        depths = [v.depth * (1 + noise * rng.standard_normal(v.depth.shape)).astype(np.float32) for v in c1.views]
+        fs = FuseSet(c1, depths, normals, states)
+        fp = abi.fuse_defaults()
+        po = fuse_oracle(orc, fs, fp)
+        pg = fuse_hip(ctx, fs, fp)
+        print(f"fusion noise {noise}: oracle {len(po)} points, gpu {len(pg)}")
+        assert abs(len(pg) - len(po)) <= 0.005 * len(po)
+        if len(pg) == len(po):
+            assert np.array_equal(pg["x"], po["x"]) and np.array_equal(pg["b"], po["b"])
+    # config 2's rule: at least 3 consistent views
+    fp3 = abi.fuse_defaults(min_consistent=3)
+    assert abs(len(fuse_hip(ctx, fs, fp3)) - len(fuse_oracle(orc, fs, fp3))) <= 0.005 * len(po)
