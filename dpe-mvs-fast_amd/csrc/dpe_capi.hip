@@ -43,7 +43,7 @@ enum BufId {
     B_LABEL, B_CANDPOS, B_COSTMAT, B_NSEL, B_RPLANE, B_RCOST, B_NOWPLANE, B_NOWDEPTH, B_NOWCOST, B_WNORM, B_POSTOP,
     B_POSTBL, B_POSTDISP, B_POSTWN, B_POSTFLAGS, B_PCOST, B_COUNTERS, B_GIVEN_PLANES, B_GIVEN_COSTS,
     B_F_VIEWS, B_F_DEPTH, B_F_NORMAL, B_F_STATES, B_F_COLOR, B_F_MASK, B_F_CLAIM, B_F_USED0, B_F_USED1, B_F_ACC0,
-    B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_COUNT
+    B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_POSTPLANES, B_COUNT
 };
 
 struct Fail {
@@ -348,6 +348,7 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *i
     d.post_wn = (float *)buffer(ctx, B_POSTWN, sizeof(float) * P);
     d.post_flags = (uint8_t *)buffer(ctx, B_POSTFLAGS, P);
     d.pcost = (float *)buffer(ctx, B_PCOST, sizeof(float) * kPostSlots * P);
+    d.post_planes = (float4 *)buffer(ctx, B_POSTPLANES, sizeof(float4) * kPostSlots * P);
     d.counters = (unsigned long long *)buffer(ctx, B_COUNTERS, sizeof(unsigned long long) * 2);
     HIPCHK(hipMemsetAsync(d.counters, 0, sizeof(unsigned long long) * 2, ctx->stream));
     // GenNeighbours constants (DPE.cu:2151-2157): host double math
@@ -387,6 +388,7 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *i
     launch_filter(d, 0, s);
     launch_filter(d, 1, s);
     launch_post_prep(d, s);
+    launch_post_planes(d, s);
     tm.ncc_launch([&] { launch_ncc_post(d, s); });
     launch_post_final(d, s);
     HIPCHK(hipGetLastError());

@@ -60,49 +60,70 @@ __global__ void k_make_quads(const float *__restrict__ img, float4 *__restrict__
 }
 
 // ------------------------------------------------------------ NCC windows
-// Reference half of the strong window of pixel (x, y): bilateral weights
-// (ComputeBilateralWeight, DPE.cu:554-559) and the hypothesis-independent sums.
+// ---- workgroup-shared reference patches (the hot-path layout)
+// A workgroup covers 64 consecutive pixels (one per lane) and 4 waves.  The
+// waves fill the 64 patches into LDS together (tap k of lane l at [k][l], so a
+// wave reads 64 consecutive float2: no bank conflicts), then every wave loops
+// over hypothesis slots of the same 64 pixels, reading the weights from LDS.
+// Taps of one window row evaluated together (registers vs. loads in flight).
+#ifndef DPE_TAP_UNROLL
+#define DPE_TAP_UNROLL 3
+#endif
+constexpr int kLanes = 64;
+constexpr int kWaves = 4;
+
 template <int NT>
-struct Patch {
-    float w[NT * NT];
-    float wr[NT * NT];
-    float sum_ref, sum_ref_ref, wsum;
+struct SharedPatch {
+    float2 wwr[NT * NT][kLanes];  // (w, w*ref) per tap
+    float sum_ref[kLanes], sum_ref_ref[kLanes], wsum[kLanes];
 };
 
 template <int NT>
-__device__ __forceinline__ void load_patch(const Dev &d, int x, int y, Patch<NT> &P) {
-    const float rc = texel(d.ref, d.W, d.H, x, y);
+__device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int x, int y, SharedPatch<NT> &S) {
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
     const float sc = d.prm.sigma_color;
     const float cden = 2.0f * sc * sc;
-    float s_r = 0.0f, s_rr = 0.0f, s_w = 0.0f;
-#pragma unroll
-    for (int ki = 0; ki < NT; ++ki) {
-        float r_r = 0.0f, r_rr = 0.0f, r_w = 0.0f;
-#pragma unroll
-        for (int kj = 0; kj < NT; ++kj) {
-            const int k = ki * NT + kj;
+    const float rc = valid ? texel(d.ref, d.W, d.H, x, y) : 0.0f;
+    for (int k = wave; k < NT * NT; k += kWaves) {
+        const int ki = k / NT, kj = k - ki * NT;
+        float2 e = make_float2(0.0f, 0.0f);
+        if (valid) {
             const float ref = texel(d.ref, d.W, d.H, x + d.tap_off0 + ki * d.tap_inc, y + d.tap_off0 + kj * d.tap_inc);
             const float w = d_expf(d.spatial[k] - fabsf(ref - rc) / cden);
-            const float wr = w * ref;
-            P.w[k] = w;
-            P.wr[k] = wr;
-            r_r = fmaf(w, ref, r_r);
-            r_rr = fmaf(wr, ref, r_rr);
-            r_w = r_w + w;
+            e = make_float2(w, w * ref);
         }
-        s_r += r_r;
-        s_rr += r_rr;
-        s_w += r_w;
+        S.wwr[k][lane] = e;
     }
-    P.sum_ref = s_r;
-    P.sum_ref_ref = s_rr;
-    P.wsum = s_w;
+    __syncthreads();
+    if (wave == 0 && valid) {
+        // hypothesis-independent sums in the reference's row-partial order
+        float s_r = 0.0f, s_rr = 0.0f, s_w = 0.0f;
+        for (int ki = 0; ki < NT; ++ki) {
+            float r_r = 0.0f, r_rr = 0.0f, r_w = 0.0f;
+#pragma unroll
+            for (int kj = 0; kj < NT; ++kj) {
+                const float2 e = S.wwr[ki * NT + kj][lane];
+                const float ref = texel(d.ref, d.W, d.H, x + d.tap_off0 + ki * d.tap_inc, y + d.tap_off0 + kj * d.tap_inc);
+                r_r = fmaf(e.x, ref, r_r);
+                r_rr = fmaf(e.y, ref, r_rr);
+                r_w = r_w + e.x;
+            }
+            s_r += r_r;
+            s_rr += r_rr;
+            s_w += r_w;
+        }
+        S.sum_ref[lane] = s_r;
+        S.sum_ref_ref[lane] = s_rr;
+        S.wsum[lane] = s_w;
+    }
+    __syncthreads();
 }
 
-// ComputeBilateralNCCOld, DPE.cu:696-782, with the reference half precomputed.
+// ComputeBilateralNCCOld (DPE.cu:696-782) with the reference half from LDS.
 template <int NT>
-__device__ __forceinline__ float ncc_patch(const Dev &d, const Patch<NT> &P, int x, int y, int v, const float4 &pl,
-                                           unsigned long long &taps) {
+__device__ __forceinline__ float ncc_shared(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y, int v,
+                                            const float4 &pl, unsigned long long &taps) {
     const ViewConst &vc = d.vc[v];
     float H[9];
     homography(vc, d.cams[0], pl, H);
@@ -110,27 +131,26 @@ __device__ __forceinline__ float ncc_patch(const Dev &d, const Patch<NT> &P, int
     if (pt.x >= (float)vc.width || pt.x < 0.0f || pt.y >= (float)vc.height || pt.y < 0.0f) return 2.0f;
     const float4 *__restrict__ q = d.quads + (size_t)v * d.qs;
     float s_s = 0.0f, s_ss = 0.0f, s_rs = 0.0f;
-#pragma unroll
+#pragma unroll 1
     for (int ki = 0; ki < NT; ++ki) {
         float r_s = 0.0f, r_ss = 0.0f, r_rs = 0.0f;
         const int rx = x + d.tap_off0 + ki * d.tap_inc;
-#pragma unroll
+#pragma unroll DPE_TAP_UNROLL
         for (int kj = 0; kj < NT; ++kj) {
-            const int k = ki * NT + kj;
             const int ry = y + d.tap_off0 + kj * d.tap_inc;
             const float2 sp = project(H, rx, ry);
             const float src = bilinear_quad(q, d.W, d.H, sp.x, sp.y);
-            const float w = P.w[k];
-            r_s = fmaf(w, src, r_s);
-            r_ss = fmaf(w * src, src, r_ss);
-            r_rs = fmaf(P.wr[k], src, r_rs);
+            const float2 e = S.wwr[ki * NT + kj][lane];
+            r_s = fmaf(e.x, src, r_s);
+            r_ss = fmaf(e.x * src, src, r_ss);
+            r_rs = fmaf(e.y, src, r_rs);
         }
         s_s += r_s;
         s_ss += r_ss;
         s_rs += r_rs;
     }
     taps += NT * NT;
-    return ncc_from_sums(P.sum_ref, P.sum_ref_ref, s_s, s_ss, s_rs, P.wsum);
+    return ncc_from_sums(S.sum_ref[lane], S.sum_ref_ref[lane], s_s, s_ss, s_rs, S.wsum[lane]);
 }
 
 // One bilateral window computed directly (any radius/increment): the shared
@@ -290,10 +310,53 @@ __device__ float geom_cost(const Dev &d, int x, int y, int v, const float4 &pl) 
 // ---------------------------------------------------- NCC cost kernels (hot)
 // Per-view cost matrix out[(slot*NV + v-1)*Qn + q] of the plane of every
 // (pixel, slot): plane_src[cand_pos[slot*Qn+q]] if cand_pos is given (a -1
-// entry skips the slot), else plane_src[slot*Qn + q].
+// entry skips the slot), else plane_src[slot*Qn + q].  One workgroup = 64
+// pixels x 4 waves; wave w evaluates slots w, w+4, ...
 template <int NT>
-__global__ __launch_bounds__(256) void k_ncc_full(Dev d, int colour, int Qn, const int *__restrict__ cand_pos,
-                                                   const float4 *__restrict__ plane_src, float *__restrict__ out) {
+__global__ __launch_bounds__(kLanes * kWaves) void k_ncc_full(Dev d, int colour, int Qn, int nslot,
+                                                              const int *__restrict__ cand_pos,
+                                                              const float4 *__restrict__ plane_src,
+                                                              float *__restrict__ out) {
+    __shared__ SharedPatch<NT> S;
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
+    const int q = blockIdx.x * kLanes + lane;
+    int x = 0, y = 0;
+    const bool valid = q < Qn && colour_pixel(d, colour, q, x, y);
+    bool any_slot = false;
+    if (valid && cand_pos) {
+        for (int s = 0; s < nslot; ++s) any_slot = any_slot || cand_pos[(size_t)s * Qn + q] >= 0;
+    } else {
+        any_slot = valid;
+    }
+    fill_shared_patch<NT>(d, any_slot, x, y, S);
+    unsigned long long hyps = 0, taps = 0;
+    for (int slot = wave; slot < nslot; slot += kWaves) {
+        bool active = any_slot;
+        float4 pl = make_float4(0, 0, 0, 0);
+        if (active) {
+            if (cand_pos) {
+                const int pos = cand_pos[(size_t)slot * Qn + q];
+                active = pos >= 0;
+                if (active) pl = plane_src[pos];
+            } else {
+                pl = plane_src[(size_t)slot * Qn + q];
+            }
+        }
+        if (active) {
+            for (int v = 1; v < d.N; ++v) {
+                out[((size_t)slot * d.NV + (v - 1)) * Qn + q] = ncc_shared<NT>(d, S, lane, x, y, v, pl, taps);
+                hyps++;
+            }
+        }
+    }
+    count_add(d.counters, hyps, taps);
+}
+
+// Generic window (any radius/increment): one (pixel, slot) per thread, direct.
+__global__ __launch_bounds__(256) void k_ncc_full_direct(Dev d, int colour, int Qn, const int *__restrict__ cand_pos,
+                                                         const float4 *__restrict__ plane_src,
+                                                         float *__restrict__ out) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int slot = blockIdx.y;
     int x = 0, y = 0;
@@ -310,22 +373,36 @@ __global__ __launch_bounds__(256) void k_ncc_full(Dev d, int colour, int Qn, con
     }
     unsigned long long hyps = 0, taps = 0;
     if (active) {
-        if constexpr (NT > 0) {
-            Patch<NT> P;
-            load_patch<NT>(d, x, y, P);
-            for (int v = 1; v < d.N; ++v) {
-                out[((size_t)slot * d.NV + (v - 1)) * Qn + q] = ncc_patch<NT>(d, P, x, y, v, pl, taps);
-                hyps++;
-            }
-        } else {
-            for (int v = 1; v < d.N; ++v) {
-                unsigned long long h = 0;
-                out[((size_t)slot * d.NV + (v - 1)) * Qn + q] = ncc_old_direct(d, x, y, v, pl, h, taps);
-                hyps += h;
-            }
+        for (int v = 1; v < d.N; ++v) {
+            unsigned long long h = 0;
+            out[((size_t)slot * d.NV + (v - 1)) * Qn + q] = ncc_old_direct(d, x, y, v, pl, h, taps);
+            hyps += h;
         }
     }
     count_add(d.counters, hyps, taps);
+}
+
+// Plane and weight set-up of one weighted slot; returns false if the slot is
+// not evaluated.  REFINE reads the refinement hypotheses; POST reads the sample
+// planes k_post_planes materialised (w = NaN: sample outside the depth range,
+// its cost already written).
+template <int MODE>
+__device__ __forceinline__ bool weighted_slot(const Dev &d, int Qn, int q, int p, int slot, float4 &pl, float &wn,
+                                              uint32_t &sel) {
+    if (MODE == MODE_REFINE) {
+        if (d.cand_pos[q] < 0) return false;
+        pl = d.rplane[(size_t)slot * Qn + q];
+        wn = d.wnorm[q];
+        return true;
+    }
+    const uint8_t fl = d.post_flags[p];
+    const bool dtw = slot < 61;
+    if (!((dtw && (fl & 1)) || (!dtw && (fl & 2)))) return false;
+    pl = d.post_planes[(size_t)slot * d.P + p];
+    if (pl.w != pl.w) return false;
+    wn = d.post_wn[p];
+    sel = d.selected[p];
+    return true;
 }
 
 // Weighted sums over views.
@@ -334,100 +411,80 @@ __global__ __launch_bounds__(256) void k_ncc_full(Dev d, int colour, int Qn, con
 //   MODE_POST: DepthToWeak samples (DPE.cu:2667-2690) and LocalRefine
 //     (DPE.cu:2776-2835), all over the pixel's selected views.
 template <int NT, int MODE>
-__global__ __launch_bounds__(256) void k_ncc_weighted(Dev d, int colour, int Qn, float *__restrict__ out) {
+__device__ __forceinline__ float weighted_sum(const Dev &d, const void *Sv, int lane, int x, int y,
+                                              int slot, const uint8_t *vw, uint32_t sel, const float4 &pl, float wn,
+                                              unsigned long long &hyps, unsigned long long &taps) {
+    float acc = 0.0f;
+    const bool geom = d.prm.geom_consistency != 0;
+    const float gf = d.prm.geom_factor;
+    for (int v = 1; v < d.N; ++v) {
+        const float wv = (float)vw[v - 1];
+        const bool use = (MODE == MODE_REFINE) ? (vw[v - 1] > 0) : (isSet(sel, v - 1) != 0);
+        if (!use) continue;
+        float c;
+        if constexpr (NT > 0) {
+            c = ncc_shared<NT>(d, *(const SharedPatch<NT> *)Sv, lane, x, y, v, pl, taps);
+            hyps++;
+        } else {
+            c = ncc_old_direct(d, x, y, v, pl, hyps, taps);
+        }
+        if (MODE == MODE_REFINE) {
+            acc += wv * c;
+        } else if (slot <= 61) {
+            float tc = c;
+            if (geom) tc += gf * geom_cost(d, x, y, v, pl);
+            acc += tc * wv;
+        } else {
+            acc += c * wv;
+            if (geom) acc += gf * geom_cost(d, x, y, v, pl) * wv;
+        }
+    }
+    acc /= wn;
+    if (MODE == MODE_POST && slot < 61) acc = fminf(2.0f, acc);
+    return acc;
+}
+
+// One workgroup = 64 pixels x 4 waves; wave w evaluates slot 4*blockIdx.y + w.
+template <int NT, int MODE>
+__global__ __launch_bounds__(kLanes * kWaves) void k_ncc_weighted(Dev d, int colour, int Qn, int nslot,
+                                                                  float *__restrict__ out) {
+    __shared__ SharedPatch<NT> S;
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
+    const int q = blockIdx.x * kLanes + lane;
+    const int slot = blockIdx.y * kWaves + wave;
+    int x = 0, y = 0;
+    const bool valid = q < Qn && colour_pixel(d, colour, q, x, y);
+    const int p = x + y * d.W;
+    bool any = false;
+    if (valid) any = MODE == MODE_REFINE ? d.cand_pos[q] >= 0 : d.post_flags[p] != 0;
+    fill_shared_patch<NT>(d, any, x, y, S);
+    unsigned long long hyps = 0, taps = 0;
+    float4 pl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float wn = 1.0f;
+    uint32_t sel = 0;
+    if (any && slot < nslot && weighted_slot<MODE>(d, Qn, q, p, slot, pl, wn, sel)) {
+        const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
+        out[(size_t)slot * Qn + q] = weighted_sum<NT, MODE>(d, &S, lane, x, y, slot, vw, sel, pl, wn, hyps, taps);
+    }
+    count_add(d.counters, hyps, taps);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_ncc_weighted_direct(Dev d, int colour, int Qn, float *__restrict__ out) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int slot = blockIdx.y;
     int x = 0, y = 0;
     bool active = q < Qn && colour_pixel(d, colour, q, x, y);
     const int p = x + y * d.W;
-    float4 pl = make_float4(0, 0, 0, 0);
-    uint32_t sel = 0;
-    float result = 0.0f;
-    const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
-    float wn = 1.0f;
-    if (active) {
-        if (MODE == MODE_REFINE) {
-            active = d.cand_pos[q] >= 0;
-            if (active) {
-                pl = d.rplane[(size_t)slot * Qn + q];
-                wn = d.wnorm[q];
-            }
-        } else {
-            const uint8_t fl = d.post_flags[p];
-            const bool dtw = slot < 61, lr = slot >= 61;
-            active = (dtw && (fl & 1)) || (lr && (fl & 2));
-            if (active) {
-                const dpe_camera &cam = d.cams[0];
-                const float4 op = d.post_op[p];
-                const float od = op.w;
-                const float bl = d.post_bl[p];
-                const float disp = d.post_disp[p];
-                wn = d.post_wn[p];
-                sel = d.selected[p];
-                float pd = od;
-                if (slot != 61) {
-                    const int k = slot < 61 ? slot - 30 : slot - 67;
-                    pd = cam.K[0] * bl / (disp + (float)k);
-                    if (pd < d.prm.depth_min || pd > d.prm.depth_max) {
-                        active = false;
-                        out[(size_t)slot * Qn + q] = slot < 61 ? 2.0f : __builtin_inff();
-                    }
-                }
-                pl = op;
-                pl.w = distance_to_origin(cam, x, y, pd, op);
-            }
-        }
-    }
     unsigned long long hyps = 0, taps = 0;
+    float4 pl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float wn = 1.0f;
+    uint32_t sel = 0;
+    if (active) active = weighted_slot<MODE>(d, Qn, q, p, slot, pl, wn, sel);
     if (active) {
-        float acc = 0.0f;
-        const bool geom = d.prm.geom_consistency != 0;
-        const float gf = d.prm.geom_factor;
-        if constexpr (NT > 0) {
-            Patch<NT> P;
-            load_patch<NT>(d, x, y, P);
-            for (int v = 1; v < d.N; ++v) {
-                const float wv = (float)vw[v - 1];
-                if (MODE == MODE_REFINE) {
-                    if (vw[v - 1] > 0) {
-                        acc += wv * ncc_patch<NT>(d, P, x, y, v, pl, taps);
-                        hyps++;
-                    }
-                } else if (isSet(sel, v - 1)) {
-                    const float c = ncc_patch<NT>(d, P, x, y, v, pl, taps);
-                    hyps++;
-                    if (slot <= 61) {
-                        float tc = c;
-                        if (geom) tc += gf * geom_cost(d, x, y, v, pl);
-                        acc += tc * wv;
-                    } else {
-                        acc += c * wv;
-                        if (geom) acc += gf * geom_cost(d, x, y, v, pl) * wv;
-                    }
-                }
-            }
-        } else {
-            for (int v = 1; v < d.N; ++v) {
-                const float wv = (float)vw[v - 1];
-                if (MODE == MODE_REFINE) {
-                    if (vw[v - 1] > 0) acc += wv * ncc_old_direct(d, x, y, v, pl, hyps, taps);
-                } else if (isSet(sel, v - 1)) {
-                    const float c = ncc_old_direct(d, x, y, v, pl, hyps, taps);
-                    if (slot <= 61) {
-                        float tc = c;
-                        if (geom) tc += gf * geom_cost(d, x, y, v, pl);
-                        acc += tc * wv;
-                    } else {
-                        acc += c * wv;
-                        if (geom) acc += gf * geom_cost(d, x, y, v, pl) * wv;
-                    }
-                }
-            }
-        }
-        acc /= wn;
-        if (MODE == MODE_POST && slot < 61) acc = fminf(2.0f, acc);
-        result = acc;
-        out[(size_t)slot * Qn + q] = result;
+        const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
+        out[(size_t)slot * Qn + q] = weighted_sum<0, MODE>(d, nullptr, 0, x, y, slot, vw, sel, pl, wn, hyps, taps);
     }
     count_add(d.counters, hyps, taps);
 }
@@ -497,14 +554,11 @@ __global__ void k_random_init_planes(Dev d) {
     if (d.prm.state == DPE_FIRST_INIT) {
         Rng rng(d.prm.seed, p, PH_RANDOM_INIT, d.prm.call_index);
         const float depth = rng.uniform() * (d.prm.depth_max - d.prm.depth_min) + d.prm.depth_min;
-        float4 pl = random_normal(cam, x, y, rng, depth);
-        pl.w = distance_to_origin(cam, x, y, depth, pl);
-        d.planes[p] = pl;
+        const float4 n = random_normal(cam, x, y, rng, depth);
+        d.planes[p] = make_float4(n.x, n.y, n.z, distance_to_origin(cam, x, y, depth, n));
     } else {
-        float4 pl = to_camera(cam, d.planes[p]);
-        const float depth = pl.w;
-        pl.w = distance_to_origin(cam, x, y, depth, pl);
-        d.planes[p] = pl;
+        const float4 n = to_camera(cam, d.planes[p]);
+        d.planes[p] = make_float4(n.x, n.y, n.z, distance_to_origin(cam, x, y, n.w, n));
     }
 }
 
@@ -912,9 +966,8 @@ __global__ void k_strong_select(Dev d, int colour, int iter) {
     const float depths[5] = {depth_rand, depth_now, depth_rand, depth_now, depth_pert};
     const float4 normals[5] = {plane_now, plane_rand, plane_rand, plane_pert, plane_now};
     for (int i = 0; i < 5; ++i) {
-        float4 tp = normals[i];
-        tp.w = distance_to_origin(cam, x, y, depths[i], tp);
-        d.rplane[(size_t)i * Q + q] = tp;
+        const float4 n = normals[i];
+        d.rplane[(size_t)i * Q + q] = make_float4(n.x, n.y, n.z, distance_to_origin(cam, x, y, depths[i], n));
     }
     d.now_plane[q] = plane_now;
     d.now_depth[q] = depth_now;
@@ -1092,25 +1145,72 @@ __global__ void k_gen_edge_inform(Dev d) {
     }
 }
 
-// FindNearestStrongPoint, DPE.cu:2859-2893
-__global__ void k_find_nearest_strong(Dev d) {
+// FindNearestStrongPoint, DPE.cu:2859-2893: the first STRONG pixel on the
+// Chebyshev rings r = 0..100 around a WEAK pixel, rings scanned x-major
+// (x = -r..r, then y = -r..r, ring cells only).  Restated exactly in two steps:
+// the ring radius r* = min over STRONG q of max(|dx|, |dy|) comes from a
+// row-wise distance map g (k_row_strong_distance), then only ring r* is scanned
+// in the reference's order.  Cost O(r*) per pixel instead of O(r*^2).
+__global__ void k_row_strong_distance(Dev d, int16_t *__restrict__ g) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= d.H) return;
+    const int W = d.W;
+    const uint8_t *wk = d.weak + (size_t)y * W;
+    int16_t *gr = g + (size_t)y * W;
+    int last = -1000;
+    for (int x = 0; x < W; ++x) {
+        if (wk[x] == DPE_STRONG) last = x;
+        gr[x] = (int16_t)min(x - last, 127);
+    }
+    last = 1000000;
+    for (int x = W - 1; x >= 0; --x) {
+        if (wk[x] == DPE_STRONG) last = x;
+        gr[x] = (int16_t)min((int)gr[x], min(last - x, 127));
+    }
+}
+
+__global__ void k_find_nearest_strong(Dev d, const int16_t *__restrict__ g) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.P) return;
     const int W = d.W, H = d.H;
     const int x = p % W, y = p / W;
-    d.nearest[p] = make_short2(-1, -1);
-    if (d.weak[p] != DPE_WEAK) return;
-    for (int r = 0; r <= 100; ++r)
-        for (int i = -r; i <= r; ++i)
-            for (int j = -r; j <= r; ++j) {
-                if (abs(i) != r && abs(j) != r) continue;
-                const int nx = x + i, ny = y + j;
-                if (nx < 0 || ny < 0 || nx >= W || ny >= H) continue;
-                if (d.weak[nx + ny * W] == DPE_STRONG) {
-                    d.nearest[p] = make_short2((short)nx, (short)ny);
-                    return;
+    short2 result = make_short2(-1, -1);
+    if (d.weak[p] == DPE_WEAK) {
+        const int max_radius = 100;
+        int best = max_radius + 1;
+        for (int k = 0; k <= max_radius && k < best; ++k) {
+            if (y - k >= 0) best = min(best, max(k, (int)g[(size_t)(y - k) * W + x]));
+            if (k > 0 && y + k < H) best = min(best, max(k, (int)g[(size_t)(y + k) * W + x]));
+        }
+        if (best <= max_radius) {
+            const int r = best;
+            bool found = false;
+            for (int i = -r; i <= r && !found; ++i) {
+                const int nx = x + i;
+                if (nx < 0 || nx >= W) continue;
+                if (abs(i) == r) {
+                    for (int j = -r; j <= r; ++j) {
+                        const int ny = y + j;
+                        if (ny < 0 || ny >= H) continue;
+                        if (d.weak[nx + ny * W] == DPE_STRONG) {
+                            result = make_short2((short)nx, (short)ny);
+                            found = true;
+                            break;
+                        }
+                    }
+                } else {
+                    if (y - r >= 0 && d.weak[nx + (y - r) * W] == DPE_STRONG) {
+                        result = make_short2((short)nx, (short)(y - r));
+                        found = true;
+                    } else if (y + r < H && d.weak[nx + (y + r) * W] == DPE_STRONG) {
+                        result = make_short2((short)nx, (short)(y + r));
+                        found = true;
+                    }
                 }
             }
+        }
+    }
+    d.nearest[p] = result;
 }
 
 // GenNeighbours, DPE.cu:2107-2467
@@ -1704,8 +1804,8 @@ __global__ __launch_bounds__(256) void k_weak_update(Dev d, int colour, int iter
         const float depths[5] = {depth_rand, depth_now, depth_rand, depth_now, depth_pert};
         const float4 normals[5] = {plane_now, plane_rand, plane_rand, plane_pert, plane_now};
         for (int i = 0; i < 5; ++i) {
-            float4 tp = normals[i];
-            tp.w = distance_to_origin(cam, x, y, depths[i], tp);
+            const float4 n = normals[i];
+            const float4 tp = make_float4(n.x, n.y, n.z, distance_to_origin(cam, x, y, depths[i], n));
             const float tc = weighted(tp);
             const float db = depth_from_plane(cam, tp, x, y);
             if (db >= dmin && db <= dmax && tc < cost_now) {
@@ -1739,9 +1839,8 @@ __global__ void k_depth_and_normal(Dev d) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.P) return;
     const int x = p % d.W, y = p / d.W;
-    float4 pl = d.planes[p];
-    pl.w = depth_from_plane(d.cams[0], pl, x, y);
-    d.planes[p] = to_world(d.cams[0], pl);
+    const float4 pl = d.planes[p];
+    d.planes[p] = to_world(d.cams[0], make_float4(pl.x, pl.y, pl.z, depth_from_plane(d.cams[0], pl, x, y)));
 }
 
 // CheckerboardFilterStrong, DPE.cu:1961-2105
@@ -1829,6 +1928,38 @@ __global__ void k_post_prep(Dev d) {
     d.post_wn[p] = wn;
 }
 
+// Sample planes of DepthToWeak (DPE.cu:2667-2675) and LocalRefine
+// (DPE.cu:2813-2819): slot 0..60 disparity offsets -30..30, slot 61 the current
+// depth, slots 62..72 offsets -5..5.  A sample outside the depth range gets its
+// final cost here (DepthToWeak 2.0; LocalRefine +inf, i.e. skipped) and w = NaN.
+__global__ void k_post_planes(Dev d) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    const uint8_t fl = d.post_flags[p];
+    if (!fl) return;
+    const int x = p % d.W, y = p / d.W;
+    const dpe_camera &cam = d.cams[0];
+    const float4 op = d.post_op[p];
+    const float bl = d.post_bl[p];
+    const float disp = d.post_disp[p];
+    const size_t P = d.P;
+    for (int slot = 0; slot < kPostSlots; ++slot) {
+        const bool dtw = slot < 61;
+        if (!((dtw && (fl & 1)) || (!dtw && (fl & 2)))) continue;
+        float pd = op.w;
+        if (slot != 61) {
+            const int k = dtw ? slot - 30 : slot - 67;
+            pd = cam.K[0] * bl / (disp + (float)k);
+            if (pd < d.prm.depth_min || pd > d.prm.depth_max) {
+                d.pcost[(size_t)slot * P + p] = dtw ? 2.0f : __builtin_inff();
+                d.post_planes[(size_t)slot * P + p] = make_float4(0.0f, 0.0f, 0.0f, __builtin_nanf(""));
+                continue;
+            }
+        }
+        d.post_planes[(size_t)slot * P + p] = make_float4(op.x, op.y, op.z, distance_to_origin(cam, x, y, pd, op));
+    }
+}
+
 // Peak analysis of DepthToWeak (DPE.cu:2699-2750) and LocalRefine's update
 // (DPE.cu:2811-2838).
 __global__ void k_post_final(Dev d) {
@@ -1903,7 +2034,9 @@ void launch_gen_edge_inform(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_gen_edge_inform, grid1(d.P), dim3(256), 0, s, d);
 }
 void launch_find_nearest_strong(const Dev &d, hipStream_t s) {
-    hipLaunchKernelGGL(k_find_nearest_strong, grid1(d.P), dim3(256), 0, s, d);
+    int16_t *g = (int16_t *)d.pcost;  // the post-pass scratch is free before the iterations
+    hipLaunchKernelGGL(k_row_strong_distance, grid1(d.H, 64), dim3(64), 0, s, d, g);
+    hipLaunchKernelGGL(k_find_nearest_strong, grid1(d.P), dim3(256), 0, s, d, (const int16_t *)g);
 }
 void launch_gen_neighbours(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_gen_neighbours, grid1(d.P, 64), dim3(64), 0, s, d);
@@ -1915,17 +2048,17 @@ void launch_random_init_planes(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_random_init_planes, grid1(d.P), dim3(256), 0, s, d);
 }
 
-template <class F6, class F0>
-static void ncc_dispatch(const Dev &d, F6 f6, F0 f0) {
-    if (d.ntap == 6) f6();
-    else f0();
-}
+static inline dim3 grid_shared(int n, int nslot) { return dim3((n + kLanes - 1) / kLanes); }
+static inline dim3 grid_shared_slots(int n, int nslot) { return dim3((n + kLanes - 1) / kLanes, (nslot + kWaves - 1) / kWaves); }
+static inline dim3 grid_direct(int n, int nslot) { return dim3((n + 255) / 256, nslot); }
 
 void launch_ncc_current(const Dev &d, hipStream_t s) {
-    dim3 g((d.P + 255) / 256, 1);
-    ncc_dispatch(
-        d, [&] { hipLaunchKernelGGL(k_ncc_full<6>, g, dim3(256), 0, s, d, -1, d.P, nullptr, d.planes, d.cost_mat); },
-        [&] { hipLaunchKernelGGL(k_ncc_full<0>, g, dim3(256), 0, s, d, -1, d.P, nullptr, d.planes, d.cost_mat); });
+    if (d.ntap == 6)
+        hipLaunchKernelGGL(k_ncc_full<6>, grid_shared(d.P, 1), dim3(kLanes * kWaves), 0, s, d, -1, d.P, 1, nullptr,
+                           d.planes, d.cost_mat);
+    else
+        hipLaunchKernelGGL(k_ncc_full_direct, grid_direct(d.P, 1), dim3(256), 0, s, d, -1, d.P, nullptr, d.planes,
+                           d.cost_mat);
 }
 void launch_random_init_costs(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_random_init_costs, grid1(d.P), dim3(256), 0, s, d);
@@ -1935,20 +2068,23 @@ void launch_strong_candidates(const Dev &d, int colour, int iter, hipStream_t s)
 }
 void launch_ncc_strong(const Dev &d, int colour, hipStream_t s) {
     const int nslot = d.prm.use_edge ? kStrongSlots : 9;
-    dim3 g((d.Q + 255) / 256, nslot);
-    ncc_dispatch(
-        d,
-        [&] { hipLaunchKernelGGL(k_ncc_full<6>, g, dim3(256), 0, s, d, colour, d.Q, d.cand_pos, d.planes, d.cost_mat); },
-        [&] { hipLaunchKernelGGL(k_ncc_full<0>, g, dim3(256), 0, s, d, colour, d.Q, d.cand_pos, d.planes, d.cost_mat); });
+    if (d.ntap == 6)
+        hipLaunchKernelGGL(k_ncc_full<6>, grid_shared(d.Q, nslot), dim3(kLanes * kWaves), 0, s, d, colour, d.Q, nslot,
+                           d.cand_pos, d.planes, d.cost_mat);
+    else
+        hipLaunchKernelGGL(k_ncc_full_direct, grid_direct(d.Q, nslot), dim3(256), 0, s, d, colour, d.Q, d.cand_pos,
+                           d.planes, d.cost_mat);
 }
 void launch_strong_select(const Dev &d, int colour, int iter, hipStream_t s) {
     hipLaunchKernelGGL(k_strong_select, grid1(d.Q, 64), dim3(64), 0, s, d, colour, iter);
 }
 void launch_ncc_refine(const Dev &d, int colour, hipStream_t s) {
-    dim3 g((d.Q + 255) / 256, kRefineSlots);
-    ncc_dispatch(
-        d, [&] { hipLaunchKernelGGL((k_ncc_weighted<6, MODE_REFINE>), g, dim3(256), 0, s, d, colour, d.Q, d.rcost); },
-        [&] { hipLaunchKernelGGL((k_ncc_weighted<0, MODE_REFINE>), g, dim3(256), 0, s, d, colour, d.Q, d.rcost); });
+    if (d.ntap == 6)
+        hipLaunchKernelGGL((k_ncc_weighted<6, MODE_REFINE>), grid_shared_slots(d.Q, kRefineSlots), dim3(kLanes * kWaves), 0,
+                           s, d, colour, d.Q, kRefineSlots, d.rcost);
+    else
+        hipLaunchKernelGGL(k_ncc_weighted_direct<MODE_REFINE>, grid_direct(d.Q, kRefineSlots), dim3(256), 0, s, d,
+                           colour, d.Q, d.rcost);
 }
 void launch_strong_accept(const Dev &d, int colour, hipStream_t s) {
     hipLaunchKernelGGL(k_strong_accept, grid1(d.Q), dim3(256), 0, s, d, colour);
@@ -1968,20 +2104,27 @@ void launch_filter(const Dev &d, int colour, hipStream_t s) {
 void launch_post_prep(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_post_prep, grid1(d.P), dim3(256), 0, s, d);
 }
+void launch_post_planes(const Dev &d, hipStream_t s) {
+    hipLaunchKernelGGL(k_post_planes, grid1(d.P), dim3(256), 0, s, d);
+}
 void launch_ncc_post(const Dev &d, hipStream_t s) {
-    dim3 g((d.P + 255) / 256, kPostSlots);
-    ncc_dispatch(
-        d, [&] { hipLaunchKernelGGL((k_ncc_weighted<6, MODE_POST>), g, dim3(256), 0, s, d, -1, d.P, d.pcost); },
-        [&] { hipLaunchKernelGGL((k_ncc_weighted<0, MODE_POST>), g, dim3(256), 0, s, d, -1, d.P, d.pcost); });
+    if (d.ntap == 6)
+        hipLaunchKernelGGL((k_ncc_weighted<6, MODE_POST>), grid_shared_slots(d.P, kPostSlots), dim3(kLanes * kWaves), 0, s,
+                           d, -1, d.P, kPostSlots, d.pcost);
+    else
+        hipLaunchKernelGGL(k_ncc_weighted_direct<MODE_POST>, grid_direct(d.P, kPostSlots), dim3(256), 0, s, d, -1, d.P,
+                           d.pcost);
 }
 void launch_post_final(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_post_final, grid1(d.P), dim3(256), 0, s, d);
 }
 void launch_ncc_given(const Dev &d, int num_hyps, const float4 *planes, float *costs, hipStream_t s) {
-    dim3 g((d.P + 255) / 256, num_hyps);
-    ncc_dispatch(
-        d, [&] { hipLaunchKernelGGL(k_ncc_full<6>, g, dim3(256), 0, s, d, -1, d.P, nullptr, planes, costs); },
-        [&] { hipLaunchKernelGGL(k_ncc_full<0>, g, dim3(256), 0, s, d, -1, d.P, nullptr, planes, costs); });
+    if (d.ntap == 6)
+        hipLaunchKernelGGL(k_ncc_full<6>, grid_shared(d.P, num_hyps), dim3(kLanes * kWaves), 0, s, d, -1, d.P,
+                           num_hyps, nullptr, planes, costs);
+    else
+        hipLaunchKernelGGL(k_ncc_full_direct, grid_direct(d.P, num_hyps), dim3(256), 0, s, d, -1, d.P, nullptr, planes,
+                           costs);
 }
 
 }  // namespace dpe

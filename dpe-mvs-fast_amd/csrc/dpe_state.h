@@ -67,6 +67,7 @@ struct Dev {
     float *post_wn;
     uint8_t *post_flags;   // bit0 DepthToWeak evaluates, bit1 LocalRefine evaluates
     float *pcost;          // [kPostSlots][P]
+    float4 *post_planes;   // [kPostSlots][P] sample planes (w = NaN: not evaluated)
     // GenNeighbours constants (DPE.cu:2151-2157, host double math)
     float gn_cos, gn_sin, gn_thr;
     int gn_shift;
@@ -93,6 +94,7 @@ void launch_weak_half(const Dev &d, int colour, int iter, hipStream_t s);
 void launch_depth_and_normal(const Dev &d, hipStream_t s);
 void launch_filter(const Dev &d, int colour, hipStream_t s);
 void launch_post_prep(const Dev &d, hipStream_t s);
+void launch_post_planes(const Dev &d, hipStream_t s);
 void launch_ncc_post(const Dev &d, hipStream_t s);
 void launch_post_final(const Dev &d, hipStream_t s);
 void launch_ncc_given(const Dev &d, int num_hyps, const float4 *planes, float *costs, hipStream_t s);
