@@ -43,7 +43,7 @@ enum BufId {
     B_LABEL, B_CANDPOS, B_COSTMAT, B_NSEL, B_RPLANE, B_RCOST, B_NOWPLANE, B_NOWDEPTH, B_NOWCOST, B_WNORM, B_POSTOP,
     B_POSTBL, B_POSTDISP, B_POSTWN, B_POSTFLAGS, B_PCOST, B_COUNTERS, B_GIVEN_PLANES, B_GIVEN_COSTS,
     B_F_VIEWS, B_F_DEPTH, B_F_NORMAL, B_F_STATES, B_F_COLOR, B_F_MASK, B_F_CLAIM, B_F_USED0, B_F_USED1, B_F_ACC0,
-    B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_POSTPLANES, B_COUNT
+    B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_POSTPLANES, B_WEAKLIST, B_COUNT
 };
 
 struct Fail {
@@ -280,6 +280,30 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *i
     int weak_count = 0;
     for (size_t i = 0; i < P; ++i)
         if (weak_h[i] == DPE_WEAK) nmap_h[i] = weak_count++;
+    // weak pixel lists: all (GenNeighbours, RANSAC) and per colour as the HALF
+    // launch covers them (weak propagation, DPE.cu:1868-1902)
+    std::vector<int> wl;
+    wl.reserve((size_t)weak_count * 2 + 2);
+    for (size_t i = 0; i < P; ++i)
+        if (weak_h[i] == DPE_WEAK) wl.push_back((int)i);
+    int cc[2] = {0, 0};
+    for (int c = 0; c < 2; ++c) {
+        for (size_t i = 0; i < P; ++i) {
+            const int x = (int)(i % d.W), y = (int)(i / d.W);
+            if (weak_h[i] == DPE_WEAK && y < d.half_rows && ((x + y) & 1) == c) {
+                wl.push_back((int)i);
+                cc[c]++;
+            }
+        }
+    }
+    int *dwl = (int *)buffer(ctx, B_WEAKLIST, sizeof(int) * (wl.size() + 1));
+    if (!wl.empty()) HIPCHK(hipMemcpyAsync(dwl, wl.data(), sizeof(int) * wl.size(), hipMemcpyHostToDevice, ctx->stream));
+    d.weak_list = dwl;
+    d.weak_count = weak_count;
+    d.weak_colour_list[0] = dwl + weak_count;
+    d.weak_colour_count[0] = cc[0];
+    d.weak_colour_list[1] = dwl + weak_count + cc[0];
+    d.weak_colour_count[1] = cc[1];
     d.weak = (uint8_t *)buffer(ctx, B_WEAK, P);
     int *nmap = (int *)buffer(ctx, B_NMAP, sizeof(int) * P);
     HIPCHK(hipMemcpyAsync(d.weak, weak_h.data(), P, hipMemcpyHostToDevice, ctx->stream));

@@ -67,7 +67,7 @@ __global__ void k_make_quads(const float *__restrict__ img, float4 *__restrict__
 // over hypothesis slots of the same 64 pixels, reading the weights from LDS.
 // Taps of one window row evaluated together (registers vs. loads in flight).
 #ifndef DPE_TAP_UNROLL
-#define DPE_TAP_UNROLL 3
+#define DPE_TAP_UNROLL 6
 #endif
 constexpr int kLanes = 64;
 constexpr int kWaves = 4;
@@ -1215,8 +1215,9 @@ __global__ void k_find_nearest_strong(Dev d, const int16_t *__restrict__ g) {
 
 // GenNeighbours, DPE.cu:2107-2467
 __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= d.P) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.weak_count) return;
+    const int p = d.weak_list[i];
     if (d.weak[p] != DPE_WEAK) return;
     const int W = d.W, H = d.H;
     const int px = p % W, py = p / W;
@@ -1514,15 +1515,19 @@ __global__ void k_neighbour_update(Dev d) {
 }
 
 // RANSACToGetFitPlane, DPE.cu:2895-3128
-__global__ void k_ransac(Dev d, int iter) {
+__global__ void k_fit_copy(Dev d) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.P) return;
+    if (d.weak[p] != DPE_WEAK) d.fit[p] = d.planes[p];
+}
+
+__global__ void k_ransac(Dev d, int iter) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.weak_count) return;
+    const int p = d.weak_list[i];
     const int W = d.W;
     const int px = p % W, py = p / W;
-    if (d.weak[p] != DPE_WEAK) {
-        d.fit[p] = d.planes[p];
-        return;
-    }
+    if (d.weak[p] != DPE_WEAK) return;
     Rng rng(d.prm.seed, p, ph_ransac(iter), d.prm.call_index);
     const dpe_camera &cam = d.cams[0];
     bool edge_limit = false;
@@ -1678,11 +1683,11 @@ __global__ void k_ransac(Dev d, int iter) {
 
 // CheckerboardPropagationWeak + PlaneHypothesisRefinementWeak, DPE.cu:1672-1866, 1124-1216
 __global__ __launch_bounds__(256) void k_weak_update(Dev d, int colour, int iter) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    int x, y;
-    if (!colour_pixel(d, colour, q, x, y)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.weak_colour_count[colour]) return;
     const int W = d.W;
-    const int p = x + y * W;
+    const int p = d.weak_colour_list[colour][i];
+    const int x = p % W, y = p / W;
     if (d.weak[p] != DPE_WEAK) return;
     const int NV = d.NV;
     const dpe_camera &cam = d.cams[0];
@@ -2039,7 +2044,7 @@ void launch_find_nearest_strong(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_find_nearest_strong, grid1(d.P), dim3(256), 0, s, d, (const int16_t *)g);
 }
 void launch_gen_neighbours(const Dev &d, hipStream_t s) {
-    hipLaunchKernelGGL(k_gen_neighbours, grid1(d.P, 64), dim3(64), 0, s, d);
+    if (d.weak_count > 0) hipLaunchKernelGGL(k_gen_neighbours, grid1(d.weak_count, 64), dim3(64), 0, s, d);
 }
 void launch_neighbour_update(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_neighbour_update, grid1(d.P), dim3(256), 0, s, d);
@@ -2090,10 +2095,12 @@ void launch_strong_accept(const Dev &d, int colour, hipStream_t s) {
     hipLaunchKernelGGL(k_strong_accept, grid1(d.Q), dim3(256), 0, s, d, colour);
 }
 void launch_ransac(const Dev &d, int iter, hipStream_t s) {
-    hipLaunchKernelGGL(k_ransac, grid1(d.P, 64), dim3(64), 0, s, d, iter);
+    hipLaunchKernelGGL(k_fit_copy, grid1(d.P), dim3(256), 0, s, d);
+    if (d.weak_count > 0) hipLaunchKernelGGL(k_ransac, grid1(d.weak_count, 64), dim3(64), 0, s, d, iter);
 }
 void launch_weak_half(const Dev &d, int colour, int iter, hipStream_t s) {
-    hipLaunchKernelGGL(k_weak_update, grid1(d.Q, 64), dim3(64), 0, s, d, colour, iter);
+    if (d.weak_colour_count[colour] > 0)
+        hipLaunchKernelGGL(k_weak_update, grid1(d.weak_colour_count[colour], 64), dim3(64), 0, s, d, colour, iter);
 }
 void launch_depth_and_normal(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_depth_and_normal, grid1(d.P), dim3(256), 0, s, d);

@@ -68,6 +68,11 @@ struct Dev {
     uint8_t *post_flags;   // bit0 DepthToWeak evaluates, bit1 LocalRefine evaluates
     float *pcost;          // [kPostSlots][P]
     float4 *post_planes;   // [kPostSlots][P] sample planes (w = NaN: not evaluated)
+    // pixels that are WEAK at the start of the call (raster order), all and per colour
+    const int *weak_list;
+    int weak_count;
+    const int *weak_colour_list[2];
+    int weak_colour_count[2];
     // GenNeighbours constants (DPE.cu:2151-2157, host double math)
     float gn_cos, gn_sin, gn_thr;
     int gn_shift;
