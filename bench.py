@@ -26,6 +26,13 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 BYTES_PER_TAP = 20     # one 4-B reference texel + one 16-B bilinear source quad
+FAMILY_NAMES = ["k_ncc_full<6>", "k_ncc_weighted<6,REFINE>", "k_ncc_weighted<6,POST>", "k_weak_update"]
+# HBM traffic of one k_ncc_full<6> launch on this workload, from two rocprofv3
+# --pmc passes (FETCH_SIZE, WRITE_SIZE) over `python bench.py --no-cpu-baseline`,
+# each scaled by the counter's reading for a 256 MiB device copy (FETCH_SIZE x2,
+# WRITE_SIZE x1); the runs and the arithmetic are in
+# profiles/ncc_full_c2_hbm_traffic.md.  Counters cannot be read inside a timed run.
+NCC_FULL_TRAFFIC_BYTES = None
 
 
 def parse():
@@ -133,15 +140,19 @@ def main():
     fparams = abi.fuse_defaults(min_consistent=3, memory=abi.MEM_DEVICE)
 
     def step():
-        tot = dict(hyps=0, taps=0, ncc_ms=0.0, gpu_ms=0.0, launches=0)
+        tot = dict(hyps=0, taps=0, gpu_ms=0.0, fam_ms=[0.0] * 4, fam_hyps=[0] * 4, fam_taps=[0] * 4,
+                   fam_launches=[0] * 4)
         for first in (True, False):
             for v in range(V):
                 st = call(v, first)
                 tot["hyps"] += st.ncc_hyps
                 tot["taps"] += st.ncc_taps
-                tot["ncc_ms"] += st.ncc_ms
                 tot["gpu_ms"] += st.gpu_ms
-                tot["launches"] += st.ncc_launches
+                for f in range(4):
+                    tot["fam_ms"][f] += st.family_ms[f]
+                    tot["fam_hyps"][f] += st.family_hyps[f]
+                    tot["fam_taps"][f] += st.family_taps[f]
+                    tot["fam_launches"][f] += st.family_launches[f]
             print(f"[bench] {'first' if first else 'geometric'} calls done", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         n = C.c_int64()
@@ -159,15 +170,25 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     hyps = sum(t["hyps"] for t in totals)
     taps = sum(t["taps"] for t in totals)
-    ncc_ms = sum(t["ncc_ms"] for t in totals)
-    launches = sum(t["launches"] for t in totals)
+    fam = {k: [sum(t[k][f] for t in totals) for f in range(4)]
+           for k in ("fam_ms", "fam_hyps", "fam_taps", "fam_launches")}
     # accuracy of the final depth maps against ground truth
     acc = []
     for v in range(V):
         gt = torch.from_numpy(sc.views[v].depth).to(dev)
         rel = (depth[v] - gt).abs() / gt
         acc.append(float((rel < 0.01).float().mean()))
-    achieved = taps * BYTES_PER_TAP / (ncc_ms * 1e-3) / 1e9
+    # roofline of the dominant kernel, k_ncc_full<6> (candidate cost vectors):
+    # algorithmic bytes per launch (20 B per tap) / its average launch time,
+    # both from the hipEvent pairs and exact tap counters of the timed steps
+    full_launches = max(fam["fam_launches"][0], 1)
+    full_avg_ms = fam["fam_ms"][0] / full_launches
+    full_bytes = fam["fam_taps"][0] * BYTES_PER_TAP / full_launches
+    achieved = full_bytes / (full_avg_ms * 1e-3) / 1e9
+    families = {FAMILY_NAMES[f]: {"ms": fam["fam_ms"][f], "launches": fam["fam_launches"][f],
+                                  "hyps": fam["fam_hyps"][f], "taps": fam["fam_taps"][f],
+                                  "hyp_per_s": fam["fam_hyps"][f] / max(fam["fam_ms"][f] * 1e-3, 1e-12)}
+                for f in range(4)}
     result = {
         "metric": "NCC plane-hyp/s + s/depth-map @1600x1200,10 src; %px |Δd|/d<1% vs plain",
         "value": hyps / elapsed,
@@ -191,10 +212,10 @@ def main():
         "ncc_hyps_per_step": hyps // args.steps,
         "ncc_taps_per_step": taps // args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "ncc (k_ncc_full / k_ncc_weighted)",
-                     "launch_avg_ms": ncc_ms / max(launches, 1),
-                     "bytes_per_launch": taps * BYTES_PER_TAP / max(launches, 1)},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": NCC_FULL_TRAFFIC_BYTES,
+                     "kernel": "k_ncc_full<6>", "launch_avg_ms": full_avg_ms, "bytes_per_launch": full_bytes,
+                     "launches": fam["fam_launches"][0]},
+        "ncc_families": families,
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(sc, args.cpu_threads)
@@ -226,9 +247,12 @@ def cpu_baseline(sc, threads):
                            use_edge=0, use_limit=0, use_label=0, geom_consistency=0)
     pin = host.ProblemInputs(imgs, cams, prm)
     lib = oracle_lib.load_oracle()
+    fam = []
     t = time.perf_counter()
-    _, (hyps, taps) = oracle_lib.run_oracle(lib, pin, nthreads)
+    oracle_lib.run_oracle(lib, pin, nthreads, fam)
     dt = time.perf_counter() - t
+    # the GPU's unit: every family but the evaluations the reference never reads
+    hyps = sum(h for h, _ in fam[:abi.NCC_FAMILIES])
     return {"value": hyps / dt, "unit": "NCC plane-hyp/s", "cores": nthreads, "kind": "port",
             "sample": f"oracle FIRST_INIT call, 1 iteration, {cw}x{ch} centre crop of view {ref} + 10 sources, "
                       f"{hyps} NCC-hyps in {dt:.1f}s"}

@@ -227,7 +227,10 @@ void copy_out(dpe_ctx *ctx, void *dst, const void *src, size_t bytes, bool devic
 struct EventTimer {
     dpe_ctx *ctx;
     int next = 0;
-    std::vector<std::pair<int, int>> ncc;
+    struct Span {
+        int a, b, family;
+    };
+    std::vector<Span> ncc;
     hipEvent_t get() {
         if (next >= (int)ctx->events.size()) {
             hipEvent_t e;
@@ -237,16 +240,38 @@ struct EventTimer {
         return ctx->events[next++];
     }
     template <class F>
-    void ncc_launch(F f) {
+    void ncc_launch(int family, F f) {
         const int a = next;
         HIPCHK(hipEventRecord(get(), ctx->stream));
         f();
         HIPCHK(hipGetLastError());
         const int b = next;
         HIPCHK(hipEventRecord(get(), ctx->stream));
-        ncc.push_back({a, b});
+        ncc.push_back({a, b, family});
+    }
+    // Per-family launch times and the counters of the NCC kernels into `stt`;
+    // gpu_ms is left to the caller.  Needs the stream synchronised.
+    void fill(dpe_stats &stt, const unsigned long long *cnt) {
+        for (const Span &sp : ncc) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ctx->events[sp.a], ctx->events[sp.b]));
+            stt.family_ms[sp.family] += ms;
+            stt.family_launches[sp.family]++;
+            if (sp.family != kFamWeak) {
+                stt.ncc_ms += ms;
+                stt.ncc_launches++;
+            }
+        }
+        for (int f = 0; f < kNumFamilies; ++f) {
+            stt.family_hyps[f] = (int64_t)cnt[2 * f];
+            stt.family_taps[f] = (int64_t)cnt[2 * f + 1];
+            stt.ncc_hyps += stt.family_hyps[f];
+            stt.ncc_taps += stt.family_taps[f];
+        }
     }
 };
+
+constexpr int kNumCounters = 2 * kNumFamilies;
 
 dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in, dpe_outputs *out) {
     dpe_status st = validate(ctx, prm, in);
@@ -373,8 +398,8 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *i
     d.post_flags = (uint8_t *)buffer(ctx, B_POSTFLAGS, P);
     d.pcost = (float *)buffer(ctx, B_PCOST, sizeof(float) * kPostSlots * P);
     d.post_planes = (float4 *)buffer(ctx, B_POSTPLANES, sizeof(float4) * kPostSlots * P);
-    d.counters = (unsigned long long *)buffer(ctx, B_COUNTERS, sizeof(unsigned long long) * 2);
-    HIPCHK(hipMemsetAsync(d.counters, 0, sizeof(unsigned long long) * 2, ctx->stream));
+    d.counters = (unsigned long long *)buffer(ctx, B_COUNTERS, sizeof(unsigned long long) * kNumCounters);
+    HIPCHK(hipMemsetAsync(d.counters, 0, sizeof(unsigned long long) * kNumCounters, ctx->stream));
     // GenNeighbours constants (DPE.cu:2151-2157): host double math
     {
         const float angle = 45.0f / prm->rotate_time;
@@ -392,20 +417,20 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *i
     launch_gen_neighbours(d, s);
     launch_neighbour_update(d, s);
     launch_random_init_planes(d, s);
-    tm.ncc_launch([&] { launch_ncc_current(d, s); });
+    tm.ncc_launch(kFamFull, [&] { launch_ncc_current(d, s); });
     launch_random_init_costs(d, s);
     for (int it = 0; it < prm->max_iterations; ++it) {
         for (int c = 0; c < 2; ++c) {
             launch_strong_candidates(d, c, it, s);
-            tm.ncc_launch([&] { launch_ncc_strong(d, c, s); });
+            tm.ncc_launch(kFamFull, [&] { launch_ncc_strong(d, c, s); });
             launch_strong_select(d, c, it, s);
-            tm.ncc_launch([&] { launch_ncc_refine(d, c, s); });
+            tm.ncc_launch(kFamRefine, [&] { launch_ncc_refine(d, c, s); });
             launch_strong_accept(d, c, s);
         }
         launch_ransac(d, it, s);
         if (weak_count > 0) {
-            launch_weak_half(d, 0, it, s);
-            launch_weak_half(d, 1, it, s);
+            for (int c = 0; c < 2; ++c)
+                if (d.weak_colour_count[c] > 0) tm.ncc_launch(kFamWeak, [&] { launch_weak_half(d, c, it, s); });
         }
     }
     launch_depth_and_normal(d, s);
@@ -413,7 +438,7 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *i
     launch_filter(d, 1, s);
     launch_post_prep(d, s);
     launch_post_planes(d, s);
-    tm.ncc_launch([&] { launch_ncc_post(d, s); });
+    tm.ncc_launch(kFamPost, [&] { launch_ncc_post(d, s); });
     launch_post_final(d, s);
     HIPCHK(hipGetLastError());
     hipEvent_t ev_stop = tm.get();
@@ -424,19 +449,12 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *i
     copy_out(ctx, out->costs, d.costs, sizeof(float) * P, dev_out);
     copy_out(ctx, out->pixel_states, d.weak, P, dev_out);
     copy_out(ctx, out->selected_views, d.selected, sizeof(uint32_t) * P, dev_out);
-    unsigned long long cnt[2] = {0, 0};
+    unsigned long long cnt[kNumCounters] = {};
     HIPCHK(hipMemcpyAsync(cnt, d.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     dpe_stats stt{};
     HIPCHK(hipEventElapsedTime(&stt.gpu_ms, ev_start, ev_stop));
-    for (auto &pr : tm.ncc) {
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, ctx->events[pr.first], ctx->events[pr.second]));
-        stt.ncc_ms += ms;
-    }
-    stt.ncc_launches = (int)tm.ncc.size();
-    stt.ncc_hyps = (int64_t)cnt[0];
-    stt.ncc_taps = (int64_t)cnt[1];
+    tm.fill(stt, cnt);
     ctx->stats = stt;
     return DPE_OK;
 }
@@ -463,19 +481,16 @@ dpe_status ncc_cost_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *
         pl = gp;
         co = gc;
     }
-    d.counters = (unsigned long long *)buffer(ctx, B_COUNTERS, sizeof(unsigned long long) * 2);
-    HIPCHK(hipMemsetAsync(d.counters, 0, sizeof(unsigned long long) * 2, ctx->stream));
-    tm.ncc_launch([&] { launch_ncc_given(d, num_hyps, pl, co, ctx->stream); });
+    d.counters = (unsigned long long *)buffer(ctx, B_COUNTERS, sizeof(unsigned long long) * kNumCounters);
+    HIPCHK(hipMemsetAsync(d.counters, 0, sizeof(unsigned long long) * kNumCounters, ctx->stream));
+    tm.ncc_launch(kFamFull, [&] { launch_ncc_given(d, num_hyps, pl, co, ctx->stream); });
     if (!dev) HIPCHK(hipMemcpyAsync(costs, co, sizeof(float) * P * num_hyps * d.NV, hipMemcpyDeviceToHost, ctx->stream));
-    unsigned long long cnt[2];
+    unsigned long long cnt[kNumCounters] = {};
     HIPCHK(hipMemcpyAsync(cnt, d.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     dpe_stats stt{};
-    HIPCHK(hipEventElapsedTime(&stt.ncc_ms, ctx->events[tm.ncc[0].first], ctx->events[tm.ncc[0].second]));
+    tm.fill(stt, cnt);
     stt.gpu_ms = stt.ncc_ms;
-    stt.ncc_launches = 1;
-    stt.ncc_hyps = (int64_t)cnt[0];
-    stt.ncc_taps = (int64_t)cnt[1];
     ctx->stats = stt;
     return DPE_OK;
 }

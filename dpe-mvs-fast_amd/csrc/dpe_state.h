@@ -76,7 +76,16 @@ struct Dev {
     // GenNeighbours constants (DPE.cu:2151-2157, host double math)
     float gn_cos, gn_sin, gn_thr;
     int gn_shift;
-    unsigned long long *counters;  // [0] NCC hyps, [1] taps
+    unsigned long long *counters;  // [2*family] NCC hyps, [2*family+1] taps; family = NccFamily
+};
+
+// NCC kernel families, each counted and timed on its own (dpe_stats.family_*).
+enum NccFamily {
+    kFamFull = 0,    // k_ncc_full: initial cost vectors and propagation candidates (8x/16x per pixel)
+    kFamRefine = 1,  // k_ncc_weighted<REFINE>: the 5 refinement hypotheses
+    kFamPost = 2,    // k_ncc_weighted<POST>: DepthToWeak (61) and LocalRefine (12) samples
+    kFamWeak = 3,    // k_weak_update: weak-pixel propagation (ComputeBilateralNCCNew windows)
+    kNumFamilies = 4
 };
 
 // Launch interface (dpe_kernels.hip); every function enqueues on `stream`.

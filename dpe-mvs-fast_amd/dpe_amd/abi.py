@@ -50,9 +50,16 @@ class Outputs(C.Structure):
                 ("selected_views", C.c_void_p)]
 
 
+NCC_FAMILIES = 4
+FAMILY_FULL, FAMILY_REFINE, FAMILY_POST, FAMILY_WEAK = range(NCC_FAMILIES)
+ABI_VERSION = 2
+
+
 class Stats(C.Structure):
     _fields_ = [("gpu_ms", C.c_float), ("ncc_ms", C.c_float), ("ncc_hyps", C.c_int64), ("ncc_taps", C.c_int64),
-                ("ncc_launches", C.c_int32)]
+                ("ncc_launches", C.c_int32), ("family_ms", C.c_float * NCC_FAMILIES),
+                ("family_hyps", C.c_int64 * NCC_FAMILIES), ("family_taps", C.c_int64 * NCC_FAMILIES),
+                ("family_launches", C.c_int32 * NCC_FAMILIES)]
 
 
 class FuseView(C.Structure):

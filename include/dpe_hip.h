@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DPE_ABI_VERSION 1
+#define DPE_ABI_VERSION 2
 /* main.h:41-44 */
 #define DPE_MAX_IMAGES 32
 #define DPE_NEIGHBOUR_NUM 9
@@ -119,14 +119,27 @@ typedef struct dpe_outputs {
     uint32_t *selected_views; /* bit i = source view i+1 selected */
 } dpe_outputs;
 
-/* Per-kernel timing of the last dpe_process call (hipEvent, milliseconds), and
- * exact counts of NCC plane hypotheses and taps evaluated on the GPU. */
+/* NCC kernel families of dpe_stats.family_*. */
+#define DPE_NCC_FAMILIES 4
+#define DPE_FAMILY_FULL 0   /* initial cost vectors + propagation candidates (DPE.cu:1039-1067, 1218-1670) */
+#define DPE_FAMILY_REFINE 1 /* refinement hypotheses (PlaneHypothesisRefinementStrong, DPE.cu:1069-1122) */
+#define DPE_FAMILY_POST 2   /* DepthToWeak + LocalRefine samples (DPE.cu:2597-2839) */
+#define DPE_FAMILY_WEAK 3   /* weak-pixel propagation (CheckerboardPropagationWeak, DPE.cu:1672-1866) */
+
+/* Timing of the last dpe_process / dpe_ncc_cost call (hipEvent pairs around
+ * each launch, milliseconds) and exact counts of the NCC plane hypotheses and
+ * taps evaluated on the GPU.  One hypothesis = one (pixel, plane, source view)
+ * call of ComputeBilateralNCCOld/New; one tap = one bilinear source sample. */
 typedef struct dpe_stats {
     float gpu_ms;             /* whole call: GetLastGpuTimeMs (DPE.cpp:1286) */
-    float ncc_ms;             /* all launches of the multi-view NCC cost kernel */
-    int64_t ncc_hyps;         /* (pixel, plane, source view) NCC windows */
-    int64_t ncc_taps;         /* bilinear source samples behind them */
-    int32_t ncc_launches;
+    float ncc_ms;             /* launches of families FULL, REFINE and POST */
+    int64_t ncc_hyps;         /* all families */
+    int64_t ncc_taps;         /* all families */
+    int32_t ncc_launches;     /* launches of families FULL, REFINE and POST */
+    float family_ms[DPE_NCC_FAMILIES];   /* the WEAK family's kernel also does the view selection */
+    int64_t family_hyps[DPE_NCC_FAMILIES];
+    int64_t family_taps[DPE_NCC_FAMILIES];
+    int32_t family_launches[DPE_NCC_FAMILIES];
 } dpe_stats;
 
 typedef struct dpe_ctx dpe_ctx;

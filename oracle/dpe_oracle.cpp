@@ -42,6 +42,13 @@ struct I2 { int x, y; };
 constexpr int MAXIMG = DPE_MAX_IMAGES;
 constexpr int NNUM = DPE_NEIGHBOUR_NUM;
 
+// NCC families (dpe_stats.family_* of the GPU library) plus the evaluations
+// whose results the reference never reads; tl_family names the family of the
+// work a thread is doing.
+constexpr int kFamFull = DPE_FAMILY_FULL, kFamRefine = DPE_FAMILY_REFINE, kFamPost = DPE_FAMILY_POST,
+              kFamWeak = DPE_FAMILY_WEAK, kFamDead = DPE_NCC_FAMILIES, kFamilies = DPE_NCC_FAMILIES + 1;
+thread_local int tl_family = kFamFull;
+
 inline F4 mk4(float x, float y, float z, float w) { return F4{x, y, z, w}; }
 
 // ---------------------------------------------------------------- call context
@@ -70,8 +77,9 @@ struct Ctx {
     // GenNeighbours constants computed as DPE.cu:2151-2157 does (host double math)
     float gn_cos_angle = 0, gn_sin_angle = 0, gn_threshold = 0;
     int gn_shift_range = 1;
-    // counters
+    // counters, in total and per NCC family (see tl_family)
     std::atomic<int64_t> ncc_hyps{0}, ncc_taps{0};
+    std::atomic<int64_t> fam_hyps[kFamilies] = {}, fam_taps[kFamilies] = {};
 };
 
 template <class F>
@@ -421,6 +429,7 @@ float BilateralNCCWindow(const Ctx &C, int src_idx, const float *H, int cx, int 
         wsum += r_w;
     }
     const_cast<Ctx &>(C).ncc_taps.fetch_add(taps, std::memory_order_relaxed);
+    const_cast<Ctx &>(C).fam_taps[tl_family].fetch_add(taps, std::memory_order_relaxed);
     const float inv = 1.0f / wsum;
     sum_ref *= inv;
     sum_ref_ref *= inv;
@@ -442,6 +451,7 @@ float ComputeBilateralNCCOld(const Ctx &C, int px, int py, int src_idx, const F4
     float H[9];
     ComputeHomography(C.cams[0], src, plane, H);
     const_cast<Ctx &>(C).ncc_hyps.fetch_add(1, std::memory_order_relaxed);
+    const_cast<Ctx &>(C).fam_hyps[tl_family].fetch_add(1, std::memory_order_relaxed);
     const F2 pt = ComputeCorrespondingPoint(H, px, py);
     if (pt.x >= (float)src.width || pt.x < 0.0f || pt.y >= (float)src.height || pt.y < 0.0f) return 2.0f;
     const float rc = texel(C.img[0], C.W, C.H, px, py);
@@ -461,6 +471,7 @@ float ComputeBilateralNCCNew(const Ctx &C, int px, int py, int src_idx, const F4
     float H[9];
     ComputeHomography(C.cams[0], src, plane, H);
     const_cast<Ctx &>(C).ncc_hyps.fetch_add(1, std::memory_order_relaxed);
+    const_cast<Ctx &>(C).fam_hyps[tl_family].fetch_add(1, std::memory_order_relaxed);
     const F2 pt = ComputeCorrespondingPoint(H, px, py);
     if (pt.x >= (float)src.width || pt.x < 0.0f || pt.y >= (float)src.height || pt.y < 0.0f) return cost_max;
     float cost = 0.0f;
@@ -618,6 +629,7 @@ float InitialCost(Ctx &C, int px, int py) {
 
 // RandomInitialization, DPE.cu:1039-1067
 void RandomInitialization(Ctx &C, int center) {
+    tl_family = kFamFull;
     const int px = center % C.W, py = center / C.W;
     const dpe_camera &cam = C.cams[0];
     if (C.prm.state == DPE_FIRST_INIT) {
@@ -700,7 +712,10 @@ void RefinementStrong(Ctx &C, F4 *plane, float *depth, float *cost, Rng &rng, co
         float cv[32] = {2.0f};
         F4 tp = normals[i];
         tp.w = GetDistance2Origin(cam, px, py, depths[i], tp);
-        CostVectorOld(C, px, py, tp, cv);
+        for (int j = 0; j < C.N - 1; ++j) {  // CostVectorOld; views without weight are never read
+            tl_family = vw[j] > 0 ? kFamRefine : kFamDead;
+            cv[j] = ComputeBilateralNCCOld(C, px, py, j + 1, tp);
+        }
         float tc = 0.0f;
         for (int j = 0; j < C.N - 1; ++j)
             if (vw[j] > 0) tc += vw[j] * cv[j];
@@ -725,7 +740,11 @@ void RefinementWeak(Ctx &C, F4 *plane, float *depth, float *cost, Rng &rng, cons
     const int NV = C.N - 1;
     auto weighted = [&](const F4 &tp) {
         float cv[32] = {2.0f};
-        CostVectorNew(C, px, py, tp, cv, C.selected);
+        for (int j = 0; j < NV; ++j) {  // CostVectorNew; views without weight are never read
+            tl_family = vw[j] > 0 ? kFamWeak : kFamDead;
+            cv[j] = ComputeBilateralNCCNew(C, px, py, j + 1, tp, C.selected);
+        }
+        tl_family = kFamWeak;
         float tc = 0.0f;
         for (int j = 0; j < NV; ++j) {
             if (vw[j] > 0) {
@@ -1010,6 +1029,7 @@ inline uint32_t snap_selected_at(const Ctx &C, long idx) {
 
 // CheckerboardPropagationStrong, DPE.cu:1218-1670
 void CheckerboardPropagationStrong(Ctx &C, int px, int py, int iter) {
+    tl_family = kFamFull;
     const int W = C.W;
     const int center = py * W + px;
     const int NV = C.N - 1;
@@ -1072,6 +1092,7 @@ void CheckerboardPropagationStrong(Ctx &C, int px, int py, int iter) {
 
 // CheckerboardPropagationWeak, DPE.cu:1672-1866
 void CheckerboardPropagationWeak(Ctx &C, int px, int py, int iter) {
+    tl_family = kFamWeak;
     const int W = C.W;
     const int center = py * W + px;
     const int NV = C.N - 1;
@@ -1823,6 +1844,9 @@ inline float CameraDistance(const Ctx &C, int src_index) {
 
 // DepthToWeak, DPE.cu:2597-2751
 void DepthToWeak(Ctx &C, int center) {
+    // The cost of the current plane below is never read afterwards in the
+    // reference (DPE.cu:2629-2658): counted apart as dead evaluations.
+    tl_family = kFamDead;
     const int W = C.W, H = C.H;
     const int px = center % W, py = center / W;
     const int min_margin = 6;
@@ -1862,6 +1886,7 @@ void DepthToWeak(Ctx &C, int center) {
     }
     cost_now /= wn;
     base_line /= valid;
+    tl_family = kFamPost;
     const float disp = cam.K[0] * base_line / od;
     const int radius = 30;
     const int n = 2 * radius + 1;
@@ -1928,6 +1953,13 @@ void LocalRefine(Ctx &C, int center) {
     const dpe_camera &cam = C.cams[0];
     const uint32_t sel = C.selected[center];
     const uint8_t *vw = &C.view_weight[(size_t)MAXIMG * center];
+    {
+        // with zero total weight the cost of the current plane is never read (early return below)
+        float wsum = 0.0f;
+        for (int s = 1; s < C.N; ++s)
+            if (isSet(sel, s - 1)) wsum += vw[s - 1];
+        tl_family = wsum == 0 ? kFamDead : kFamPost;
+    }
     F4 op = TransformNormal2RefCam(cam, C.planes[center]);
     const float od = op.w;
     if (od == 0) return;
@@ -1949,6 +1981,7 @@ void LocalRefine(Ctx &C, int center) {
         }
     }
     if (wn == 0 || valid == 0) return;
+    tl_family = kFamPost;
     cost_now /= wn;
     base_line /= valid;
     const float disp = cam.K[0] * base_line / od;
@@ -2162,6 +2195,10 @@ int oracle_process(const dpe_params *params, const dpe_inputs *in, dpe_outputs *
     if (counters) {
         counters[0] = C.ncc_hyps.load();
         counters[1] = C.ncc_taps.load();
+        for (int f = 0; f < kFamilies; ++f) {
+            counters[2 + 2 * f] = C.fam_hyps[f].load();
+            counters[3 + 2 * f] = C.fam_taps[f].load();
+        }
     }
     return DPE_OK;
 }
@@ -2178,6 +2215,7 @@ int oracle_ncc_cost(const dpe_params *params, const dpe_inputs *in, int num_hyps
     const int NV = C.N - 1;
     const size_t P = (size_t)C.P;
     parallel_for(C.P * num_hyps, C.nthreads, [&](int k) {
+        tl_family = kFamFull;
         const int h = k / C.P, i = k % C.P;
         const float *pl = planes + ((size_t)h * P + i) * 4;
         const F4 plane{pl[0], pl[1], pl[2], pl[3]};

@@ -7,7 +7,10 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
-/* One process call (DPE.cu:3130-3262); counters[0..1] = NCC hyps, taps. */
+/* One process call (DPE.cu:3130-3262).  counters (NULL or 12 entries):
+ * [0..1] NCC hyps, taps in total; [2+2f], [3+2f] hyps, taps of family f =
+ * DPE_FAMILY_* (0..3) and f = 4, evaluations whose results the reference never
+ * reads (the GPU library skips them). */
 int oracle_process(const dpe_params *params, const dpe_inputs *in, dpe_outputs *out, int nthreads,
                    int64_t *counters);
 /* ComputeMultiViewCostVectorOld for given camera-frame planes (layout of dpe_ncc_cost). */

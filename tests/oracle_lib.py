@@ -47,12 +47,18 @@ def load_oracle():
     return lib
 
 
-def run_oracle(lib, pin, nthreads=8):
-    """One oracle process call on ProblemInputs; returns (ProblemOutputs, (hyps, taps))."""
+ORACLE_FAMILIES = 5  # the GPU's four NCC families + evaluations the reference never reads
+
+
+def run_oracle(lib, pin, nthreads=8, families=None):
+    """One oracle process call on ProblemInputs; returns (ProblemOutputs, (hyps, taps)).
+    If `families` is a list, it receives (hyps, taps) of each NCC family."""
     from dpe_amd import host
     out = host.ProblemOutputs(pin.width, pin.height)
-    cnt = (C.c_int64 * 2)()
+    cnt = (C.c_int64 * (2 + 2 * ORACLE_FAMILIES))()
     st = lib.oracle_process(C.byref(pin.params), C.byref(pin.struct), C.byref(out.struct), nthreads, cnt)
     if st != 0:
         raise RuntimeError(f"oracle_process failed with status {st}")
+    if families is not None:
+        families[:] = [(cnt[2 + 2 * f], cnt[3 + 2 * f]) for f in range(ORACLE_FAMILIES)]
     return out, (cnt[0], cnt[1])

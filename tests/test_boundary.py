@@ -19,7 +19,7 @@ def test_library_exports_every_declared_symbol():
     assert declared == set(abi.EXPORTS)
     for name in declared:
         assert hasattr(lib, name), name
-    assert lib.dpe_abi_version() == 1
+    assert lib.dpe_abi_version() == abi.ABI_VERSION
 
 
 def test_struct_layouts():
@@ -28,6 +28,42 @@ def test_struct_layouts():
     assert C.sizeof(abi.Params) == 112
     assert abi.Params.seed.offset == 104
     assert C.sizeof(abi.Point) == 16
+
+
+STRUCTS = {"dpe_camera": abi.Camera, "dpe_params": abi.Params, "dpe_inputs": abi.Inputs, "dpe_outputs": abi.Outputs,
+           "dpe_stats": abi.Stats, "dpe_fuse_view": abi.FuseView, "dpe_fuse_params": abi.FuseParams,
+           "dpe_point": abi.Point}
+
+
+def test_ctypes_mirror_matches_the_c_header(tmp_path):
+    """Every field offset and struct size of the ctypes mirror equals what a C
+    compiler lays out from include/dpe_hip.h, and the #define constants agree."""
+    import subprocess
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "dpe_hip.h"', "int main(void) {"]
+    for cname, cls in STRUCTS.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    for const in ("DPE_ABI_VERSION", "DPE_MAX_IMAGES", "DPE_NCC_FAMILIES", "DPE_FAMILY_WEAK"):
+        lines.append(f'printf("const {const} %d\\n", (int)({const}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        a, b, v = line.split()
+        got[(a, b)] = int(v)
+    for cname, cls in STRUCTS.items():
+        assert got[(cname, "size")] == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
+    assert got[("const", "DPE_ABI_VERSION")] == abi.ABI_VERSION
+    assert got[("const", "DPE_MAX_IMAGES")] == abi.MAX_IMAGES
+    assert got[("const", "DPE_NCC_FAMILIES")] == abi.NCC_FAMILIES
+    assert got[("const", "DPE_FAMILY_WEAK")] == abi.FAMILY_WEAK
 
 
 def test_result_assembly_zeroes_out_of_range_depth():

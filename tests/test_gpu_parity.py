@@ -93,6 +93,16 @@ def test_ncc_cost_vectors_match_oracle(ctx, orc, c1):
     assert st.ncc_hyps == nh * NV * H * W
 
 
+def _check_counts(st, fam_o, label):
+    """The GPU evaluates exactly the oracle's NCC hypotheses and taps, family by
+    family (the metric's unit), except the evaluations whose results the
+    reference never reads (the oracle's last family), which it skips."""
+    fam_g = [(st.family_hyps[f], st.family_taps[f]) for f in range(abi.NCC_FAMILIES)]
+    print(f"{label}: NCC (hyps, taps) per family gpu {fam_g} oracle {fam_o}")
+    assert sum(h for h, _ in fam_g) == st.ncc_hyps and sum(t for _, t in fam_g) == st.ncc_taps
+    assert fam_g == [tuple(x) for x in fam_o[:abi.NCC_FAMILIES]]
+
+
 def _compare(res_g, res_o, label):
     da = depth_agreement(res_g["depth"], res_o["depth"])
     na = normal_agreement(res_g["normal"], res_o["normal"])
@@ -108,7 +118,9 @@ def test_config1_first_init_matches_oracle(ctx, orc, c1):
     ref = c1.meta["ref_mid"]
     pin = host.first_call_inputs(c1, ref, c1.pairs[ref], max_iterations=3)
     out_g = host.run_hip(ctx, pin)
-    out_o, _ = oracle_lib.run_oracle(orc, pin, 8)
+    fam_o = []
+    out_o, _ = oracle_lib.run_oracle(orc, pin, 8, fam_o)
+    _check_counts(ctx.stats(), fam_o, "config1")
     dmin, dmax = pin.params.depth_min, pin.params.depth_max
     res_g = host.assemble(out_g, dmin, dmax)
     res_o = host.assemble(out_o, dmin, dmax)
@@ -132,7 +144,9 @@ def test_config1b_weak_path_matches_oracle(ctx, orc, c1):
     pin = host.refine_call_inputs(c1, ref, c1.pairs[ref], store_o, max_iterations=3, call_index=1)
     weak = int((pin.weak_info == abi.WEAK).sum())
     out_g = host.run_hip(ctx, pin)
-    out_o, _ = oracle_lib.run_oracle(orc, pin, 8)
+    fam_o = []
+    out_o, _ = oracle_lib.run_oracle(orc, pin, 8, fam_o)
+    _check_counts(ctx.stats(), fam_o, "config1b")
     dmin, dmax = pin.params.depth_min, pin.params.depth_max
     res_g = host.assemble(out_g, dmin, dmax)
     res_o = host.assemble(out_o, dmin, dmax)
