@@ -138,8 +138,10 @@ dpe_status validate(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in) {
     if (in->num_images > DPE_MAX_IMAGES)
         return fail(ctx, DPE_ERR_TOO_MANY_IMAGES, "Can't process so much images: " + std::to_string(in->num_images));
     if (in->num_images < 2) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "need a reference and at least one source");
-    if (in->width <= 0 || in->height <= 0 || (long)in->width * in->height > (1L << 30))
-        return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad image size");
+    if (in->width <= 0 || in->height <= 0) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad image size");
+    // quad offsets are 32-bit byte offsets; tap coordinates stay within 2^17
+    if (in->width > 65536 || in->height > 65536 || (long)(in->width + 1) * (in->height + 1) >= (1L << 28))
+        return fail(ctx, DPE_ERR_UNSUPPORTED, "image larger than 65536 per side or (W+1)(H+1) >= 2^28");
     if (in->memory != DPE_MEM_HOST && in->memory != DPE_MEM_DEVICE)
         return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad memory kind");
     for (int i = 0; i < in->num_images; ++i) {

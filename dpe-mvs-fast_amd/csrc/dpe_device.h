@@ -273,10 +273,66 @@ __device__ __forceinline__ float2 project(const float H[9], int px, int py) {
     return make_float2(u / w, v / w);
 }
 
+// ---- exact division without the scaling steps
+// For IEEE a / w the compiler emits w' = div_scale(w), a' = div_scale(a),
+// y = rcp(w'), y1 = fma(fma(-w', y, 1), y, y), q0 = a' * y1,
+// q1 = fma(fma(-w', q0, a'), y1, q0), q2 = fma(fma(-w', q1, a'), y1, q1)
+// (div_fmas), then div_fixup.  Scaling is the identity and the fixup passes q2
+// through when w and 1/w are normal, a is 0 or |a| >= 2^-103, the exponents of
+// a and w differ by less than 96 and a / w is normal.  Under those conditions
+// the two functions below give the bits of a / w (for a = 0 possibly with the
+// other sign of zero), and y1 is shared by every numerator over the same w.
+__device__ __forceinline__ float recip_refined(float w) {
+    const float y = __builtin_amdgcn_rcpf(w);
+    return fmaf(fmaf(-w, y, 1.0f), y, y);
+}
+__device__ __forceinline__ float quotient_refined(float a, float w, float y1) {
+    const float q0 = a * y1;
+    const float q1 = fmaf(fmaf(-w, q0, a), y1, q0);
+    return fmaf(fmaf(-w, q1, a), y1, q1);
+}
+
+// project() through the refined division; equal to project() wherever
+// refined_division_ok() holds for a window containing (px, py), except that a
+// zero coordinate may carry the other sign (floor and fraction are the same).
+__device__ __forceinline__ float2 project_refined(const float H[9], int px, int py) {
+    const float x = (float)px, y = (float)py;
+    const float u = fmaf(H[0], x, fmaf(H[1], y, H[2]));
+    const float v = fmaf(H[3], x, fmaf(H[4], y, H[5]));
+    const float w = fmaf(H[6], x, fmaf(H[7], y, H[8]));
+    const float r = recip_refined(w);
+    return make_float2(quotient_refined(u, w, r), quotient_refined(v, w, r));
+}
+
+// True when the refined division is exact at every tap (X, Y) of the window
+// [x0, x1] x [y0, y1] (|X|, |Y| <= 2^17):
+//  - each of H[0..5] is 0 or has magnitude in [2^-60, 2^20]: then u and v are
+//    multiples of 2^-83 (so 0 or >= 2^-83 in magnitude) and below 2^38;
+//  - w keeps one sign with |w| in [2^-40, 2^40]: w is affine in (X, Y), so its
+//    extremes sit at the window corners, and computed values differ from exact
+//    ones by less than 2^-22 M, M = |H6| max|X| + |H7| max|Y| + |H8|.
+// Hence w, 1/w and u/w, v/w are normal and the exponent gaps stay below 80.
+__device__ __forceinline__ bool refined_division_ok(const float H[9], int x0, int x1, int y0, int y1) {
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const float a = fabsf(H[i]);
+        ok = ok && (a == 0.0f || (a >= 0x1p-60f && a <= 0x1p20f));
+    }
+    const float X0 = (float)x0, X1 = (float)x1, Y0 = (float)y0, Y1 = (float)y1;
+    const float i0 = fmaf(H[7], Y0, H[8]), i1 = fmaf(H[7], Y1, H[8]);
+    const float w00 = fmaf(H[6], X0, i0), w10 = fmaf(H[6], X1, i0);
+    const float w01 = fmaf(H[6], X0, i1), w11 = fmaf(H[6], X1, i1);
+    const float lo = fminf(fminf(w00, w10), fminf(w01, w11));
+    const float hi = fmaxf(fmaxf(w00, w10), fmaxf(w01, w11));
+    const float M = fabsf(H[6]) * fmaxf(fabsf(X0), fabsf(X1)) + fabsf(H[7]) * fmaxf(fabsf(Y0), fabsf(Y1)) + fabsf(H[8]);
+    const float mn = lo > 0.0f ? lo : -hi;  // min |w| when the corners share a sign
+    return ok && (lo > 0.0f || hi < 0.0f) && M <= 0x1p39f && mn >= 0x1p-19f * M + 0x1p-39f;
+}
+
+// (int) of a clamped float; NaN gives lo.  |lo|, |hi| <= 2^24 (exact floats).
 __device__ __forceinline__ int f2i_clamp(float f, int lo, int hi) {
-    if (!(f >= (float)lo)) return lo;
-    if (f > (float)hi) return hi;
-    return (int)f;
+    return (int)fminf(fmaxf(f, (float)lo), (float)hi);
 }
 
 __device__ __forceinline__ float texel(const float *im, int W, int H, int x, int y) {
@@ -293,7 +349,9 @@ __device__ __forceinline__ float bilinear_quad(const float4 *__restrict__ q, int
     const float a = u - fx, b = v - fy;
     const int ix = f2i_clamp(fx, -1, W - 1);
     const int iy = f2i_clamp(fy, -1, H - 1);
-    const float4 t = q[(size_t)(iy + 1) * (W + 1) + (ix + 1)];
+    // 32-bit byte offset: (W + 1) * (H + 1) * 16 < 2^32 is checked at set-up
+    const uint32_t off = ((uint32_t)(iy + 1) * (uint32_t)(W + 1) + (uint32_t)(ix + 1)) * 16u;
+    const float4 t = *(const float4 *)((const char *)q + off);
     const float top = fmaf(a, t.y - t.x, t.x);
     const float bot = fmaf(a, t.w - t.z, t.z);
     return fmaf(b, bot - top, top);

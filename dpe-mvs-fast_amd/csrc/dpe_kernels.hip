@@ -129,16 +129,12 @@ __device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int 
     __syncthreads();
 }
 
-// ComputeBilateralNCCOld (DPE.cu:696-782) with the reference half from LDS.
-template <int NT>
-__device__ __forceinline__ float ncc_shared(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y, int v,
-                                            const float4 &pl, unsigned long long &taps) {
-    const ViewConst &vc = d.vc[v];
-    float H[9];
-    homography(vc, d.cams[0], pl, H);
-    const float2 pt = project(H, x, y);
-    if (pt.x >= (float)vc.width || pt.x < 0.0f || pt.y >= (float)vc.height || pt.y < 0.0f) return 2.0f;
-    float s_s = 0.0f, s_ss = 0.0f, s_rs = 0.0f;
+// Source sums of one window (DPE.cu:728-758) with the reference weights from
+// LDS; REFINED: the projection uses the refined division (exact, see
+// refined_division_ok).
+template <int NT, bool REFINED>
+__device__ __forceinline__ void shared_window_sums(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y,
+                                                   int v, const float H[9], float &s_s, float &s_ss, float &s_rs) {
 #pragma unroll 1
     for (int ki = 0; ki < NT; ++ki) {
         float r_s = 0.0f, r_ss = 0.0f, r_rs = 0.0f;
@@ -146,7 +142,7 @@ __device__ __forceinline__ float ncc_shared(const Dev &d, const SharedPatch<NT> 
 #pragma unroll DPE_TAP_UNROLL
         for (int kj = 0; kj < NT; ++kj) {
             const int ry = y + d.tap_off0 + kj * d.tap_inc;
-            const float2 sp = project(H, rx, ry);
+            const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
             const float src = sample_src(d, v, sp.x, sp.y);
             const float2 e = S.wwr[ki * NT + kj][lane];
             r_s = fmaf(e.x, src, r_s);
@@ -157,14 +153,31 @@ __device__ __forceinline__ float ncc_shared(const Dev &d, const SharedPatch<NT> 
         s_ss += r_ss;
         s_rs += r_rs;
     }
+}
+
+// ComputeBilateralNCCOld (DPE.cu:696-782) with the reference half from LDS.
+template <int NT>
+__device__ __forceinline__ float ncc_shared(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y, int v,
+                                            const float4 &pl, unsigned long long &taps) {
+    const ViewConst &vc = d.vc[v];
+    float H[9];
+    homography(vc, d.cams[0], pl, H);
+    const float2 pt = project(H, x, y);
+    if (pt.x >= (float)vc.width || pt.x < 0.0f || pt.y >= (float)vc.height || pt.y < 0.0f) return 2.0f;
+    float s_s = 0.0f, s_ss = 0.0f, s_rs = 0.0f;
+    const int x0 = x + d.tap_off0, y0 = y + d.tap_off0, span = (NT - 1) * d.tap_inc;
+    if (refined_division_ok(H, x0, x0 + span, y0, y0 + span))
+        shared_window_sums<NT, true>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+    else
+        shared_window_sums<NT, false>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
     taps += NT * NT;
     return ncc_from_sums(S.sum_ref[lane], S.sum_ref_ref[lane], s_s, s_ss, s_rs, S.wsum[lane]);
 }
 
-// One bilateral window computed directly (any radius/increment): the shared
-// body of ComputeBilateralNCCNew (DPE.cu:613-672) and ComputeBilateralNCCOld.
-__device__ float ncc_window_direct(const Dev &d, int v, const float H[9], int cx, int cy, float rc, int radius,
-                                   int inc, unsigned long long &taps) {
+// Sums of one bilateral window computed directly (any radius/increment).
+template <bool REFINED>
+__device__ __forceinline__ float ncc_window_sums(const Dev &d, int v, const float H[9], int cx, int cy, float rc,
+                                                 int radius, int inc, unsigned long long &taps) {
     const float ss = d.prm.sigma_spatial, sc = d.prm.sigma_color;
     float sum_ref = 0.0f, sum_ref_ref = 0.0f, sum_src = 0.0f, sum_src_src = 0.0f, sum_ref_src = 0.0f, wsum = 0.0f;
     for (int i = -radius; i <= radius; i += inc) {
@@ -172,7 +185,7 @@ __device__ float ncc_window_direct(const Dev &d, int v, const float H[9], int cx
         for (int j = -radius; j <= radius; j += inc) {
             const int rx = cx + i, ry = cy + j;
             const float ref = texel(d.ref, d.W, d.H, rx, ry);
-            const float2 sp = project(H, rx, ry);
+            const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
             const float src = sample_src(d, v, sp.x, sp.y);
             const float fi = (float)i, fj = (float)j;
             const float spatial = sqrtf(fi * fi + fj * fj);
@@ -195,6 +208,16 @@ __device__ float ncc_window_direct(const Dev &d, int v, const float H[9], int cx
         wsum += r_w;
     }
     return ncc_from_sums(sum_ref, sum_ref_ref, sum_src, sum_src_src, sum_ref_src, wsum);
+}
+
+// One bilateral window computed directly (any radius/increment): the shared
+// body of ComputeBilateralNCCNew (DPE.cu:613-672) and ComputeBilateralNCCOld.
+__device__ float ncc_window_direct(const Dev &d, int v, const float H[9], int cx, int cy, float rc, int radius,
+                                   int inc, unsigned long long &taps) {
+    const int last = -radius + ((2 * radius) / inc) * inc;  // offset of the last tap
+    if (refined_division_ok(H, cx - radius, cx + last, cy - radius, cy + last))
+        return ncc_window_sums<true>(d, v, H, cx, cy, rc, radius, inc, taps);
+    return ncc_window_sums<false>(d, v, H, cx, cy, rc, radius, inc, taps);
 }
 
 __device__ float ncc_old_direct(const Dev &d, int x, int y, int v, const float4 &pl, unsigned long long &hyps,
