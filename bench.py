@@ -29,10 +29,11 @@ BYTES_PER_TAP = 20     # one 4-B reference texel + one 16-B bilinear source quad
 FAMILY_NAMES = ["k_ncc_full<6>", "k_ncc_weighted<6,REFINE>", "k_ncc_weighted<6,POST>", "k_weak_update"]
 # HBM traffic of one k_ncc_full<6> launch on this workload, from two rocprofv3
 # --pmc passes (FETCH_SIZE, WRITE_SIZE) over `python bench.py --no-cpu-baseline`,
-# each scaled by the counter's reading for a 256 MiB device copy (FETCH_SIZE x2,
-# WRITE_SIZE x1); the runs and the arithmetic are in
-# profiles/ncc_full_c2_hbm_traffic.md.  Counters cannot be read inside a timed run.
-NCC_FULL_TRAFFIC_BYTES = None
+# each scaled by the counter's reading for a 256 MiB device copy
+# (`python bench.py --copy-calibration`: FETCH_SIZE x2.000, WRITE_SIZE x1.000);
+# the runs and the arithmetic are in profiles/ncc_kernels_c2_hbm_traffic.md.
+# Counters cannot be read inside a timed run.
+NCC_FULL_TRAFFIC_BYTES = 3.1004e10
 
 
 def parse():

@@ -48,6 +48,12 @@ constexpr int NNUM = DPE_NEIGHBOUR_NUM;
 constexpr int kFamFull = DPE_FAMILY_FULL, kFamRefine = DPE_FAMILY_REFINE, kFamPost = DPE_FAMILY_POST,
               kFamWeak = DPE_FAMILY_WEAK, kFamDead = DPE_NCC_FAMILIES, kFamilies = DPE_NCC_FAMILIES + 1;
 thread_local int tl_family = kFamFull;
+// Index of the parallel_for worker running on this thread: each worker counts
+// into its own cache line of Ctx::tally (no shared atomics in the hot loop).
+thread_local int tl_worker = 0;
+struct alignas(64) Tally {
+    int64_t hyps[kFamilies] = {}, taps[kFamilies] = {};
+};
 
 inline F4 mk4(float x, float y, float z, float w) { return F4{x, y, z, w}; }
 
@@ -78,13 +84,13 @@ struct Ctx {
     float gn_cos_angle = 0, gn_sin_angle = 0, gn_threshold = 0;
     int gn_shift_range = 1;
     // counters, in total and per NCC family (see tl_family)
-    std::atomic<int64_t> ncc_hyps{0}, ncc_taps{0};
-    std::atomic<int64_t> fam_hyps[kFamilies] = {}, fam_taps[kFamilies] = {};
+    std::vector<Tally> tally;  // one per worker
 };
 
 template <class F>
 void parallel_for(int n, int nthreads, F &&fn) {
     if (nthreads <= 1 || n < 256) {
+        tl_worker = 0;
         for (int i = 0; i < n; ++i) fn(i);
         return;
     }
@@ -92,7 +98,8 @@ void parallel_for(int n, int nthreads, F &&fn) {
     std::atomic<int> next{0};
     const int chunk = 64;
     for (int t = 0; t < nthreads; ++t) {
-        pool.emplace_back([&]() {
+        pool.emplace_back([&, t]() {
+            tl_worker = t;
             for (;;) {
                 const int s = next.fetch_add(chunk);
                 if (s >= n) break;
@@ -428,8 +435,7 @@ float BilateralNCCWindow(const Ctx &C, int src_idx, const float *H, int cx, int 
         sum_ref_src += r_rs;
         wsum += r_w;
     }
-    const_cast<Ctx &>(C).ncc_taps.fetch_add(taps, std::memory_order_relaxed);
-    const_cast<Ctx &>(C).fam_taps[tl_family].fetch_add(taps, std::memory_order_relaxed);
+    const_cast<Ctx &>(C).tally[tl_worker].taps[tl_family] += taps;
     const float inv = 1.0f / wsum;
     sum_ref *= inv;
     sum_ref_ref *= inv;
@@ -450,8 +456,7 @@ float ComputeBilateralNCCOld(const Ctx &C, int px, int py, int src_idx, const F4
     const dpe_camera &src = C.cams[src_idx];
     float H[9];
     ComputeHomography(C.cams[0], src, plane, H);
-    const_cast<Ctx &>(C).ncc_hyps.fetch_add(1, std::memory_order_relaxed);
-    const_cast<Ctx &>(C).fam_hyps[tl_family].fetch_add(1, std::memory_order_relaxed);
+    const_cast<Ctx &>(C).tally[tl_worker].hyps[tl_family]++;
     const F2 pt = ComputeCorrespondingPoint(H, px, py);
     if (pt.x >= (float)src.width || pt.x < 0.0f || pt.y >= (float)src.height || pt.y < 0.0f) return 2.0f;
     const float rc = texel(C.img[0], C.W, C.H, px, py);
@@ -470,8 +475,7 @@ float ComputeBilateralNCCNew(const Ctx &C, int px, int py, int src_idx, const F4
     const float cost_max = 2.0f;
     float H[9];
     ComputeHomography(C.cams[0], src, plane, H);
-    const_cast<Ctx &>(C).ncc_hyps.fetch_add(1, std::memory_order_relaxed);
-    const_cast<Ctx &>(C).fam_hyps[tl_family].fetch_add(1, std::memory_order_relaxed);
+    const_cast<Ctx &>(C).tally[tl_worker].hyps[tl_family]++;
     const F2 pt = ComputeCorrespondingPoint(H, px, py);
     if (pt.x >= (float)src.width || pt.x < 0.0f || pt.y >= (float)src.height || pt.y < 0.0f) return cost_max;
     float cost = 0.0f;
@@ -2032,6 +2036,7 @@ int setup(Ctx &C, const dpe_params *params, const dpe_inputs *in, int nthreads) 
     C.prm = *params;
     C.prm.num_images = C.N;
     C.nthreads = nthreads;
+    C.tally.assign(nthreads, Tally{});
     for (int i = 0; i < C.N; ++i) {
         C.cams[i] = in->cameras[i];
         C.img[i] = in->images[i];
@@ -2193,11 +2198,17 @@ int oracle_process(const dpe_params *params, const dpe_inputs *in, dpe_outputs *
         out->selected_views[i] = C.selected[i];
     }
     if (counters) {
-        counters[0] = C.ncc_hyps.load();
-        counters[1] = C.ncc_taps.load();
+        counters[0] = counters[1] = 0;
         for (int f = 0; f < kFamilies; ++f) {
-            counters[2 + 2 * f] = C.fam_hyps[f].load();
-            counters[3 + 2 * f] = C.fam_taps[f].load();
+            int64_t h = 0, t = 0;
+            for (const Tally &w : C.tally) {
+                h += w.hyps[f];
+                t += w.taps[f];
+            }
+            counters[2 + 2 * f] = h;
+            counters[3 + 2 * f] = t;
+            counters[0] += h;
+            counters[1] += t;
         }
     }
     return DPE_OK;
