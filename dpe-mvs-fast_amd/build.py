@@ -28,8 +28,33 @@ def hipcc():
             return c
 
 
+HOST_LIB = os.path.join(HERE, "libdpe_host.so")
+DRIVER = os.path.join(HERE, "dpe_driver")
+HOST_SOURCES = [os.path.join(HERE, "host", "dpe_hostio.cpp")]
+DRIVER_SOURCES = [os.path.join(HERE, "host", "dpe_driver.cpp")]
+GXX_FLAGS = ["-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include")]
+
+
+def build_host(verbose=True):
+    """libdpe_host.so (formats, pyramid, support maps; CPU only) and the
+    dpe_driver binary over it and libdpe_hip.so (rpath $ORIGIN)."""
+    cmd = ["g++"] + GXX_FLAGS + ["-fPIC", "-shared", "-o", HOST_LIB + ".tmp"] + HOST_SOURCES
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    cmd = (["g++"] + GXX_FLAGS + ["-o", DRIVER + ".tmp"] + DRIVER_SOURCES +
+           ["-L", HERE, "-ldpe_hip", "-ldpe_host", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"])
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return HOST_LIB, DRIVER
+
+
 def build(verbose=True, jobs=2):
-    """Compile every HIP source for gfx950 and link libdpe_hip.so (always rebuilds)."""
+    """Compile every HIP source for gfx950 and link libdpe_hip.so, then the host
+    library and the driver (always rebuilds)."""
     objs = []
     procs = []
     for src in SOURCES:
@@ -50,6 +75,7 @@ def build(verbose=True, jobs=2):
     os.replace(LIB + ".tmp", LIB)
     for o in objs:
         os.remove(o)
+    build_host(verbose)
     return LIB
 
 

@@ -217,3 +217,56 @@ def refine_call_inputs(scene, ref, srcs, store, max_iterations=3, seed=0x5EED000
         edges, labels = support
     return ProblemInputs(imgs, cams, prm, geom_depths=geom, weak_info=st["pixel_states"], prior_planes=planes,
                          prior_selected=st["selected_views"], edges=edges, edges_low=edges, labels=labels)
+
+
+def reference_schedule(round_num):
+    """The call groups of main.cpp:390-457 (one INIT call, then 3 REFINE_ITER
+    calls per round, coarse to fine), as dicts; the C++ driver builds the same
+    list (dpe_driver --dry-run prints it)."""
+    calls = []
+    for i in range(round_num):
+        scale = 1 << (round_num - 1 - i)
+        calls.append(dict(scale=scale, state=abi.FIRST_INIT if i == 0 else abi.REFINE_INIT, iters=3,
+                          apd=0 if i == 0 else 1, edge=0 if i == 0 else 1, geom=0, peak=6,
+                          ransac=0.005 if i == 0 else 0.01 - i * 0.00125, rotate=4 if i == 0 else min(1 << i, 4),
+                          final=False))
+        for j in range(3):
+            calls.append(dict(scale=scale, state=abi.REFINE_ITER, iters=3, apd=0 if i == 0 else 1,
+                              edge=0 if i == 0 else 1, geom=1, peak=max(4 - 2 * j, 2), ransac=0.01 - i * 0.00125,
+                              rotate=min(1 << i, 4), final=(i == round_num - 1 and j == 2)))
+    return calls
+
+
+def run_schedule_scale1(ctx, scene, calls, views=None):
+    """Python mirror of the C++ driver for schedules at scale 1 over a synthetic
+    scene: every call group over every view in order, state hand-off through a
+    StateStore, call_index = call group index.  Returns the StateStore."""
+    from . import support as sup
+    views = list(range(len(scene.views))) if views is None else list(views)
+    store = StateStore()
+    maps = {i: sup.support_maps(scene.views[i].image) for i in views}
+    for it, c in enumerate(calls):
+        assert c["scale"] == 1
+        for ref in views:
+            srcs = [j for j in scene.pairs[ref] if j in views]
+            ids = [ref] + srcs
+            imgs = [scene.views[i].image.astype(np.float32) for i in ids]
+            cams = scene_cameras(scene, ids)
+            prm = call_params(scene.depth_min, scene.depth_max, max_iterations=c["iters"], state=c["state"],
+                              use_APD=c["apd"], use_edge=c["edge"], geom_consistency=c["geom"],
+                              weak_peak_radius=c["peak"], ransac_threshold=np.float32(c["ransac"]),
+                              rotate_time=c["rotate"], call_index=it)
+            kw = {}
+            if c["geom"]:
+                zero = np.zeros((scene.height, scene.width), np.float32)
+                kw["geom_depths"] = [store.get(i)["depth"] if store.get(i) is not None else zero for i in ids]
+            st = store.get(ref)
+            if c["apd"]:
+                kw["weak_info"] = st["pixel_states"]
+            if c["state"] != abi.FIRST_INIT:
+                kw["prior_planes"] = np.concatenate([st["normal"], st["depth"][..., None]], -1).astype(np.float32)
+                kw["prior_selected"] = st["selected_views"]
+            edges, labels = maps[ref]
+            pin = ProblemInputs(imgs, cams, prm, edges=edges, edges_low=edges, labels=labels, **kw)
+            store.put(ref, assemble(run_hip(ctx, pin), prm.depth_min, prm.depth_max))
+    return store

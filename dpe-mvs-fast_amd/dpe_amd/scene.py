@@ -249,3 +249,40 @@ def config1(seed=0x5EED0001):
 def config2(seed=0x5EED0001, n_views=20):
     """1600x1200, f=1500, 20 cameras on a 60 degree arc, 10 nearest sources each."""
     return make_scene(1600, 1200, 1500.0, n_views, 60.0, seed=seed, n_src=10, supersample=1)
+
+
+def write_folder(scene, folder, views=None):
+    """Write the scene in the reference's input layout (main.cpp:336-341):
+    images/%08d.pgm (lossless 8-bit gray instead of JPEG), cams/%08d_cam.txt
+    (extrinsic 4x4, intrinsic 3x3, depth_min interval depth_num depth_max, as
+    ReadCamera parses them, DPE.cpp:341-382) and pair.txt (GenerateSampleList's
+    format, main.cpp:102-145; score = 1 / (1 + distance between centres))."""
+    import os
+    views = list(range(len(scene.views))) if views is None else list(views)
+    os.makedirs(os.path.join(folder, "images"), exist_ok=True)
+    os.makedirs(os.path.join(folder, "cams"), exist_ok=True)
+    f9 = lambda v: "%.9g" % float(np.float32(v))
+    for i in views:
+        v = scene.views[i]
+        H, W = v.image.shape
+        with open(os.path.join(folder, "images", "%08d.pgm" % i), "wb") as f:
+            f.write(b"P5\n%d %d\n255\n" % (W, H))
+            f.write(np.ascontiguousarray(v.image, np.uint8).tobytes())
+        lines = ["extrinsic"]
+        for r in range(3):
+            lines.append(" ".join(f9(v.R[r, c]) for c in range(3)) + " " + f9(v.t[r]))
+        lines += ["0 0 0 1", "", "intrinsic"]
+        for r in range(3):
+            lines.append(" ".join(f9(v.K[r, c]) for c in range(3)))
+        lines += ["", "%s %s %d %s" % (f9(scene.depth_min), f9((scene.depth_max - scene.depth_min) / 256), 256,
+                                        f9(scene.depth_max))]
+        with open(os.path.join(folder, "cams", "%08d_cam.txt" % i), "w") as f:
+            f.write("\n".join(lines) + "\n")
+    centers = {i: -scene.views[i].R.astype(np.float64).T @ scene.views[i].t.astype(np.float64) for i in views}
+    with open(os.path.join(folder, "pair.txt"), "w") as f:
+        f.write("%d\n" % len(views))
+        for i in views:
+            srcs = [j for j in scene.pairs[i] if j in centers]
+            f.write("%d\n" % i)
+            f.write("%d %s\n" % (len(srcs), " ".join(
+                "%d %.6f" % (j, 1.0 / (1.0 + np.linalg.norm(centers[i] - centers[j]))) for j in srcs)))

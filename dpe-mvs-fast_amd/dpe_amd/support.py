@@ -6,14 +6,17 @@ components for the label map).  OpenCV is absent, so this module is the
 project's own documented detector; it keeps the reference's map semantics and
 file formats, not its pixel values (parity with OpenCV is unpinned):
   * edge map: uint8, 255 on edges, 0 elsewhere.  Sobel gradient magnitude of
-    the image above the `edge_quantile` quantile of the magnitudes, with the
-    one-pixel border copy rule of DPE.cpp:262-273.
+    the image above the order statistic k = int(edge_quantile * (n - 1)) of the
+    n magnitudes (at least 16), with the one-pixel border copy rule of
+    DPE.cpp:262-273.
   * label map: int32, 0 on edge pixels, -1 on regions of at most
     weak_tex_num = rows*cols / (1024 << scale << scale) pixels (DPE.cpp:140,
     282-284), region index > 0 elsewhere.  Edges for the labels come from a
     Roberts cross (DPE.cpp:9-25) on a 4x box-downsampled image, threshold
     robthr = 4 (DPE.cpp:139), upsampled by nearest neighbour; regions are
     4-connected components of non-edge pixels (Connect, DPE.cpp:28-134).
+The C++ host driver restates both maps (dpe-mvs-fast_amd/host/dpe_hostio.cpp)
+and a CPU test checks that the two agree bit for bit.
 """
 import numpy as np
 from scipy import ndimage
@@ -34,7 +37,9 @@ def edge_map(image_u8, edge_quantile=0.85):
     gx = ndimage.sobel(img, axis=1, mode="nearest")
     gy = ndimage.sobel(img, axis=0, mode="nearest")
     mag = np.hypot(gx, gy)
-    thr = max(float(np.quantile(mag, edge_quantile)), 16.0)
+    flat = mag.ravel()
+    k = int(edge_quantile * (flat.size - 1))
+    thr = max(float(np.partition(flat, k)[k]), 16.0)
     e = np.where(mag > thr, 255, 0).astype(np.uint8)
     return _border_rule(e)
 

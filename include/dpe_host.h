@@ -1,0 +1,73 @@
+/*
+ * dpe_host.h -- C entry points of libdpe_host.so, the host side of the C++
+ * driver (dpe-mvs-fast_amd/host/): the reference's file formats, image pyramid,
+ * state rescaling and the support (edge / label) maps, without OpenCV or Boost.
+ * The driver binary `dpe_driver` (main.cpp's CLI and schedule) is built on these
+ * and on libdpe_hip.so.  Pure CPU code: nothing here touches a GPU.
+ *
+ * Matrices follow the reference's .dmb container (ReadBinMat / WriteBinMat,
+ * DPE.cpp:293-339): int32 version = 1, rows, cols, OpenCV type code, then the
+ * raw row-major data.  Type codes used: 0 = CV_8UC1, 4 = CV_32SC1,
+ * 5 = CV_32FC1, 21 = CV_32FC3.
+ */
+#ifndef DPE_HOST_H_
+#define DPE_HOST_H_
+
+#include <stdint.h>
+#include <stddef.h>
+
+#include "dpe_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPE_HOST_CV_8UC1 0
+#define DPE_HOST_CV_32SC1 4
+#define DPE_HOST_CV_32FC1 5
+#define DPE_HOST_CV_32FC3 21
+
+/* Bytes per element of a type code (0 for an unknown code). */
+int dpe_host_type_size(int type);
+
+/* WriteBinMat (DPE.cpp:319-339).  Returns 0 on success. */
+int dpe_host_write_bin_mat(const char *path, int rows, int cols, int type, const void *data);
+/* ReadBinMat (DPE.cpp:293-317): reads the header; with data != NULL also the
+ * payload (capacity bytes available).  Returns 0 on success, -1 unreadable,
+ * -2 version error, -3 capacity too small. */
+int dpe_host_read_bin_mat(const char *path, int *rows, int *cols, int *type, void *data, size_t capacity);
+
+/* ReadCamera (DPE.cpp:341-382): "extrinsic" 4x4, "intrinsic" 3x3, then
+ * depth_min interval depth_num depth_max; c = -R^T t in double.  width/height
+ * are left 0 (the caller sets the image size).  Returns 0 on success. */
+int dpe_host_read_camera(const char *path, dpe_camera *cam);
+
+/* GenerateSampleList (main.cpp:102-145): parses pair.txt.  Fills up to
+ * max_problems entries of ref_ids[] and, per problem, src_counts[i] source ids
+ * (scores <= 0 dropped) into srcs[i * DPE_MAX_IMAGES ...].  Returns the number
+ * of problems, or -1 if the file cannot be read. */
+int dpe_host_read_pairs(const char *path, int max_problems, int32_t *ref_ids, int32_t *src_counts, int32_t *srcs);
+
+/* cv::resize(..., INTER_LINEAR) of a float image (SceneCache::GetGrayFloatImage,
+ * DPE.cpp:720-737): half-pixel centres, clamped borders.  A factor-2 reduction
+ * of an even-sized image is the 2x2 box mean. */
+void dpe_host_resize_linear(const float *src, int w, int h, float *dst, int dw, int dh);
+
+/* RescaleMatToTargetSize (DPE.cpp:1313-1335): nearest, o = int(r / s);
+ * elem_size bytes per element. */
+void dpe_host_rescale_nearest(const void *src, int w, int h, void *dst, int dw, int dh, int elem_size);
+
+/* The support maps the weak-texture path consumes (replaces EdgeSegment,
+ * DPE.cpp:136-291, with this project's detector; same semantics as
+ * dpe_amd/support.py): edges u8 (255 = edge), labels i32 (0 = edge,
+ * -1 = small region, > 0 = region).  Either output may be NULL. */
+void dpe_host_support_maps(const uint8_t *image, int w, int h, int scale, uint8_t *edges, int32_t *labels);
+
+/* ExportPointCloud (DPE.cpp:532-572): binary little-endian PLY, xyz float +
+ * bgr uchar per vertex.  Returns 0 on success. */
+int dpe_host_write_ply(const char *path, const dpe_point *points, int64_t count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DPE_HOST_H_ */
