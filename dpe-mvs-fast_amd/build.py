@@ -11,9 +11,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libdpe_hip.so")
-SOURCES = [os.path.join(HERE, "csrc", "dpe_kernels.hip"), os.path.join(HERE, "csrc", "dpe_fuse.hip"),
-           os.path.join(HERE, "csrc", "dpe_capi.hip")]
+SOURCES = [os.path.join(HERE, "csrc", "dpe_kernels.hip"), os.path.join(HERE, "csrc", "dpe_adjoint.hip"),
+           os.path.join(HERE, "csrc", "dpe_fuse.hip"), os.path.join(HERE, "csrc", "dpe_capi.hip")]
 HEADERS = [os.path.join(HERE, "csrc", "dpe_device.h"), os.path.join(HERE, "csrc", "dpe_state.h"),
+           os.path.join(HERE, "csrc", "dpe_ncc.h"),
            os.path.join(HERE, "csrc", "dpe_fuse.h"),
            os.path.join(ROOT, "include", "dpe_hip.h")]
 

@@ -43,7 +43,8 @@ enum BufId {
     B_LABEL, B_CANDPOS, B_COSTMAT, B_NSEL, B_RPLANE, B_RCOST, B_NOWPLANE, B_NOWDEPTH, B_NOWCOST, B_WNORM, B_POSTOP,
     B_POSTBL, B_POSTDISP, B_POSTWN, B_POSTFLAGS, B_PCOST, B_COUNTERS, B_GIVEN_PLANES, B_GIVEN_COSTS,
     B_F_VIEWS, B_F_DEPTH, B_F_NORMAL, B_F_STATES, B_F_COLOR, B_F_MASK, B_F_CLAIM, B_F_USED0, B_F_USED1, B_F_ACC0,
-    B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_POSTPLANES, B_WEAKLIST, B_COUNT
+    B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_POSTPLANES, B_WEAKLIST, B_ADJ_WEIGHTS,
+    B_ADJ_GRADS, B_COUNT
 };
 
 struct Fail {
@@ -461,6 +462,54 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *i
     return DPE_OK;
 }
 
+dpe_status ncc_adjoint_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in, const float *plane_params,
+                            const float *view_weights, float *costs, float *grads) {
+    dpe_params p = *prm;
+    p.use_APD = p.use_edge = p.use_limit = p.use_label = p.geom_consistency = 0;
+    p.state = DPE_FIRST_INIT;
+    dpe_status st = validate(ctx, &p, in);
+    if (st != DPE_OK) return st;
+    if (!plane_params || !costs || !grads) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "null plane parameters or outputs");
+    if (2 * p.strong_radius / p.strong_increment + 1 != 6)
+        return fail(ctx, DPE_ERR_UNSUPPORTED, "the adjoint needs the 11x11 step-2 window");
+    HIPCHK(hipSetDevice(ctx->device));
+    EventTimer tm{ctx};
+    Dev d = setup_views(ctx, &p, in);
+    const size_t P = d.P;
+    const bool dev = in->memory == DPE_MEM_DEVICE;
+    const float4 *tpd = (const float4 *)plane_params;
+    const float *vw = view_weights;
+    float *co = costs;
+    float4 *gr = (float4 *)grads;
+    if (!dev) {
+        float4 *b_tpd = (float4 *)buffer(ctx, B_GIVEN_PLANES, sizeof(float4) * P);
+        HIPCHK(hipMemcpyAsync(b_tpd, plane_params, sizeof(float4) * P, hipMemcpyHostToDevice, ctx->stream));
+        tpd = b_tpd;
+        if (view_weights) {
+            float *b_vw = (float *)buffer(ctx, B_ADJ_WEIGHTS, sizeof(float) * P * d.NV);
+            HIPCHK(hipMemcpyAsync(b_vw, view_weights, sizeof(float) * P * d.NV, hipMemcpyHostToDevice, ctx->stream));
+            vw = b_vw;
+        }
+        co = (float *)buffer(ctx, B_GIVEN_COSTS, sizeof(float) * P);
+        gr = (float4 *)buffer(ctx, B_ADJ_GRADS, sizeof(float4) * P);
+    }
+    d.counters = (unsigned long long *)buffer(ctx, B_COUNTERS, sizeof(unsigned long long) * kNumCounters);
+    HIPCHK(hipMemsetAsync(d.counters, 0, sizeof(unsigned long long) * kNumCounters, ctx->stream));
+    tm.ncc_launch(kFamFull, [&] { launch_ncc_adjoint(d, tpd, vw, co, gr, ctx->stream); });
+    if (!dev) {
+        HIPCHK(hipMemcpyAsync(costs, co, sizeof(float) * P, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(grads, gr, sizeof(float4) * P, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    unsigned long long cnt[kNumCounters] = {};
+    HIPCHK(hipMemcpyAsync(cnt, d.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    dpe_stats stt{};
+    tm.fill(stt, cnt);
+    stt.gpu_ms = stt.ncc_ms;
+    ctx->stats = stt;
+    return DPE_OK;
+}
+
 dpe_status ncc_cost_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in, int num_hyps, const float *planes,
                          float *costs) {
     dpe_params p = *prm;
@@ -673,6 +722,16 @@ dpe_status dpe_ncc_cost(dpe_ctx *ctx, const dpe_params *params, const dpe_inputs
     if (!ctx) return DPE_ERR_INVALID_ARGUMENT;
     try {
         return ncc_cost_impl(ctx, params, in, num_hyps, planes, costs);
+    } catch (const Fail &f) {
+        return f.st;
+    }
+}
+
+dpe_status dpe_ncc_cost_adjoint(dpe_ctx *ctx, const dpe_params *params, const dpe_inputs *in,
+                                const float *plane_params, const float *view_weights, float *costs, float *grads) {
+    if (!ctx) return DPE_ERR_INVALID_ARGUMENT;
+    try {
+        return ncc_adjoint_impl(ctx, params, in, plane_params, view_weights, costs, grads);
     } catch (const Fail &f) {
         return f.st;
     }

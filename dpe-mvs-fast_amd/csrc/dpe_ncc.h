@@ -1,0 +1,153 @@
+// dpe_ncc.h -- the shared pieces of the NCC kernels (dpe_kernels.hip,
+// dpe_adjoint.hip): checkerboard pixel indexing, wave-reduced counters, the
+// workgroup-shared reference patch and the source sums of one window.
+#pragma once
+#include "dpe_state.h"
+
+namespace dpe {
+
+__device__ __forceinline__ bool colour_pixel(const Dev &d, int colour, int q, int &x, int &y) {
+    if (colour < 0) {
+        if (q >= d.P) return false;
+        y = q / d.W;
+        x = q - y * d.W;
+        return true;
+    }
+    if (q >= d.Q) return false;
+    y = q / d.Wh;
+    const int k = q - y * d.Wh;
+    x = 2 * k + ((y + colour) & 1);
+    return y < d.half_rows && x < d.W;
+}
+
+// Wave-reduced NCC counts of one kernel family; every lane of the wave must call.
+__device__ __forceinline__ void count_add(unsigned long long *counters, int family, unsigned long long hyps,
+                                          unsigned long long taps) {
+    if (!counters) return;
+    unsigned long long *c = counters + 2 * family;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        hyps += __shfl_down(hyps, off, 64);
+        taps += __shfl_down(taps, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (hyps) atomicAdd(c, hyps);
+        if (taps) atomicAdd(c + 1, taps);
+    }
+}
+
+
+// ------------------------------------------------------------ NCC windows
+// ---- workgroup-shared reference patches (the hot-path layout)
+// A workgroup covers 64 consecutive pixels (one per lane) and 4 waves.  The
+// waves fill the 64 patches into LDS together (tap k of lane l at [k][l], so a
+// wave reads 64 consecutive float2: no bank conflicts), then every wave loops
+// over hypothesis slots of the same 64 pixels, reading the weights from LDS.
+// Taps of one window row evaluated together (registers vs. loads in flight).
+#ifndef DPE_TAP_UNROLL
+#define DPE_TAP_UNROLL 6
+#endif
+// Bilinear taps read the quad layout of the source view: one 16-B load per tap.
+// (Measured at 1600x1200 with 10 sources: two 8-B pairs from the plain image,
+// a quarter of the footprint, ran 4-15 % slower.)
+__device__ __forceinline__ float sample_src(const Dev &d, int v, float u, float w) {
+    return bilinear_quad(d.quads + (size_t)v * d.qs, d.W, d.H, u, w);
+}
+constexpr int kLanes = 64;
+constexpr int kWaves = 4;
+
+template <int NT>
+struct SharedPatch {
+    float2 wwr[NT * NT][kLanes];  // (w, w*ref) per tap
+    float sum_ref[kLanes], sum_ref_ref[kLanes], wsum[kLanes];
+};
+
+template <int NT>
+__device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int x, int y, SharedPatch<NT> &S) {
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
+    const float sc = d.prm.sigma_color;
+    const float cden = 2.0f * sc * sc;
+    const float rc = valid ? texel(d.ref, d.W, d.H, x, y) : 0.0f;
+    for (int k = wave; k < NT * NT; k += kWaves) {
+        const int ki = k / NT, kj = k - ki * NT;
+        float2 e = make_float2(0.0f, 0.0f);
+        if (valid) {
+            const float ref = texel(d.ref, d.W, d.H, x + d.tap_off0 + ki * d.tap_inc, y + d.tap_off0 + kj * d.tap_inc);
+            const float w = d_expf(d.spatial[k] - fabsf(ref - rc) / cden);
+            e = make_float2(w, w * ref);
+        }
+        S.wwr[k][lane] = e;
+    }
+    __syncthreads();
+    if (wave == 0 && valid) {
+        // hypothesis-independent sums in the reference's row-partial order
+        float s_r = 0.0f, s_rr = 0.0f, s_w = 0.0f;
+        for (int ki = 0; ki < NT; ++ki) {
+            float r_r = 0.0f, r_rr = 0.0f, r_w = 0.0f;
+#pragma unroll
+            for (int kj = 0; kj < NT; ++kj) {
+                const float2 e = S.wwr[ki * NT + kj][lane];
+                const float ref = texel(d.ref, d.W, d.H, x + d.tap_off0 + ki * d.tap_inc, y + d.tap_off0 + kj * d.tap_inc);
+                r_r = fmaf(e.x, ref, r_r);
+                r_rr = fmaf(e.y, ref, r_rr);
+                r_w = r_w + e.x;
+            }
+            s_r += r_r;
+            s_rr += r_rr;
+            s_w += r_w;
+        }
+        S.sum_ref[lane] = s_r;
+        S.sum_ref_ref[lane] = s_rr;
+        S.wsum[lane] = s_w;
+    }
+    __syncthreads();
+}
+
+// Source sums of one window (DPE.cu:728-758) with the reference weights from
+// LDS; REFINED: the projection uses the refined division (exact, see
+// refined_division_ok).
+template <int NT, bool REFINED>
+__device__ __forceinline__ void shared_window_sums(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y,
+                                                   int v, const float H[9], float &s_s, float &s_ss, float &s_rs) {
+#pragma unroll 1
+    for (int ki = 0; ki < NT; ++ki) {
+        float r_s = 0.0f, r_ss = 0.0f, r_rs = 0.0f;
+        const int rx = x + d.tap_off0 + ki * d.tap_inc;
+#pragma unroll DPE_TAP_UNROLL
+        for (int kj = 0; kj < NT; ++kj) {
+            const int ry = y + d.tap_off0 + kj * d.tap_inc;
+            const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
+            const float src = sample_src(d, v, sp.x, sp.y);
+            const float2 e = S.wwr[ki * NT + kj][lane];
+            r_s = fmaf(e.x, src, r_s);
+            r_ss = fmaf(e.x * src, src, r_ss);
+            r_rs = fmaf(e.y, src, r_rs);
+        }
+        s_s += r_s;
+        s_ss += r_ss;
+        s_rs += r_rs;
+    }
+}
+
+// ComputeBilateralNCCOld (DPE.cu:696-782) with the reference half from LDS.
+template <int NT>
+__device__ __forceinline__ float ncc_shared(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y, int v,
+                                            const float4 &pl, unsigned long long &taps) {
+    const ViewConst &vc = d.vc[v];
+    float H[9];
+    homography(vc, d.cams[0], pl, H);
+    const float2 pt = project(H, x, y);
+    if (pt.x >= (float)vc.width || pt.x < 0.0f || pt.y >= (float)vc.height || pt.y < 0.0f) return 2.0f;
+    float s_s = 0.0f, s_ss = 0.0f, s_rs = 0.0f;
+    const int x0 = x + d.tap_off0, y0 = y + d.tap_off0, span = (NT - 1) * d.tap_inc;
+    if (refined_division_ok(H, x0, x0 + span, y0, y0 + span))
+        shared_window_sums<NT, true>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+    else
+        shared_window_sums<NT, false>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+    taps += NT * NT;
+    return ncc_from_sums(S.sum_ref[lane], S.sum_ref_ref[lane], s_s, s_ss, s_rs, S.wsum[lane]);
+}
+
+
+}  // namespace dpe

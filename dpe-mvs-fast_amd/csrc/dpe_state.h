@@ -112,5 +112,8 @@ void launch_post_planes(const Dev &d, hipStream_t s);
 void launch_ncc_post(const Dev &d, hipStream_t s);
 void launch_post_final(const Dev &d, hipStream_t s);
 void launch_ncc_given(const Dev &d, int num_hyps, const float4 *planes, float *costs, hipStream_t s);
+// dpe_adjoint.hip: aggregated cost and its (theta, phi, depth) gradient per pixel
+void launch_ncc_adjoint(const Dev &d, const float4 *tpd, const float *view_w, float *cost, float4 *grad,
+                        hipStream_t s);
 
 }  // namespace dpe

@@ -52,7 +52,7 @@ class Outputs(C.Structure):
 
 NCC_FAMILIES = 4
 FAMILY_FULL, FAMILY_REFINE, FAMILY_POST, FAMILY_WEAK = range(NCC_FAMILIES)
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 class Stats(C.Structure):
@@ -141,6 +141,9 @@ def declare(lib):
     lib.dpe_last_stats.restype = C.c_int
     lib.dpe_ncc_cost.argtypes = [C.c_void_p, P(Params), P(Inputs), C.c_int, C.c_void_p, C.c_void_p]
     lib.dpe_ncc_cost.restype = C.c_int
+    lib.dpe_ncc_cost_adjoint.argtypes = [C.c_void_p, P(Params), P(Inputs), C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]
+    lib.dpe_ncc_cost_adjoint.restype = C.c_int
     lib.dpe_fuse.argtypes = [C.c_void_p, P(FuseView), C.c_int, P(FuseParams), C.c_void_p, C.c_int64,
                              P(C.c_int64)]
     lib.dpe_fuse.restype = C.c_int
@@ -149,4 +152,4 @@ def declare(lib):
 
 # Every symbol include/dpe_hip.h declares.
 EXPORTS = ["dpe_create", "dpe_destroy", "dpe_last_error", "dpe_abi_version", "dpe_process", "dpe_last_stats",
-           "dpe_ncc_cost", "dpe_fuse"]
+           "dpe_ncc_cost", "dpe_ncc_cost_adjoint", "dpe_fuse"]

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DPE_ABI_VERSION 2
+#define DPE_ABI_VERSION 3
 /* main.h:41-44 */
 #define DPE_MAX_IMAGES 32
 #define DPE_NEIGHBOUR_NUM 9
@@ -167,6 +167,22 @@ dpe_status dpe_last_stats(const dpe_ctx *ctx, dpe_stats *stats);
  * cost against source view v+1.  num_hyps >= 1. */
 dpe_status dpe_ncc_cost(dpe_ctx *ctx, const dpe_params *params, const dpe_inputs *in,
                         int num_hyps, const float *planes, float *costs);
+
+/* Adjoint form (no counterpart in the reference, which has no derivatives):
+ * the aggregated multi-view NCC cost C = sum_v w_v c_v / sum_v w_v of every
+ * pixel's plane, c_v = ComputeBilateralNCCOld (DPE.cu:696-782), and its
+ * gradient with respect to the plane parameters.
+ *   plane_params: float4 per pixel (theta, phi, depth, unused); the
+ *     reference-camera normal is n = (sin t cos p, sin t sin p, -cos t) and
+ *     depth is the plane's z at the pixel (d = -n . depth K^-1 [x, y, 1]).
+ *   view_weights: float [num_images-1][H*W] (w_v per pixel), or NULL for 1.
+ *   costs: float [H*W]; grads: float4 per pixel (dC/dtheta, dC/dphi,
+ *     dC/ddepth, 0).  Views whose centre projects outside the source, whose
+ *     patch variance is under the floor or whose cost is clamped to 0 or 2
+ *     contribute their cost and no gradient.
+ * Needs the 11x11 step-2 window (strong_radius 5, strong_increment 2). */
+dpe_status dpe_ncc_cost_adjoint(dpe_ctx *ctx, const dpe_params *params, const dpe_inputs *in,
+                                const float *plane_params, const float *view_weights, float *costs, float *grads);
 
 /* Fusion (RunFusion, DPE.cpp:1387-1548).  One entry per reference image in
  * pair order; src_index lists indices into the same array. */

@@ -1,0 +1,268 @@
+"""The adjoint form on the GPU: the aggregated 10-view NCC cost of each pixel's
+(theta, phi, depth) and its gradient (dpe_ncc_cost_adjoint), on 320x240 inputs
+with 10 source views.
+
+Checks (gradient tolerances written here):
+  * the cost equals the mean of dpe_ncc_cost's per-view costs for the same plane;
+  * the gradient matches an FP64 reference gradient on 10^4 seeded pixels: the
+    complex-step derivative of a float64 NumPy restatement of the cost (exact to
+    rounding, and independent of the kernel's hand-derived chain rule);
+  * central finite differences of the same float64 cost: with relative step
+    1e-5 they agree with the FP64 gradient where no tap crosses a bilinear cell
+    edge and no clamp, variance floor or bounds test switches (kink-adjacent
+    pixels counted and logged); with relative step 1e-3 the angle derivatives
+    agree with the GPU's and the depth derivative in median (the depth step
+    moves taps across cell edges, averaging the piecewise-linear slope);
+  * the torch.autograd.Function wrapper's backward is the adjoint's gradient,
+    and a gradient step lowers the cost of most pixels.
+"""
+import ctypes as C
+import warnings
+
+import numpy as np
+import pytest
+
+from dpe_amd import abi, host, scene
+
+pytestmark = pytest.mark.gpu
+
+OFFS = np.array([-5, -3, -1, 1, 3, 5], np.float64)
+
+
+def _setup():
+    s = scene.make_scene(320, 240, 300.0, 11, 30.0, seed=0x5EED0004, n_src=10)
+    ref = s.meta["ref_mid"]
+    ids = [ref] + s.pairs[ref]
+    imgs = [s.views[i].image.astype(np.float32) for i in ids]
+    cams = host.scene_cameras(s, ids)
+    prm = host.call_params(s.depth_min, s.depth_max, use_APD=0, use_edge=0, use_limit=0, use_label=0)
+    pin = host.ProblemInputs(imgs, cams, prm)
+    return s, ref, ids, imgs, cams, pin
+
+
+def _params_from_gt(s, ref, rng):
+    v = s.views[ref]
+    n = v.normal @ v.R.T  # camera frame, facing the camera (n_z < 0)
+    theta = np.arccos(np.clip(-n[..., 2], -1, 1)) + 0.05 * rng.standard_normal(n.shape[:2])
+    phi = np.arctan2(n[..., 1], n[..., 0]) + 0.05 * rng.standard_normal(n.shape[:2])
+    depth = v.depth * (1 + 0.01 * rng.standard_normal(n.shape[:2]))
+    return np.stack([theta, phi, depth], -1).astype(np.float32)
+
+
+def _cam64(c):
+    K = np.array(c.K, np.float64).reshape(3, 3)
+    R = np.array(c.R, np.float64).reshape(3, 3)
+    t = np.array(c.t, np.float64)
+    return K, R, t
+
+
+def _bilinear(im, u, v):
+    Hh, Ww = im.shape
+    fx = np.floor(u.real)
+    fy = np.floor(v.real)
+    a = u - fx
+    b = v - fy
+    ix = np.clip(fx, -1, Ww - 1).astype(np.int64)
+    iy = np.clip(fy, -1, Hh - 1).astype(np.int64)
+    x0, x1 = np.clip(ix, 0, Ww - 1), np.clip(ix + 1, 0, Ww - 1)
+    y0, y1 = np.clip(iy, 0, Hh - 1), np.clip(iy + 1, 0, Hh - 1)
+    top = im[y0, x0] + a * (im[y0, x1] - im[y0, x0])
+    bot = im[y1, x0] + a * (im[y1, x1] - im[y1, x0])
+    return top + b * (bot - top), (ix, iy)
+
+
+def agg_cost64(pix, tpd, imgs, cams):
+    """float64 (or complex128) restatement of the aggregated cost; returns the
+    cost and a state signature (tap cells, validity, clamps) for kink tests."""
+    warnings.filterwarnings("ignore", category=RuntimeWarning)  # degenerate views give 0/0, masked below
+    x = pix[:, 0].astype(np.float64)
+    y = pix[:, 1].astype(np.float64)
+    th, ph, D = tpd[:, 0], tpd[:, 1], tpd[:, 2]
+    n = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), -np.cos(th)], -1)
+    Kr, Rr_, tr_ = _cam64(cams[0])
+    ray = np.stack([(x - Kr[0, 2]) / Kr[0, 0], (y - Kr[1, 2]) / Kr[1, 1], np.ones_like(x)], -1)
+    d = -D * np.sum(n * ray, -1)
+    nu = n / d[:, None]
+    oi, oj = np.meshgrid(OFFS, OFFS, indexing="ij")
+    oi, oj = oi.ravel(), oj.ravel()
+    X = x[:, None] + oi[None]
+    Y = y[:, None] + oj[None]
+    ref = imgs[0].astype(np.float64)
+    Hh, Ww = ref.shape
+    r = ref[np.clip(Y, 0, Hh - 1).astype(int), np.clip(X, 0, Ww - 1).astype(int)]
+    rc = ref[pix[:, 1], pix[:, 0]]
+    w = np.exp(-np.sqrt(oi ** 2 + oj ** 2)[None] / 50.0 - np.abs(r - rc[:, None]) / 18.0)
+    Wt = w.sum(1)
+    mu_r = (w * r).sum(1) / Wt
+    var_r = (w * r * r).sum(1) / Wt - mu_r ** 2
+    Kr_inv = np.linalg.inv(Kr)
+    Cr = -Rr_.T @ tr_
+    costs = []
+    sig = []
+    for k in range(1, len(imgs)):
+        Ks, Rs, ts = _cam64(cams[k])
+        Cs = -Rs.T @ ts
+        Rrel = Rs @ Rr_.T
+        trel = Rs @ (Cr - Cs)
+        Gm = Rrel[None] - trel[None, :, None] * nu[:, None, :]
+        Hm = Ks[None] @ Gm @ Kr_inv[None]
+        hc = Hm @ np.stack([x, y, np.ones_like(x)], -1)[..., None]
+        qcx, qcy = hc[:, 0, 0] / hc[:, 2, 0], hc[:, 1, 0] / hc[:, 2, 0]
+        valid = (qcx.real >= 0) & (qcx.real < Ww) & (qcy.real >= 0) & (qcy.real < Hh)
+        u = Hm[:, 0, 0, None] * X + Hm[:, 0, 1, None] * Y + Hm[:, 0, 2, None]
+        v = Hm[:, 1, 0, None] * X + Hm[:, 1, 1, None] * Y + Hm[:, 1, 2, None]
+        ww = Hm[:, 2, 0, None] * X + Hm[:, 2, 1, None] * Y + Hm[:, 2, 2, None]
+        src = imgs[k].astype(np.float64)
+        sv, cells = _bilinear(src, u / ww, v / ww)
+        mu_s = (w * sv).sum(1) / Wt
+        var_s = (w * sv * sv).sum(1) / Wt - mu_s ** 2
+        cov = (w * r * sv).sum(1) / Wt - mu_r * mu_s
+        c = 1 - cov / np.sqrt(var_r * var_s)
+        floor = (var_r < 1e-5) | (var_s.real < 1e-5)
+        lo, hi = c.real <= 0, c.real >= 2
+        c = np.where(lo, 0.0, np.where(hi, 2.0, c))
+        c = np.where(floor | ~valid, 2.0, c)
+        costs.append(c)
+        sig.append((cells[0], cells[1], valid, floor, lo, hi))
+    return np.mean(costs, 0), sig
+
+
+def _same_cells(a, b):
+    ok = np.ones(a[0][0].shape[0], bool)
+    for sa, sb in zip(a, b):
+        ok &= np.all(sa[0] == sb[0], 1) & np.all(sa[1] == sb[1], 1)
+    return ok
+
+
+def _same_state(a, b):
+    ok = np.ones(a[0][0].shape[0], bool)
+    for sa, sb in zip(a, b):
+        for k in range(2, 6):
+            ok &= sa[k] == sb[k]
+    return ok
+
+
+def test_adjoint_gradient_against_fp64_and_finite_differences():
+    s, ref, ids, imgs, cams, pin = _setup()
+    H, W = s.height, s.width
+    rng = np.random.default_rng(11)
+    tpd = _params_from_gt(s, ref, rng)
+    planes = np.zeros((H, W, 4), np.float32)
+    planes[..., :3] = tpd
+    costs = np.zeros((H, W), np.float32)
+    grads = np.zeros((H, W, 4), np.float32)
+    ctx = host.Context(0)
+    try:
+        ctx.check(ctx.lib.dpe_ncc_cost_adjoint(ctx.h, C.byref(pin.params), C.byref(pin.struct), planes.ctypes.data,
+                                               None, costs.ctypes.data, grads.ctypes.data))
+        # forward consistency with dpe_ncc_cost on the same (n, d) plane
+        th, ph, D = tpd[..., 0], tpd[..., 1], tpd[..., 2]
+        n = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), -np.cos(th)], -1).astype(np.float32)
+        K = s.views[ref].K
+        ys, xs = np.mgrid[0:H, 0:W]
+        X = np.stack([D * (xs - K[0, 2]) / K[0, 0], D * (ys - K[1, 2]) / K[1, 1], D], -1)
+        nd = np.concatenate([n, -np.sum(n * X, -1, keepdims=True)], -1).astype(np.float32)[None]
+        NV = len(ids) - 1
+        per_view = np.zeros((1, NV, H, W), np.float32)
+        ctx.check(ctx.lib.dpe_ncc_cost(ctx.h, C.byref(pin.params), C.byref(pin.struct), 1, nd.ctypes.data,
+                                       per_view.ctypes.data))
+    finally:
+        ctx.close()
+    mean_cost = per_view[0].mean(0)
+    agree = float((np.abs(costs - mean_cost) < 1e-3).mean())
+    print(f"adjoint cost vs mean of dpe_ncc_cost: {agree:.5f} within 1e-3")
+    assert agree >= 0.999
+
+    # 10^4 seeded interior pixels
+    ny, nx = np.mgrid[8:H - 8, 8:W - 8]
+    cand = np.stack([nx.ravel(), ny.ravel()], -1)
+    pix = cand[rng.choice(len(cand), 10000, replace=False)]
+    p0 = tpd[pix[:, 1], pix[:, 0]].astype(np.float64)
+    g_gpu = grads[pix[:, 1], pix[:, 0], :3].astype(np.float64)
+    c64, sig0 = agg_cost64(pix, p0, imgs, cams)
+    # FP64 reference gradient: complex step
+    g_ref = np.zeros_like(g_gpu)
+    for k in range(3):
+        pc = p0.astype(np.complex128)
+        pc[:, k] += 1e-30j
+        g_ref[:, k] = agg_cost64(pix, pc, imgs, cams)[0].imag / 1e-30
+    scale = np.median(np.abs(g_ref), 0)
+    err = np.abs(g_gpu - g_ref) / (np.abs(g_ref) + 0.01 * scale)
+    good = np.all(err < 1e-2, 1)
+    print(f"GPU gradient vs FP64 complex-step gradient: {good.mean():.5f} of pixels within 1e-2 (relative); "
+          f"median relative error {np.median(err):.2e}")
+    assert good.mean() >= 0.99
+    # finite differences of the float64 cost
+    def fd(rel):
+        steps = np.stack([np.full(len(p0), rel), np.full(len(p0), rel), rel * p0[:, 2]], -1)
+        g = np.zeros_like(g_gpu)
+        cells_same = np.ones(len(p0), bool)
+        state_same = np.ones(len(p0), bool)
+        for k in range(3):
+            pp, pm = p0.copy(), p0.copy()
+            pp[:, k] += steps[:, k]
+            pm[:, k] -= steps[:, k]
+            cp, sp = agg_cost64(pix, pp, imgs, cams)
+            cm, sm = agg_cost64(pix, pm, imgs, cams)
+            g[:, k] = (cp - cm) / (2 * steps[:, k])
+            for other in (sp, sm):
+                cells_same &= _same_cells(sig0, other)
+                state_same &= _same_state(sig0, other)
+        return g, cells_same, state_same
+    # (a) a fine step (relative 1e-5) pins the complex-step gradient where no tap changes cell
+    g5, cells5, state5 = fd(1e-5)
+    smooth5 = cells5 & state5
+    ok5 = np.all(np.abs(g5 - g_ref) <= 1e-3 * (np.abs(g_ref) + 0.01 * scale), 1)
+    print(f"FD (rel. step 1e-5) vs FP64 complex step: {ok5[smooth5].mean():.5f} on {int(smooth5.sum())} "
+          f"kink-free pixels; {int((~smooth5).sum())} kink-adjacent excluded")
+    assert smooth5.sum() >= 1000 and ok5[smooth5].mean() >= 0.99
+    # (b) relative step 1e-3: the depth step moves the taps by ~0.1 px, across bilinear cell edges
+    # (kinks of the piecewise-linear sample) for almost every pixel, so the difference quotient is
+    # a slope averaged over the step: theta / phi must agree to 1e-2, depth in median to 5e-2.
+    g3, cells3, state3 = fd(1e-3)
+    kinks = int((~state3).sum())
+    print(f"FD (rel. step 1e-3): {kinks} pixels with a clamp / variance floor / bounds switch excluded; "
+          f"{int((~cells3).sum())} of {len(p0)} have a tap changing bilinear cell (logged, kept)")
+    rel3 = np.abs(g_gpu - g3) / (np.abs(g3) + 0.01 * scale)
+    ang_ok = np.all(rel3[:, :2] < 1e-2, 1)[state3].mean()
+    med_depth = float(np.median(rel3[state3, 2]))
+    print(f"GPU vs FD(1e-3): theta/phi within 1e-2 on {ang_ok:.5f}; depth median relative difference {med_depth:.2e}")
+    assert ang_ok >= 0.99
+    assert med_depth <= 5e-2
+
+
+def test_autograd_wrapper_and_descent():
+    import torch
+    from ncc_autograd import AggregatedNCC, device_inputs
+    s, ref, ids, imgs, cams, pin = _setup()
+    rng = np.random.default_rng(12)
+    tpd_np = _params_from_gt(s, ref, rng)
+    ctx = host.Context(0)
+    try:
+        inputs, keep = device_inputs(pin)
+        tpd = torch.from_numpy(tpd_np).to("cuda:0").requires_grad_(True)
+        cost = AggregatedNCC.apply(tpd, None, ctx, pin.params, inputs)
+        weights = torch.rand(cost.shape, device="cuda:0")
+        (cost * weights).sum().backward()
+        planes = torch.zeros(tpd.shape[:2] + (4,), device="cuda:0")
+        planes[..., :3] = tpd.detach()
+        c2 = torch.empty_like(cost)
+        g2 = torch.empty(tpd.shape[:2] + (4,), device="cuda:0")
+        torch.cuda.synchronize()
+        ctx.check(ctx.lib.dpe_ncc_cost_adjoint(ctx.h, C.byref(pin.params), C.byref(inputs), planes.data_ptr(), None,
+                                               c2.data_ptr(), g2.data_ptr()))
+        torch.cuda.synchronize()
+        assert torch.equal(tpd.grad, weights[..., None] * g2[..., :3])
+        # a small normalised gradient step lowers the cost of most non-degenerate pixels
+        g = g2[..., :3]
+        scale = torch.tensor([0.01, 0.01, 0.005], device="cuda:0")
+        step = -scale * g / (g.abs().amax(dim=(0, 1)) + 1e-12)
+        step[..., 2] *= tpd[..., 2].detach()
+        with torch.no_grad():
+            c_new = AggregatedNCC.apply(tpd.detach() + step, None, ctx, pin.params, inputs)
+        live = (c2 < 1.5) & (g.abs().sum(-1) > 0)
+        frac = float((c_new[live] <= c2[live]).float().mean())
+        print(f"gradient step lowers the cost on {frac:.3f} of {int(live.sum())} pixels")
+        assert frac > 0.8
+    finally:
+        ctx.close()
