@@ -98,6 +98,25 @@ __device__ __forceinline__ float ncc_adjoint_view(const Dev &d, const SharedPatc
     return c;
 }
 
+// Chain rule from g_nu = dC/dnu (nu = n / d) to (dC/dtheta, dC/dphi, dC/ddepth)
+// at the plane n(theta, phi), d = -depth (n . r), r = K^-1 [x, y, 1].
+__device__ __forceinline__ float4 grad_theta_phi_depth(const dpe_camera &cam, int x, int y, float st, float ct,
+                                                      float sp, float cp, float D, const float4 &n, float dd,
+                                                      const float gnu[3]) {
+    float4 grad = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (dd == 0.0f) return grad;
+    const float id = 1.0f / dd;
+    const float gd = -(gnu[0] * n.x + gnu[1] * n.y + gnu[2] * n.z) * id * id;
+    const float r0 = ((float)x - cam.K[2]) / cam.K[0], r1 = ((float)y - cam.K[5]) / cam.K[4];
+    const float gn0 = gnu[0] * id - gd * D * r0;
+    const float gn1 = gnu[1] * id - gd * D * r1;
+    const float gn2 = gnu[2] * id - gd * D;
+    grad.z = -gd * (n.x * r0 + n.y * r1 + n.z);
+    grad.x = gn0 * (ct * cp) + gn1 * (ct * sp) + gn2 * st;
+    grad.y = gn0 * (-st * sp) + gn1 * (st * cp);
+    return grad;
+}
+
 // One workgroup = 64 pixels x 4 waves; wave w takes views 1+w, 5+w, ...; the
 // waves' partial sums meet in LDS.
 template <int NT>
@@ -145,25 +164,92 @@ __global__ __launch_bounds__(kLanes * kWaves) void k_ncc_adjoint(Dev d, const fl
         float4 grad = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (s[1] > 0.0f) {
             C = s[0] / s[1];
-            if (dd != 0.0f) {
-                const float gnu[3] = {s[2] / s[1], s[3] / s[1], s[4] / s[1]};
-                // nu = n / d
-                const float id = 1.0f / dd;
-                const float gd = -(gnu[0] * n.x + gnu[1] * n.y + gnu[2] * n.z) * id * id;
-                // d = -depth (n . r), r = K^-1 [x, y, 1]
-                const float r0 = ((float)x - cam.K[2]) / cam.K[0], r1 = ((float)y - cam.K[5]) / cam.K[4];
-                const float gn0 = gnu[0] * id - gd * D * r0;
-                const float gn1 = gnu[1] * id - gd * D * r1;
-                const float gn2 = gnu[2] * id - gd * D;
-                grad.z = -gd * (n.x * r0 + n.y * r1 + n.z);
-                grad.x = gn0 * (ct * cp) + gn1 * (ct * sp) + gn2 * st;
-                grad.y = gn0 * (-st * sp) + gn1 * (st * cp);
-            }
+            const float gnu[3] = {s[2] / s[1], s[3] / s[1], s[4] / s[1]};
+            grad = grad_theta_phi_depth(cam, x, y, st, ct, sp, cp, D, n, dd, gnu);
         }
         cost_out[p] = C;
         grad_out[p] = grad;
     }
     count_add(d.counters, kFamFull, hyps, taps);
+}
+
+// One Gauss-Newton step of the aggregated cost (the view weights of the pixel's
+// view selection) for the refinement of the strong half-iteration: from the
+// camera-frame plane src[q] to dst[q].  With one residual C >= 0 the step is
+// delta = -C g / (g . g) in the scaled parameters (theta, phi, depth / depth0),
+// limited to 0.05 rad and 3 % of the depth; the acceptance test of the
+// refinement (the hypothesis must lower the cost) keeps only improvements.
+// Pixels without a gradient keep the source plane.
+template <int NT>
+__global__ __launch_bounds__(kLanes * kWaves) void k_gn_step(Dev d, int colour, const float4 *__restrict__ src,
+                                                             float4 *__restrict__ dst) {
+    __shared__ SharedPatch<NT> S;
+    __shared__ float red[kWaves][5][kLanes];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
+    const int q = blockIdx.x * kLanes + lane;
+    int x = 0, y = 0;
+    const bool valid = q < d.Q && colour_pixel(d, colour, q, x, y) && d.cand_pos[q] >= 0;
+    fill_shared_patch<NT>(d, valid, x, y, S);
+    const dpe_camera &cam = d.cams[0];
+    const float4 pl = valid ? src[q] : make_float4(0.0f, 0.0f, -1.0f, 1.0f);
+    const int p = x + y * d.W;
+    float acc[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    unsigned long long hyps = 0, taps = 0;
+    if (valid) {
+        const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
+        for (int v = 1 + wave; v < d.N; v += kWaves) {
+            const float wv = (float)vw[v - 1];
+            if (!(wv > 0.0f)) continue;
+            float g[3];
+            const float c = ncc_adjoint_view<NT>(d, S, lane, x, y, v, pl, g, taps);
+            hyps++;
+            acc[0] += wv * c;
+            acc[1] += wv;
+            acc[2] += wv * g[0];
+            acc[3] += wv * g[1];
+            acc[4] += wv * g[2];
+        }
+    }
+    for (int k = 0; k < 5; ++k) red[wave][k][lane] = acc[k];
+    __syncthreads();
+    if (wave == 0 && valid) {
+        float s[5];
+        for (int k = 0; k < 5; ++k) s[k] = red[0][k][lane] + red[1][k][lane] + red[2][k][lane] + red[3][k][lane];
+        float4 out = pl;
+        const float D = depth_from_plane(cam, pl, x, y);
+        if (s[1] > 0.0f && D > 0.0f) {
+            const float C = s[0] / s[1];
+            const float gnu[3] = {s[2] / s[1], s[3] / s[1], s[4] / s[1]};
+            const float th = acosf(fminf(1.0f, fmaxf(-1.0f, -pl.z)));
+            const float ph = atan2f(pl.y, pl.x);
+            const float st = sinf(th), ct = cosf(th), sp = sinf(ph), cp = cosf(ph);
+            const float4 n = make_float4(st * cp, st * sp, -ct, 0.0f);
+            const float dd = distance_to_origin(cam, x, y, D, n);
+            const float4 g = grad_theta_phi_depth(cam, x, y, st, ct, sp, cp, D, n, dd, gnu);
+            const float g2 = g.z * D;  // scaled depth
+            const float gg = g.x * g.x + g.y * g.y + g2 * g2;
+            if (gg > 0.0f && C > 0.0f) {
+                float dt = -C * g.x / gg, dp = -C * g.y / gg, dz = -C * g2 / gg;
+                const float lim = fmaxf(fmaxf(fabsf(dt) / 0.05f, fabsf(dp) / 0.05f), fabsf(dz) / 0.03f);
+                if (lim > 1.0f) {
+                    dt /= lim;
+                    dp /= lim;
+                    dz /= lim;
+                }
+                const float th2 = th + dt, ph2 = ph + dp, D2 = D * (1.0f + dz);
+                const float4 n2 = make_float4(sinf(th2) * cosf(ph2), sinf(th2) * sinf(ph2), -cosf(th2), 0.0f);
+                out = make_float4(n2.x, n2.y, n2.z, distance_to_origin(cam, x, y, D2, n2));
+            }
+        }
+        dst[q] = out;
+    }
+    count_add(d.counters, kFamRefine, hyps, taps);
+}
+
+void launch_gn_step(const Dev &d, int colour, const float4 *src, float4 *dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_gn_step<6>, dim3((d.Q + kLanes - 1) / kLanes), dim3(kLanes * kWaves), 0, s, d, colour, src,
+                       dst);
 }
 
 void launch_ncc_adjoint(const Dev &d, const float4 *tpd, const float *view_w, float *cost, float4 *grad,

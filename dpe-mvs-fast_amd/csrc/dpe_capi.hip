@@ -276,9 +276,18 @@ struct EventTimer {
 
 constexpr int kNumCounters = 2 * kNumFamilies;
 
-dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in, dpe_outputs *out) {
+dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *form_in, const dpe_inputs *in,
+                        dpe_outputs *out) {
     dpe_status st = validate(ctx, prm, in);
     if (st != DPE_OK) return st;
+    const dpe_form form = form_in ? *form_in : dpe_form{};
+    if (form.gauss_newton_steps < 0 || form.gauss_newton_steps > 2)
+        return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "gauss_newton_steps must be 0, 1 or 2");
+    if (form.precision != DPE_PRECISION_FP32) return fail(ctx, DPE_ERR_UNSUPPORTED, "unknown precision");
+    for (int r : form.reserved)
+        if (r != 0) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "reserved form fields must be 0");
+    if (form.gauss_newton_steps > 0 && 2 * prm->strong_radius / prm->strong_increment + 1 != 6)
+        return fail(ctx, DPE_ERR_UNSUPPORTED, "Gauss-Newton steps need the 11x11 step-2 window");
     if (!out || !out->planes || !out->costs || !out->pixel_states || !out->selected_views)
         return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "null output");
     HIPCHK(hipSetDevice(ctx->device));
@@ -427,6 +436,10 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *i
             launch_strong_candidates(d, c, it, s);
             tm.ncc_launch(kFamFull, [&] { launch_ncc_strong(d, c, s); });
             launch_strong_select(d, c, it, s);
+            if (form.gauss_newton_steps >= 1)
+                tm.ncc_launch(kFamRefine, [&] { launch_gn_step(d, c, d.now_plane, d.rplane + 3 * Q, s); });
+            if (form.gauss_newton_steps >= 2)
+                tm.ncc_launch(kFamRefine, [&] { launch_gn_step(d, c, d.rplane + 3 * Q, d.rplane + 4 * Q, s); });
             tm.ncc_launch(kFamRefine, [&] { launch_ncc_refine(d, c, s); });
             launch_strong_accept(d, c, s);
         }
@@ -691,21 +704,27 @@ dpe_status dpe_create(int device, dpe_ctx **out) {
 
 void dpe_destroy(dpe_ctx *ctx) {
     if (!ctx) return;
-    hipSetDevice(ctx->device);
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    // teardown: errors cannot be reported any more, so they are ignored
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &b : ctx->bufs)
-        if (b.ptr) hipFree(b.ptr);
-    for (auto e : ctx->events) hipEventDestroy(e);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+        if (b.ptr) (void)hipFree(b.ptr);
+    for (auto e : ctx->events) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
 const char *dpe_last_error(const dpe_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
 
 dpe_status dpe_process(dpe_ctx *ctx, const dpe_params *params, const dpe_inputs *in, dpe_outputs *out) {
+    return dpe_process_ex(ctx, params, nullptr, in, out);
+}
+
+dpe_status dpe_process_ex(dpe_ctx *ctx, const dpe_params *params, const dpe_form *form, const dpe_inputs *in,
+                          dpe_outputs *out) {
     if (!ctx) return DPE_ERR_INVALID_ARGUMENT;
     try {
-        return process_impl(ctx, params, in, out);
+        return process_impl(ctx, params, form, in, out);
     } catch (const Fail &f) {
         return f.st;
     }

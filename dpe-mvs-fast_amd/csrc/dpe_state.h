@@ -115,5 +115,7 @@ void launch_ncc_given(const Dev &d, int num_hyps, const float4 *planes, float *c
 // dpe_adjoint.hip: aggregated cost and its (theta, phi, depth) gradient per pixel
 void launch_ncc_adjoint(const Dev &d, const float4 *tpd, const float *view_w, float *cost, float4 *grad,
                         hipStream_t s);
+// one Gauss-Newton step of the refinement from src[Q] to dst[Q] (camera-frame planes)
+void launch_gn_step(const Dev &d, int colour, const float4 *src, float4 *dst, hipStream_t s);
 
 }  // namespace dpe

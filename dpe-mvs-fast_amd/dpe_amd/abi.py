@@ -52,7 +52,7 @@ class Outputs(C.Structure):
 
 NCC_FAMILIES = 4
 FAMILY_FULL, FAMILY_REFINE, FAMILY_POST, FAMILY_WEAK = range(NCC_FAMILIES)
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 class Stats(C.Structure):
@@ -60,6 +60,13 @@ class Stats(C.Structure):
                 ("ncc_launches", C.c_int32), ("family_ms", C.c_float * NCC_FAMILIES),
                 ("family_hyps", C.c_int64 * NCC_FAMILIES), ("family_taps", C.c_int64 * NCC_FAMILIES),
                 ("family_launches", C.c_int32 * NCC_FAMILIES)]
+
+
+PRECISION_FP32 = 0
+
+
+class Form(C.Structure):
+    _fields_ = [("gauss_newton_steps", C.c_int32), ("precision", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class FuseView(C.Structure):
@@ -137,6 +144,8 @@ def declare(lib):
     lib.dpe_abi_version.restype = C.c_int
     lib.dpe_process.argtypes = [C.c_void_p, P(Params), P(Inputs), P(Outputs)]
     lib.dpe_process.restype = C.c_int
+    lib.dpe_process_ex.argtypes = [C.c_void_p, P(Params), C.c_void_p, P(Inputs), P(Outputs)]
+    lib.dpe_process_ex.restype = C.c_int
     lib.dpe_last_stats.argtypes = [C.c_void_p, P(Stats)]
     lib.dpe_last_stats.restype = C.c_int
     lib.dpe_ncc_cost.argtypes = [C.c_void_p, P(Params), P(Inputs), C.c_int, C.c_void_p, C.c_void_p]
@@ -151,5 +160,5 @@ def declare(lib):
 
 
 # Every symbol include/dpe_hip.h declares.
-EXPORTS = ["dpe_create", "dpe_destroy", "dpe_last_error", "dpe_abi_version", "dpe_process", "dpe_last_stats",
-           "dpe_ncc_cost", "dpe_ncc_cost_adjoint", "dpe_fuse"]
+EXPORTS = ["dpe_create", "dpe_destroy", "dpe_last_error", "dpe_abi_version", "dpe_process", "dpe_process_ex",
+           "dpe_last_stats", "dpe_ncc_cost", "dpe_ncc_cost_adjoint", "dpe_fuse"]

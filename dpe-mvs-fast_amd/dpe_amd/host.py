@@ -121,10 +121,14 @@ class ProblemOutputs:
                                   _ptr(self.selected_views))
 
 
-def run_hip(ctx, pin):
-    """One dpe_process call on the GPU; returns ProblemOutputs."""
+def run_hip(ctx, pin, form=None):
+    """One dpe_process (dpe_process_ex with a form) call on the GPU; returns ProblemOutputs."""
     out = ProblemOutputs(pin.width, pin.height)
-    ctx.check(ctx.lib.dpe_process(ctx.h, C.byref(pin.params), C.byref(pin.struct), C.byref(out.struct)))
+    if form is None:
+        ctx.check(ctx.lib.dpe_process(ctx.h, C.byref(pin.params), C.byref(pin.struct), C.byref(out.struct)))
+    else:
+        ctx.check(ctx.lib.dpe_process_ex(ctx.h, C.byref(pin.params), C.cast(C.byref(form), C.c_void_p),
+                                         C.byref(pin.struct), C.byref(out.struct)))
     return out
 
 

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DPE_ABI_VERSION 3
+#define DPE_ABI_VERSION 4
 /* main.h:41-44 */
 #define DPE_MAX_IMAGES 32
 #define DPE_NEIGHBOUR_NUM 9
@@ -159,6 +159,24 @@ int dpe_abi_version(void);
 dpe_status dpe_process(dpe_ctx *ctx, const dpe_params *params,
                        const dpe_inputs *in, dpe_outputs *out);
 dpe_status dpe_last_stats(const dpe_ctx *ctx, dpe_stats *stats);
+
+/* Forms of the process call beyond the plain path (no counterpart in the
+ * reference; results are not those of the reference's algorithm and are
+ * judged against ground truth, not the oracle). */
+#define DPE_PRECISION_FP32 0
+typedef struct dpe_form {
+    int32_t gauss_newton_steps; /* 0: plain.  1 or 2: refinement hypotheses 4 and 5 of
+                                   PlaneHypothesisRefinementStrong (DPE.cu:1080-1104, the perturbed
+                                   normal and the perturbed depth) become Gauss-Newton steps of the
+                                   aggregated cost (dpe_ncc_cost_adjoint's gradient) from the current
+                                   plane, then from the first step */
+    int32_t precision;          /* DPE_PRECISION_FP32 */
+    int32_t reserved[6];        /* 0 */
+} dpe_form;
+
+/* dpe_process with a form; form == NULL or all zero is dpe_process. */
+dpe_status dpe_process_ex(dpe_ctx *ctx, const dpe_params *params, const dpe_form *form,
+                          const dpe_inputs *in, dpe_outputs *out);
 
 /* Multi-view bilateral NCC cost vectors, the device function
  * ComputeMultiViewCostVectorOld (DPE.cu:874-883) exposed for tests and

@@ -32,7 +32,7 @@ def test_struct_layouts():
 
 STRUCTS = {"dpe_camera": abi.Camera, "dpe_params": abi.Params, "dpe_inputs": abi.Inputs, "dpe_outputs": abi.Outputs,
            "dpe_stats": abi.Stats, "dpe_fuse_view": abi.FuseView, "dpe_fuse_params": abi.FuseParams,
-           "dpe_point": abi.Point}
+           "dpe_point": abi.Point, "dpe_form": abi.Form}
 
 
 def test_ctypes_mirror_matches_the_c_header(tmp_path):

@@ -1,0 +1,52 @@
+"""Forms beyond the plain path, judged against the synthetic ground truth (their
+results are not the reference algorithm's, so the oracle does not apply).
+
+Gauss-Newton form (dpe_form.gauss_newton_steps = 2): on config-1 inputs the
+refinement's perturbed-normal and perturbed-depth hypotheses are replaced by two
+Gauss-Newton steps of the aggregated cost.  Reported: ground-truth accuracy
+(|dd|/d < 1 % and < 0.2 %) per iteration count for plain and Gauss-Newton, and
+the iteration count each needs to reach plain's 4-iteration accuracy.  Bar: the
+form is never less accurate than plain at the same iteration count (0.5 points
+of slack for the different random draws)."""
+import numpy as np
+import pytest
+
+from dpe_amd import abi, host, scene
+
+pytestmark = pytest.mark.gpu
+
+ITERS = (1, 2, 3, 4, 6)
+
+
+def _accuracy(ctx, s, ref, iters, form):
+    gt = s.views[ref].depth
+    pin = host.first_call_inputs(s, ref, s.pairs[ref], max_iterations=iters)
+    out = host.run_hip(ctx, pin, form)
+    r = host.assemble(out, pin.params.depth_min, pin.params.depth_max)
+    rel = np.abs(r["depth"] - gt) / gt
+    return float((rel < 0.01).mean()), float((rel < 0.002).mean()), ctx.stats().gpu_ms
+
+
+def test_gauss_newton_form_against_ground_truth():
+    s = scene.config1()
+    ref = s.meta["ref_mid"]
+    ctx = host.Context(0)
+    try:
+        res = {}
+        for gn in (0, 2):
+            form = abi.Form(gauss_newton_steps=gn)
+            res[gn] = [_accuracy(ctx, s, ref, it, form) for it in ITERS]
+            print(f"gauss_newton_steps={gn}: " + ", ".join(
+                f"{it} it: {a1:.4f} @1% / {a02:.4f} @0.2% ({ms:.1f} ms)" for it, (a1, a02, ms) in zip(ITERS, res[gn])))
+        target = res[0][ITERS.index(4)][0]
+        need = {gn: next((it for it, r in zip(ITERS, res[gn]) if r[0] >= target), None) for gn in (0, 2)}
+        print(f"iterations to reach plain's 4-iteration accuracy {target:.4f} @1%: plain {need[0]}, "
+              f"Gauss-Newton {need[2]}")
+        for (p1, p02, _), (g1, g02, _) in zip(res[0], res[2]):
+            assert g1 >= p1 - 0.005 and g02 >= p02 - 0.005
+        # an invalid form is rejected, not run
+        pin = host.first_call_inputs(s, ref, s.pairs[ref], max_iterations=1)
+        with pytest.raises(host.DPEError):
+            host.run_hip(ctx, pin, abi.Form(gauss_newton_steps=3))
+    finally:
+        ctx.close()
