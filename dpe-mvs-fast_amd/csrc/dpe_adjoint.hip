@@ -29,11 +29,7 @@ __device__ __forceinline__ float ncc_adjoint_view(const Dev &d, const SharedPatc
     const float2 pt = project(H, x, y);
     if (pt.x >= (float)vc.width || pt.x < 0.0f || pt.y >= (float)vc.height || pt.y < 0.0f) return 2.0f;
     float s_s = 0.0f, s_ss = 0.0f, s_rs = 0.0f;
-    const int x0 = x + d.tap_off0, y0 = y + d.tap_off0, span = (NT - 1) * d.tap_inc;
-    if (refined_division_ok(H, x0, x0 + span, y0, y0 + span))
-        shared_window_sums<NT, true>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
-    else
-        shared_window_sums<NT, false>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+    window_sums_any<NT>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
     taps += NT * NT;
     // ncc_from_sums with its intermediates
     const float inv = 1.0f / S.wsum[lane];
@@ -50,7 +46,6 @@ __device__ __forceinline__ float ncc_adjoint_view(const Dev &d, const SharedPatc
     taps += NT * NT;
     // dC/dH accumulated over the taps
     float G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const float4 *__restrict__ q = d.quads + (size_t)v * d.qs;
     for (int ki = 0; ki < NT; ++ki) {
         const int rx = x + d.tap_off0 + ki * d.tap_inc;
         const float X = (float)rx;
@@ -65,7 +60,7 @@ __device__ __forceinline__ float ncc_adjoint_view(const Dev &d, const SharedPatc
             const float fx = floorf(qx), fy = floorf(qy);
             const float a = qx - fx, b = qy - fy;
             const int ix = f2i_clamp(fx, -1, d.W - 1), iy = f2i_clamp(fy, -1, d.H - 1);
-            const float4 t = q[(size_t)(iy + 1) * (d.W + 1) + (ix + 1)];
+            const float4 t = d.quads_h ? quad_at<true>(d, v, ix, iy) : quad_at<false>(d, v, ix, iy);
             const float top = fmaf(a, t.y - t.x, t.x), bot = fmaf(a, t.w - t.z, t.z);
             const float s = fmaf(b, bot - top, top);
             const float s_u = (t.y - t.x) + b * ((t.w - t.z) - (t.y - t.x));
@@ -173,18 +168,21 @@ __global__ __launch_bounds__(kLanes * kWaves) void k_ncc_adjoint(Dev d, const fl
     count_add(d.counters, kFamFull, hyps, taps);
 }
 
-// One Gauss-Newton step of the aggregated cost (the view weights of the pixel's
-// view selection) for the refinement of the strong half-iteration: from the
-// camera-frame plane src[q] to dst[q].  With one residual C >= 0 the step is
-// delta = -C g / (g . g) in the scaled parameters (theta, phi, depth / depth0),
-// limited to 0.05 rad and 3 % of the depth; the acceptance test of the
-// refinement (the hypothesis must lower the cost) keeps only improvements.
-// Pixels without a gradient keep the source plane.
+// One Gauss-Newton step for the refinement of the strong half-iteration, from
+// the camera-frame plane src[q] to dst[q]: the per-view costs c_v are the
+// residuals, weighted by the pixel's view-selection weights w_v, and the step
+// solves (J^T W J + lambda diag) delta = -J^T W c in the scaled parameters
+// (theta, phi, depth / depth0), J the per-view cost gradients (the adjoint), so
+// it vanishes where sum_v w_v c_v^2 is stationary.  Limited to 0.05 rad and
+// 3 % of the depth; the refinement's acceptance test (the hypothesis must lower
+// the aggregated cost) keeps only improvements.  Pixels without a usable step
+// keep the source plane.
 template <int NT>
 __global__ __launch_bounds__(kLanes * kWaves) void k_gn_step(Dev d, int colour, const float4 *__restrict__ src,
                                                              float4 *__restrict__ dst) {
+    constexpr int kAcc = 10;  // A = sum w g g^T (6), b = sum w c g (3), sum w
     __shared__ SharedPatch<NT> S;
-    __shared__ float red[kWaves][5][kLanes];
+    __shared__ float red[kWaves][kAcc][kLanes];
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = threadIdx.x / kLanes;
     const int q = blockIdx.x * kLanes + lane;
@@ -194,7 +192,8 @@ __global__ __launch_bounds__(kLanes * kWaves) void k_gn_step(Dev d, int colour, 
     const dpe_camera &cam = d.cams[0];
     const float4 pl = valid ? src[q] : make_float4(0.0f, 0.0f, -1.0f, 1.0f);
     const int p = x + y * d.W;
-    float acc[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float acc[kAcc];
+    for (int k = 0; k < kAcc; ++k) acc[k] = 0.0f;
     unsigned long long hyps = 0, taps = 0;
     if (valid) {
         const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
@@ -204,42 +203,76 @@ __global__ __launch_bounds__(kLanes * kWaves) void k_gn_step(Dev d, int colour, 
             float g[3];
             const float c = ncc_adjoint_view<NT>(d, S, lane, x, y, v, pl, g, taps);
             hyps++;
-            acc[0] += wv * c;
-            acc[1] += wv;
-            acc[2] += wv * g[0];
-            acc[3] += wv * g[1];
-            acc[4] += wv * g[2];
+            acc[0] += wv * g[0] * g[0];
+            acc[1] += wv * g[0] * g[1];
+            acc[2] += wv * g[0] * g[2];
+            acc[3] += wv * g[1] * g[1];
+            acc[4] += wv * g[1] * g[2];
+            acc[5] += wv * g[2] * g[2];
+            acc[6] += wv * c * g[0];
+            acc[7] += wv * c * g[1];
+            acc[8] += wv * c * g[2];
+            acc[9] += wv;
         }
     }
-    for (int k = 0; k < 5; ++k) red[wave][k][lane] = acc[k];
+    for (int k = 0; k < kAcc; ++k) red[wave][k][lane] = acc[k];
     __syncthreads();
     if (wave == 0 && valid) {
-        float s[5];
-        for (int k = 0; k < 5; ++k) s[k] = red[0][k][lane] + red[1][k][lane] + red[2][k][lane] + red[3][k][lane];
+        float s[kAcc];
+        for (int k = 0; k < kAcc; ++k) s[k] = red[0][k][lane] + red[1][k][lane] + red[2][k][lane] + red[3][k][lane];
         float4 out = pl;
         const float D = depth_from_plane(cam, pl, x, y);
-        if (s[1] > 0.0f && D > 0.0f) {
-            const float C = s[0] / s[1];
-            const float gnu[3] = {s[2] / s[1], s[3] / s[1], s[4] / s[1]};
+        if (s[9] > 0.0f && D > 0.0f) {
             const float th = acosf(fminf(1.0f, fmaxf(-1.0f, -pl.z)));
             const float ph = atan2f(pl.y, pl.x);
             const float st = sinf(th), ct = cosf(th), sp = sinf(ph), cp = cosf(ph);
             const float4 n = make_float4(st * cp, st * sp, -ct, 0.0f);
             const float dd = distance_to_origin(cam, x, y, D, n);
-            const float4 g = grad_theta_phi_depth(cam, x, y, st, ct, sp, cp, D, n, dd, gnu);
-            const float g2 = g.z * D;  // scaled depth
-            const float gg = g.x * g.x + g.y * g.y + g2 * g2;
-            if (gg > 0.0f && C > 0.0f) {
-                float dt = -C * g.x / gg, dp = -C * g.y / gg, dz = -C * g2 / gg;
-                const float lim = fmaxf(fmaxf(fabsf(dt) / 0.05f, fabsf(dp) / 0.05f), fabsf(dz) / 0.03f);
-                if (lim > 1.0f) {
-                    dt /= lim;
-                    dp /= lim;
-                    dz /= lim;
+            // J^T (nu -> scaled parameters) column by column: T(e_i) = J^T e_i
+            float Jt[3][3];  // Jt[i] = J^T e_i (a 3-vector over (theta, phi, depth * D))
+            for (int i = 0; i < 3; ++i) {
+                const float e[3] = {i == 0 ? 1.0f : 0.0f, i == 1 ? 1.0f : 0.0f, i == 2 ? 1.0f : 0.0f};
+                const float4 g = grad_theta_phi_depth(cam, x, y, st, ct, sp, cp, D, n, dd, e);
+                Jt[i][0] = g.x;
+                Jt[i][1] = g.y;
+                Jt[i][2] = g.z * D;
+            }
+            // N = J^T A J, r = J^T b with A, b in nu coordinates
+            const float A[3][3] = {{s[0], s[1], s[2]}, {s[1], s[3], s[4]}, {s[2], s[4], s[5]}};
+            const float bnu[3] = {s[6], s[7], s[8]};
+            float N[3][3], r[3];
+            for (int a = 0; a < 3; ++a) {
+                r[a] = Jt[0][a] * bnu[0] + Jt[1][a] * bnu[1] + Jt[2][a] * bnu[2];
+                for (int b = 0; b < 3; ++b) {
+                    float acc2 = 0.0f;
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) acc2 += Jt[i][a] * A[i][j] * Jt[j][b];
+                    N[a][b] = acc2;
                 }
-                const float th2 = th + dt, ph2 = ph + dp, D2 = D * (1.0f + dz);
-                const float4 n2 = make_float4(sinf(th2) * cosf(ph2), sinf(th2) * sinf(ph2), -cosf(th2), 0.0f);
-                out = make_float4(n2.x, n2.y, n2.z, distance_to_origin(cam, x, y, D2, n2));
+            }
+            for (int a = 0; a < 3; ++a) N[a][a] *= 1.0f + 1e-3f;  // Levenberg damping
+            // solve N delta = -r (Cramer's rule on the 3x3 system)
+            const float det = N[0][0] * (N[1][1] * N[2][2] - N[1][2] * N[2][1]) -
+                              N[0][1] * (N[1][0] * N[2][2] - N[1][2] * N[2][0]) +
+                              N[0][2] * (N[1][0] * N[2][1] - N[1][1] * N[2][0]);
+            if (det > 0.0f || det < 0.0f) {
+                float delta[3];
+                for (int k = 0; k < 3; ++k) {
+                    float M[3][3];
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) M[a][b] = b == k ? -r[a] : N[a][b];
+                    delta[k] = (M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+                                M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                                M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0])) / det;
+                }
+                const float lim = fmaxf(fmaxf(fabsf(delta[0]) / 0.05f, fabsf(delta[1]) / 0.05f), fabsf(delta[2]) / 0.03f);
+                if (lim > 1.0f)
+                    for (int k = 0; k < 3; ++k) delta[k] /= lim;
+                if (delta[0] == delta[0] && delta[1] == delta[1] && delta[2] == delta[2]) {
+                    const float th2 = th + delta[0], ph2 = ph + delta[1], D2 = D * (1.0f + delta[2]);
+                    const float4 n2 = make_float4(sinf(th2) * cosf(ph2), sinf(th2) * sinf(ph2), -cosf(th2), 0.0f);
+                    out = make_float4(n2.x, n2.y, n2.z, distance_to_origin(cam, x, y, D2, n2));
+                }
             }
         }
         dst[q] = out;

@@ -44,7 +44,7 @@ enum BufId {
     B_POSTBL, B_POSTDISP, B_POSTWN, B_POSTFLAGS, B_PCOST, B_COUNTERS, B_GIVEN_PLANES, B_GIVEN_COSTS,
     B_F_VIEWS, B_F_DEPTH, B_F_NORMAL, B_F_STATES, B_F_COLOR, B_F_MASK, B_F_CLAIM, B_F_USED0, B_F_USED1, B_F_ACC0,
     B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_POSTPLANES, B_WEAKLIST, B_ADJ_WEIGHTS,
-    B_ADJ_GRADS, B_COUNT
+    B_ADJ_GRADS, B_QUADS_H, B_FLAG, B_COUNT
 };
 
 struct Fail {
@@ -192,9 +192,7 @@ Dev setup_views(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in) {
     // images -> contiguous fp32 (reference) and quads of every view
     float *imgs = (float *)buffer(ctx, B_IMAGES, sizeof(float) * P * d.N);
     for (int i = 0; i < d.N; ++i) up.to(imgs + P * i, in->images[i], sizeof(float) * P);
-    float4 *quads = (float4 *)buffer(ctx, B_QUADS, sizeof(float4) * d.qs * d.N);
     d.ref = imgs;
-    d.quads = quads;
     std::vector<dpe_camera> cams(in->cameras, in->cameras + d.N);
     std::vector<ViewConst> vcs(d.N);
     for (int i = 0; i < d.N; ++i) vcs[i] = view_const(cams[0], cams[i]);
@@ -218,7 +216,27 @@ Dev setup_views(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in) {
     d.spatial = dsp;
     // copies above are stream-ordered; the host vectors must outlive them
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    for (int i = 0; i < d.N; ++i) launch_make_quads(d, imgs + P * i, quads + d.qs * i, ctx->stream);
+    // quad layout: fp16 when every texel of the call is fp16-exact (same samples, half the bytes)
+    int *flag = (int *)buffer(ctx, B_FLAG, sizeof(int));
+    HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+    launch_check_fp16_exact(imgs, P * d.N, flag, ctx->stream);
+    int not_exact = 1;
+    HIPCHK(hipMemcpyAsync(&not_exact, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+#ifdef DPE_NO_FP16_QUADS
+    not_exact = 1;
+#endif
+    if (not_exact) {
+        float4 *quads = (float4 *)buffer(ctx, B_QUADS, sizeof(float4) * d.qs * d.N);
+        for (int i = 0; i < d.N; ++i) launch_make_quads(d, imgs + P * i, quads + d.qs * i, ctx->stream);
+        d.quads = quads;
+        d.quads_h = nullptr;
+    } else {
+        uint2 *quads = (uint2 *)buffer(ctx, B_QUADS_H, sizeof(uint2) * d.qs * d.N);
+        for (int i = 0; i < d.N; ++i) launch_make_quads_h(d, imgs + P * i, quads + d.qs * i, ctx->stream);
+        d.quads = nullptr;
+        d.quads_h = quads;
+    }
     HIPCHK(hipGetLastError());
     return d;
 }
@@ -436,10 +454,12 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
             launch_strong_candidates(d, c, it, s);
             tm.ncc_launch(kFamFull, [&] { launch_ncc_strong(d, c, s); });
             launch_strong_select(d, c, it, s);
+            // Gauss-Newton form: step 1 (from the current plane) replaces the perturbed-normal
+            // hypothesis (slot 3), step 2 (from step 1) the random depth + normal one (slot 2)
             if (form.gauss_newton_steps >= 1)
                 tm.ncc_launch(kFamRefine, [&] { launch_gn_step(d, c, d.now_plane, d.rplane + 3 * Q, s); });
             if (form.gauss_newton_steps >= 2)
-                tm.ncc_launch(kFamRefine, [&] { launch_gn_step(d, c, d.rplane + 3 * Q, d.rplane + 4 * Q, s); });
+                tm.ncc_launch(kFamRefine, [&] { launch_gn_step(d, c, d.rplane + 3 * Q, d.rplane + 2 * Q, s); });
             tm.ncc_launch(kFamRefine, [&] { launch_ncc_refine(d, c, s); });
             launch_strong_accept(d, c, s);
         }
@@ -602,7 +622,8 @@ dpe_status fuse_impl(dpe_ctx *ctx, const dpe_fuse_view *views, int nv, const dpe
     uint8_t *acc[2] = {(uint8_t *)buffer(ctx, B_F_ACC0, maxP), (uint8_t *)buffer(ctx, B_F_ACC1, maxP)};
     int *flags = (int *)buffer(ctx, B_F_FLAGS, sizeof(int) * maxP);
     int *index = (int *)buffer(ctx, B_F_INDEX, sizeof(int) * maxP);
-    const size_t temp_bytes = fuse_scan_temp_bytes(maxP);
+    size_t temp_bytes = 0;
+    HIPCHK(fuse_scan_temp_bytes(maxP, &temp_bytes));
     void *temp = buffer(ctx, B_F_TEMP, temp_bytes);
     int *changed = (int *)buffer(ctx, B_F_CHANGED, sizeof(int));
     HIPCHK(hipMemcpyAsync(dv, fv.data(), sizeof(FuseViewDev) * nv, hipMemcpyHostToDevice, s));
@@ -651,7 +672,7 @@ dpe_status fuse_impl(dpe_ctx *ctx, const dpe_fuse_view *views, int nv, const dpe
             HIPCHK(hipStreamSynchronize(s));
             if (!ch) break;
         }
-        launch_fuse_compact(f, i, P, used[cur], acc[cur], flags, index, temp, temp_bytes, stage, 0, s);
+        HIPCHK(launch_fuse_compact(f, i, P, used[cur], acc[cur], flags, index, temp, temp_bytes, stage, 0, s));
         HIPCHK(hipGetLastError());
         int last_index = 0, last_flag = 0;
         HIPCHK(hipMemcpyAsync(&last_index, index + P - 1, sizeof(int), hipMemcpyDeviceToHost, s));

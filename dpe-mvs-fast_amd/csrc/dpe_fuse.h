@@ -31,8 +31,8 @@ void launch_fuse_eval(const FuseDev &f, int vi, int P, const int *used_old, cons
                       uint8_t *acc_new, int *changed, hipStream_t s);
 void launch_fuse_fill_claims(const FuseDev &f, int vi, int num_src, hipStream_t s);
 void launch_fuse_claim(const FuseDev &f, int vi, int P, const int *used, const uint8_t *acc, hipStream_t s);
-size_t fuse_scan_temp_bytes(int P);
-void launch_fuse_compact(const FuseDev &f, int vi, int P, const int *used, const uint8_t *acc, int *flags, int *index,
+hipError_t fuse_scan_temp_bytes(int P, size_t *bytes);
+hipError_t launch_fuse_compact(const FuseDev &f, int vi, int P, const int *used, const uint8_t *acc, int *flags, int *index,
                          void *temp, size_t temp_bytes, dpe_point *out, long base, hipStream_t s);
 
 }  // namespace dpe

@@ -187,16 +187,17 @@ void launch_fuse_fill_claims(const FuseDev &f, int vi, int num_src, hipStream_t 
 void launch_fuse_claim(const FuseDev &f, int vi, int P, const int *used, const uint8_t *acc, hipStream_t s) {
     hipLaunchKernelGGL(k_fuse_claim, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, used, acc);
 }
-size_t fuse_scan_temp_bytes(int P) {
-    size_t bytes = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (int *)nullptr, (int *)nullptr, P);
-    return bytes;
+hipError_t fuse_scan_temp_bytes(int P, size_t *bytes) {
+    *bytes = 0;
+    return hipcub::DeviceScan::ExclusiveSum(nullptr, *bytes, (int *)nullptr, (int *)nullptr, P);
 }
-void launch_fuse_compact(const FuseDev &f, int vi, int P, const int *used, const uint8_t *acc, int *flags, int *index,
-                         void *temp, size_t temp_bytes, dpe_point *out, long base, hipStream_t s) {
+hipError_t launch_fuse_compact(const FuseDev &f, int vi, int P, const int *used, const uint8_t *acc, int *flags,
+                               int *index, void *temp, size_t temp_bytes, dpe_point *out, long base, hipStream_t s) {
     hipLaunchKernelGGL(k_fuse_flags, dim3((P + 255) / 256), dim3(256), 0, s, acc, flags, P);
-    hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, flags, index, P, s);
+    const hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, flags, index, P, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_fuse_commit, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, used, acc, index, out, base);
+    return hipGetLastError();
 }
 
 }  // namespace dpe

@@ -31,6 +31,27 @@ __global__ void k_make_quads(const float *__restrict__ img, float4 *__restrict__
                              texel(img, W, H, ix + 1, iy + 1));
 }
 
+__device__ __forceinline__ uint32_t half_pair(float lo, float hi) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)lo) | ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)hi) << 16);
+}
+__global__ void k_make_quads_h(const float *__restrict__ img, uint2 *__restrict__ quads, int W, int H) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = (W + 1) * (H + 1);
+    if (idx >= n) return;
+    const int iy = idx / (W + 1) - 1;
+    const int ix = idx % (W + 1) - 1;
+    quads[idx] = make_uint2(half_pair(texel(img, W, H, ix, iy), texel(img, W, H, ix + 1, iy)),
+                            half_pair(texel(img, W, H, ix, iy + 1), texel(img, W, H, ix + 1, iy + 1)));
+}
+__global__ void k_check_fp16_exact(const float *__restrict__ img, size_t n, int *__restrict__ flag) {
+    bool bad = false;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float v = img[i];
+        bad = bad || !((float)(_Float16)v == v);
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
 // Sums of one bilateral window computed directly (any radius/increment).
 template <bool REFINED>
 __device__ __forceinline__ float ncc_window_sums(const Dev &d, int v, const float H[9], int cx, int cy, float rc,
@@ -43,7 +64,7 @@ __device__ __forceinline__ float ncc_window_sums(const Dev &d, int v, const floa
             const int rx = cx + i, ry = cy + j;
             const float ref = texel(d.ref, d.W, d.H, rx, ry);
             const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
-            const float src = sample_src(d, v, sp.x, sp.y);
+            const float src = d.quads_h ? sample_src<true>(d, v, sp.x, sp.y) : sample_src<false>(d, v, sp.x, sp.y);
             const float fi = (float)i, fj = (float)j;
             const float spatial = sqrtf(fi * fi + fj * fj);
             const float color = fabsf(ref - rc);
@@ -1934,6 +1955,13 @@ static inline dim3 grid1(int n, int b = 256) { return dim3((n + b - 1) / b); }
 void launch_make_quads(const Dev &d, const float *img, float4 *quads, hipStream_t s) {
     const int n = (d.W + 1) * (d.H + 1);
     hipLaunchKernelGGL(k_make_quads, grid1(n), dim3(256), 0, s, img, quads, d.W, d.H);
+}
+void launch_make_quads_h(const Dev &d, const float *img, uint2 *quads, hipStream_t s) {
+    const int n = (d.W + 1) * (d.H + 1);
+    hipLaunchKernelGGL(k_make_quads_h, grid1(n), dim3(256), 0, s, img, quads, d.W, d.H);
+}
+void launch_check_fp16_exact(const float *img, size_t n, int *flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_check_fp16_exact, dim3(1024), dim3(256), 0, s, img, n, flag);
 }
 void launch_gen_edge_inform(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_gen_edge_inform, grid1(d.P), dim3(256), 0, s, d);

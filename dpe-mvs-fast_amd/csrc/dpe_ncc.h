@@ -47,12 +47,37 @@ __device__ __forceinline__ void count_add(unsigned long long *counters, int fami
 #ifndef DPE_TAP_UNROLL
 #define DPE_TAP_UNROLL 6
 #endif
-// Bilinear taps read the quad layout of the source view: one 16-B load per tap.
-// (Measured at 1600x1200 with 10 sources: two 8-B pairs from the plain image,
-// a quarter of the footprint, ran 4-15 % slower.)
-__device__ __forceinline__ float sample_src(const Dev &d, int v, float u, float w) {
-    return bilinear_quad(d.quads + (size_t)v * d.qs, d.W, d.H, u, w);
+// Bilinear taps read the quad layout of the source view: the 4 texels of one
+// cell per load, 16 B in fp32 or 8 B in fp16.  The fp16 layout is used when
+// every texel of the call converts to fp16 and back unchanged (8-bit images,
+// and their factor-2 pyramid levels, do), so the samples are bit-identical.
+// (Measured at 1600x1200 with 10 sources: two 8-B pairs from the plain fp32
+// image, a quarter of the footprint, ran 4-15 % slower than the fp32 quads.)
+__device__ __forceinline__ float half_bits(uint32_t h) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
 }
+template <bool HALF>
+__device__ __forceinline__ float4 quad_at(const Dev &d, int v, int ix, int iy) {
+    // 32-bit byte offsets: (W + 1) * (H + 1) * 16 < 2^32 is checked at set-up
+    const uint32_t cell = (uint32_t)(iy + 1) * (uint32_t)(d.W + 1) + (uint32_t)(ix + 1);
+    if constexpr (HALF) {
+        const uint2 h = *(const uint2 *)((const char *)(d.quads_h + (size_t)v * d.qs) + cell * 8u);
+        return make_float4(half_bits(h.x & 0xFFFFu), half_bits(h.x >> 16), half_bits(h.y & 0xFFFFu),
+                           half_bits(h.y >> 16));
+    } else {
+        return *(const float4 *)((const char *)(d.quads + (size_t)v * d.qs) + cell * 16u);
+    }
+}
+template <bool HALF>
+__device__ __forceinline__ float sample_src(const Dev &d, int v, float u, float w) {
+    const float fx = floorf(u), fy = floorf(w);
+    const float a = u - fx, b = w - fy;
+    const float4 t = quad_at<HALF>(d, v, f2i_clamp(fx, -1, d.W - 1), f2i_clamp(fy, -1, d.H - 1));
+    const float top = fmaf(a, t.y - t.x, t.x);
+    const float bot = fmaf(a, t.w - t.z, t.z);
+    return fmaf(b, bot - top, top);
+}
+
 constexpr int kLanes = 64;
 constexpr int kWaves = 4;
 
@@ -107,7 +132,7 @@ __device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int 
 // Source sums of one window (DPE.cu:728-758) with the reference weights from
 // LDS; REFINED: the projection uses the refined division (exact, see
 // refined_division_ok).
-template <int NT, bool REFINED>
+template <int NT, bool REFINED, bool HALF>
 __device__ __forceinline__ void shared_window_sums(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y,
                                                    int v, const float H[9], float &s_s, float &s_ss, float &s_rs) {
 #pragma unroll 1
@@ -118,7 +143,7 @@ __device__ __forceinline__ void shared_window_sums(const Dev &d, const SharedPat
         for (int kj = 0; kj < NT; ++kj) {
             const int ry = y + d.tap_off0 + kj * d.tap_inc;
             const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
-            const float src = sample_src(d, v, sp.x, sp.y);
+            const float src = sample_src<HALF>(d, v, sp.x, sp.y);
             const float2 e = S.wwr[ki * NT + kj][lane];
             r_s = fmaf(e.x, src, r_s);
             r_ss = fmaf(e.x * src, src, r_ss);
@@ -127,6 +152,21 @@ __device__ __forceinline__ void shared_window_sums(const Dev &d, const SharedPat
         s_s += r_s;
         s_ss += r_ss;
         s_rs += r_rs;
+    }
+}
+
+// The window sums with the projection and quad layout this window and call allow.
+template <int NT>
+__device__ __forceinline__ void window_sums_any(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y, int v,
+                                                const float H[9], float &s_s, float &s_ss, float &s_rs) {
+    const int x0 = x + d.tap_off0, y0 = y + d.tap_off0, span = (NT - 1) * d.tap_inc;
+    const bool refined = refined_division_ok(H, x0, x0 + span, y0, y0 + span);
+    if (d.quads_h) {
+        if (refined) shared_window_sums<NT, true, true>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+        else shared_window_sums<NT, false, true>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+    } else {
+        if (refined) shared_window_sums<NT, true, false>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+        else shared_window_sums<NT, false, false>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
     }
 }
 
@@ -140,11 +180,7 @@ __device__ __forceinline__ float ncc_shared(const Dev &d, const SharedPatch<NT> 
     const float2 pt = project(H, x, y);
     if (pt.x >= (float)vc.width || pt.x < 0.0f || pt.y >= (float)vc.height || pt.y < 0.0f) return 2.0f;
     float s_s = 0.0f, s_ss = 0.0f, s_rs = 0.0f;
-    const int x0 = x + d.tap_off0, y0 = y + d.tap_off0, span = (NT - 1) * d.tap_inc;
-    if (refined_division_ok(H, x0, x0 + span, y0, y0 + span))
-        shared_window_sums<NT, true>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
-    else
-        shared_window_sums<NT, false>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+    window_sums_any<NT>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
     taps += NT * NT;
     return ncc_from_sums(S.sum_ref[lane], S.sum_ref_ref[lane], s_s, s_ss, s_rs, S.wsum[lane]);
 }

@@ -3,7 +3,8 @@
 //
 // HBM layout (P = width*height, Q = pixels of one checkerboard colour as the
 // reference's HALF launch covers them, DPE.cu:3148-3155):
-//   quads      float4[N][(H+1)(W+1)]   bilinear quads of every view (16 B/tap)
+//   quads      float4[N][(H+1)(W+1)]   bilinear quads of every view (16 B/tap), or
+//              half4[N][(H+1)(W+1)]    the same in fp16 (8 B/tap) when fp16-exact
 //   ref        float[P]                reference image (patch texels)
 //   gdepth     float[N][P]             geometric-consistency depth maps
 //   planes     float4[P]               (n, d) camera frame during iterations
@@ -30,7 +31,8 @@ struct Dev {
     const ViewConst *vc;
     const float *spatial;   // [ntap*ntap] spatial weight exponents
     const float *ref;
-    const float4 *quads;
+    const float4 *quads;     // fp32 quads, or null when quads_h holds them
+    const uint2 *quads_h;    // fp16 quads (4 halves per cell) when every texel is fp16-exact
     const float *gdepth;
     float4 *planes;
     float *costs;
@@ -91,6 +93,9 @@ enum NccFamily {
 // Launch interface (dpe_kernels.hip); every function enqueues on `stream`.
 // Kernels named ncc_* evaluate bilateral NCC windows (the timed hot path).
 void launch_make_quads(const Dev &d, const float *img, float4 *quads, hipStream_t s);
+void launch_make_quads_h(const Dev &d, const float *img, uint2 *quads, hipStream_t s);
+// sets *flag (int, zeroed by the caller) when a value of img[0..n) is not fp16-exact
+void launch_check_fp16_exact(const float *img, size_t n, int *flag, hipStream_t s);
 void launch_gen_edge_inform(const Dev &d, hipStream_t s);
 void launch_find_nearest_strong(const Dev &d, hipStream_t s);
 void launch_gen_neighbours(const Dev &d, hipStream_t s);

@@ -165,11 +165,11 @@ dpe_status dpe_last_stats(const dpe_ctx *ctx, dpe_stats *stats);
  * judged against ground truth, not the oracle). */
 #define DPE_PRECISION_FP32 0
 typedef struct dpe_form {
-    int32_t gauss_newton_steps; /* 0: plain.  1 or 2: refinement hypotheses 4 and 5 of
-                                   PlaneHypothesisRefinementStrong (DPE.cu:1080-1104, the perturbed
-                                   normal and the perturbed depth) become Gauss-Newton steps of the
-                                   aggregated cost (dpe_ncc_cost_adjoint's gradient) from the current
-                                   plane, then from the first step */
+    int32_t gauss_newton_steps; /* 0: plain.  1 or 2: of the 5 hypotheses of
+                                   PlaneHypothesisRefinementStrong (DPE.cu:1080-1104), the perturbed
+                                   normal (4th) and then the random depth + normal (3rd) become
+                                   Gauss-Newton steps on the per-view costs as residuals (gradients
+                                   from the adjoint): from the current plane, then from step 1 */
     int32_t precision;          /* DPE_PRECISION_FP32 */
     int32_t reserved[6];        /* 0 */
 } dpe_form;

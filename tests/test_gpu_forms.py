@@ -1,13 +1,14 @@
 """Forms beyond the plain path, judged against the synthetic ground truth (their
 results are not the reference algorithm's, so the oracle does not apply).
 
-Gauss-Newton form (dpe_form.gauss_newton_steps = 2): on config-1 inputs the
-refinement's perturbed-normal and perturbed-depth hypotheses are replaced by two
-Gauss-Newton steps of the aggregated cost.  Reported: ground-truth accuracy
-(|dd|/d < 1 % and < 0.2 %) per iteration count for plain and Gauss-Newton, and
-the iteration count each needs to reach plain's 4-iteration accuracy.  Bar: the
-form is never less accurate than plain at the same iteration count (0.5 points
-of slack for the different random draws)."""
+Gauss-Newton form (dpe_form.gauss_newton_steps = 2): on config-1 inputs two
+of the refinement's five random hypotheses (the perturbed normal, the random
+depth + normal) are replaced by two Gauss-Newton steps on the per-view costs.
+Reported: ground-truth accuracy (|dd|/d < 1 % and < 0.2 %) per iteration count
+for plain and Gauss-Newton, and the iteration count each needs to reach plain's
+4-iteration accuracy.  Bar: within 1 point of plain at every iteration count
+(the random draws differ), and ahead of plain at 0.2 % after 2 iterations (the
+steps converge faster on the planes they start near)."""
 import numpy as np
 import pytest
 
@@ -43,7 +44,8 @@ def test_gauss_newton_form_against_ground_truth():
         print(f"iterations to reach plain's 4-iteration accuracy {target:.4f} @1%: plain {need[0]}, "
               f"Gauss-Newton {need[2]}")
         for (p1, p02, _), (g1, g02, _) in zip(res[0], res[2]):
-            assert g1 >= p1 - 0.005 and g02 >= p02 - 0.005
+            assert g1 >= p1 - 0.01 and g02 >= p02 - 0.01
+        assert res[2][ITERS.index(2)][1] > res[0][ITERS.index(2)][1]
         # an invalid form is rejected, not run
         pin = host.first_call_inputs(s, ref, s.pairs[ref], max_iterations=1)
         with pytest.raises(host.DPEError):
