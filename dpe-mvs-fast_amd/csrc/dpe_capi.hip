@@ -44,7 +44,7 @@ enum BufId {
     B_POSTBL, B_POSTDISP, B_POSTWN, B_POSTFLAGS, B_PCOST, B_COUNTERS, B_GIVEN_PLANES, B_GIVEN_COSTS,
     B_F_VIEWS, B_F_DEPTH, B_F_NORMAL, B_F_STATES, B_F_COLOR, B_F_MASK, B_F_CLAIM, B_F_USED0, B_F_USED1, B_F_ACC0,
     B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_POSTPLANES, B_WEAKLIST, B_ADJ_WEIGHTS,
-    B_ADJ_GRADS, B_QUADS_H, B_FLAG, B_COUNT
+    B_ADJ_GRADS, B_QUADS_H, B_FLAG, B_WPRE, B_WSUM, B_COUNT
 };
 
 struct Fail {
@@ -359,6 +359,15 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
     d.weak_colour_count[0] = cc[0];
     d.weak_colour_list[1] = dwl + weak_count + cc[0];
     d.weak_colour_count[1] = cc[1];
+    {
+        const size_t n = (size_t)cc[0] + cc[1] + 1;
+        float2 *wpre = (float2 *)buffer(ctx, B_WPRE, sizeof(float2) * kWeakPatchTaps * n);
+        float *wsum = (float *)buffer(ctx, B_WSUM, sizeof(float) * 3 * kNeighbourNum * n);
+        d.weak_wpre[0] = wpre;
+        d.weak_wpre[1] = wpre + (size_t)kWeakPatchTaps * cc[0];
+        d.weak_wsum[0] = wsum;
+        d.weak_wsum[1] = wsum + (size_t)3 * kNeighbourNum * cc[0];
+    }
     d.weak = (uint8_t *)buffer(ctx, B_WEAK, P);
     int *nmap = (int *)buffer(ctx, B_NMAP, sizeof(int) * P);
     HIPCHK(hipMemcpyAsync(d.weak, weak_h.data(), P, hipMemcpyHostToDevice, ctx->stream));

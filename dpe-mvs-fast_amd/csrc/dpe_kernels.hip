@@ -98,6 +98,68 @@ __device__ float ncc_window_direct(const Dev &d, int v, const float H[9], int cx
     return ncc_window_sums<false>(d, v, H, cx, cy, rc, radius, inc, taps);
 }
 
+// ---- weak-pixel windows with the reference half precomputed
+// The windows of ComputeBilateralNCCNew (DPE.cu:613-672) at a weak pixel -- the
+// centre window (adaptive radius) and up to 8 neighbour windows -- depend on
+// neither the hypothesis nor the view.  k_weak_patch stores, per weak pixel of
+// a colour list (entry i of n), the (w, w*ref) of every tap at
+// wpre[(t0_k + t) * n + i] and the reference sums (sum_ref, sum_ref_ref, wsum)
+// of window k at wsum[(3k + j) * n + i], computed with the same operations in
+// the same order as ncc_window_sums; wsum[2n + i] < 0 marks a pixel whose
+// centre window does not fit kWeakCentreTaps (it takes the direct path).
+constexpr int kWeakCentreTaps = 36;
+constexpr int kWeakNeighbourTaps = 9;
+static_assert(kWeakCentreTaps + 8 * kWeakNeighbourTaps == kWeakPatchTaps, "weak patch layout");
+__device__ __forceinline__ int weak_window_first_tap(int k) { return k == 0 ? 0 : kWeakCentreTaps + (k - 1) * kWeakNeighbourTaps; }
+
+__device__ __forceinline__ void weak_window_shape(const Dev &d, int center, int k, int &radius, int &inc) {
+    radius = (k == 0 ? d.prm.strong_radius : d.prm.weak_radius);
+    inc = (k == 0 ? d.prm.strong_increment : d.prm.weak_increment);
+    if (d.prm.use_radius && k == 0) {
+        radius = d.radius[center];
+        inc = max(2, (int)(2.0 * radius / 5.0));
+    }
+}
+
+template <bool REFINED, bool HALF>
+__device__ __forceinline__ float ncc_window_pre(const Dev &d, int v, const float H[9], int cx, int cy, int radius,
+                                                int inc, const float2 *__restrict__ tw, int n,
+                                                const float *__restrict__ ws, unsigned long long &taps) {
+    float sum_src = 0.0f, sum_src_src = 0.0f, sum_ref_src = 0.0f;
+    int t = 0;
+    for (int i = -radius; i <= radius; i += inc) {
+        float r_src = 0.0f, r_ss = 0.0f, r_rs = 0.0f;
+        for (int j = -radius; j <= radius; j += inc) {
+            const int rx = cx + i, ry = cy + j;
+            const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
+            const float src = sample_src<HALF>(d, v, sp.x, sp.y);
+            const float2 e = tw[(size_t)t * n];
+            r_src = fmaf(e.x, src, r_src);
+            r_ss = fmaf(e.x * src, src, r_ss);
+            r_rs = fmaf(e.y, src, r_rs);
+            t++;
+        }
+        sum_src += r_src;
+        sum_src_src += r_ss;
+        sum_ref_src += r_rs;
+    }
+    taps += t;
+    return ncc_from_sums(ws[0], ws[n], sum_src, sum_src_src, sum_ref_src, ws[2 * (size_t)n]);
+}
+
+__device__ float ncc_window_pre_any(const Dev &d, int v, const float H[9], int cx, int cy, int radius, int inc,
+                                    const float2 *__restrict__ tw, int n, const float *__restrict__ ws,
+                                    unsigned long long &taps) {
+    const int last = -radius + ((2 * radius) / inc) * inc;
+    const bool refined = refined_division_ok(H, cx - radius, cx + last, cy - radius, cy + last);
+    if (d.quads_h) {
+        if (refined) return ncc_window_pre<true, true>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
+        return ncc_window_pre<false, true>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
+    }
+    if (refined) return ncc_window_pre<true, false>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
+    return ncc_window_pre<false, false>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
+}
+
 __device__ float ncc_old_direct(const Dev &d, int x, int y, int v, const float4 &pl, unsigned long long &hyps,
                                 unsigned long long &taps) {
     const ViewConst &vc = d.vc[v];
@@ -169,6 +231,108 @@ __device__ float ncc_new_direct(const Dev &d, int x, int y, int v, const float4 
         }
     }
     return cost;
+}
+
+// ComputeBilateralNCCNew with the precomputed windows of colour-list entry i
+// (same result as ncc_new_direct).
+__device__ float ncc_new_pre(const Dev &d, int x, int y, int v, const float4 &pl, int colour, int i,
+                             unsigned long long &hyps, unsigned long long &taps) {
+    const int n = d.weak_colour_count[colour];
+    const float *ws = d.weak_wsum[colour] + i;
+    if (ws[2 * (size_t)n] < 0.0f) return ncc_new_direct(d, x, y, v, pl, hyps, taps);
+    const ViewConst &vc = d.vc[v];
+    const int center = x + y * d.W;
+    const float cost_max = 2.0f;
+    float H[9];
+    homography(vc, d.cams[0], pl, H);
+    hyps++;
+    const float2 pt = project(H, x, y);
+    if (pt.x >= (float)vc.width || pt.x < 0.0f || pt.y >= (float)vc.height || pt.y < 0.0f) return cost_max;
+    float center_cost = 0.0f, strong_cost = 0.0f;
+    int strong_count = 0;
+    const float2 *tw = d.weak_wpre[colour] + i;
+    for (int k = 0; k < kNeighbourNum; ++k) {
+        const short2 np = neighbour_point(d, center, k);
+        if (np.x == -1 || np.y == -1) continue;
+        const float2 nsp = project(H, np.x, np.y);
+        if (nsp.x < 0 || nsp.y < 0 || nsp.x >= (float)d.W || nsp.y >= (float)d.H) {
+            if (k != 0) {
+                const uint32_t vi = d.selected[np.x + np.y * d.W];
+                if (isSet(vi, v - 1)) {
+                    strong_cost += cost_max;
+                    strong_count++;
+                }
+                continue;
+            } else {
+                return cost_max;
+            }
+        }
+        int radius, inc;
+        weak_window_shape(d, center, k, radius, inc);
+        const float tc = ncc_window_pre_any(d, v, H, np.x, np.y, radius, inc, tw + (size_t)weak_window_first_tap(k) * n,
+                                            n, ws + (size_t)3 * k * n, taps);
+        if (k == 0) {
+            center_cost = tc;
+        } else {
+            strong_cost += tc;
+            strong_count++;
+        }
+    }
+    if (strong_count == 0) return center_cost;
+    strong_cost /= strong_count;
+    strong_cost = fminf(strong_cost, cost_max);
+    return (float)(0.25 * center_cost + 0.75 * strong_cost);
+}
+
+// Reference halves of the weak windows of every entry of a colour list
+// (the layout described at ncc_window_pre).
+__global__ __launch_bounds__(256) void k_weak_patch(Dev d, int colour) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = d.weak_colour_count[colour];
+    if (i >= n) return;
+    const int p = d.weak_colour_list[colour][i];
+    const int x = p % d.W, y = p / d.W;
+    float2 *tw = d.weak_wpre[colour] + i;
+    float *ws = d.weak_wsum[colour] + i;
+    if (d.weak[p] != DPE_WEAK) return;
+    const float ss = d.prm.sigma_spatial, sc = d.prm.sigma_color;
+    const float rc = texel(d.ref, d.W, d.H, x, y);
+    for (int k = 0; k < kNeighbourNum; ++k) {
+        const short2 np = neighbour_point(d, p, k);
+        if (np.x == -1 || np.y == -1) continue;
+        int radius, inc;
+        weak_window_shape(d, p, k, radius, inc);
+        const int side = (2 * radius) / inc + 1;
+        if (side * side > (k == 0 ? kWeakCentreTaps : kWeakNeighbourTaps)) {
+            ws[2 * (size_t)n] = -1.0f;  // centre window or neighbour windows too large: direct path
+            return;
+        }
+        int t = weak_window_first_tap(k);
+        float sum_ref = 0.0f, sum_ref_ref = 0.0f, wsum = 0.0f;
+        for (int a = -radius; a <= radius; a += inc) {
+            float r_ref = 0.0f, r_rr = 0.0f, r_w = 0.0f;
+            for (int b = -radius; b <= radius; b += inc) {
+                const float ref = texel(d.ref, d.W, d.H, np.x + a, np.y + b);
+                const float fi = (float)a, fj = (float)b;
+                const float spatial = sqrtf(fi * fi + fj * fj);
+                const float color = fabsf(ref - rc);
+                const float w = d_expf(-spatial / (2.0f * ss * ss) - color / (2.0f * sc * sc));
+                const float wr = w * ref;
+                r_ref = fmaf(w, ref, r_ref);
+                r_rr = fmaf(wr, ref, r_rr);
+                r_w = r_w + w;
+                tw[(size_t)t * n] = make_float2(w, wr);
+                t++;
+            }
+            sum_ref += r_ref;
+            sum_ref_ref += r_rr;
+            wsum += r_w;
+        }
+        ws[(size_t)(3 * k) * n] = sum_ref;
+        ws[(size_t)(3 * k + 1) * n] = sum_ref_ref;
+        if (k != 0) ws[(size_t)(3 * k + 2) * n] = wsum;
+        else ws[2 * (size_t)n] = wsum;
+    }
 }
 
 // Get3DPointonWorld_cu / ProjectonCamera_cu, DPE.cu:885-917
@@ -1599,10 +1763,10 @@ __global__ void k_ransac(Dev d, int iter) {
 // CheckerboardPropagationWeak + PlaneHypothesisRefinementWeak, DPE.cu:1672-1866, 1124-1216
 __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int iter, unsigned long long &hyps,
                                                   unsigned long long &taps) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.weak_colour_count[colour]) return;
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= d.weak_colour_count[colour]) return;
     const int W = d.W;
-    const int p = d.weak_colour_list[colour][i];
+    const int p = d.weak_colour_list[colour][li];
     const int x = p % W, y = p / W;
     if (d.weak[p] != DPE_WEAK) return;
     const int NV = d.NV;
@@ -1625,7 +1789,7 @@ __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int 
         positions[i] = np.x + np.y * W;
         flag[i] = true;
         const float4 pl = d.planes[positions[i]];
-        for (int v = 1; v < d.N; ++v) cost_array[i][v - 1] = ncc_new_direct(d, x, y, v, pl, hyps, taps);
+        for (int v = 1; v < d.N; ++v) cost_array[i][v - 1] = ncc_new_pre(d, x, y, v, pl, colour, li, hyps, taps);
         new_plane[i] = pl;
     }
     float priors[32];
@@ -1667,7 +1831,7 @@ __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int 
     const float4 cur = d.planes[p];
     float cost_now = 0.0f;
     for (int v = 1; v < d.N; ++v) {
-        const float c = ncc_new_direct(d, x, y, v, cur, hyps, taps);
+        const float c = ncc_new_pre(d, x, y, v, cur, colour, li, hyps, taps);
         if (geom) cost_now += vwl[v - 1] * (c + gf * geom_cost(d, x, y, v, cur));
         else cost_now += vwl[v - 1] * c;
     }
@@ -1692,7 +1856,7 @@ __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int 
         float tc = 0.0f;
         for (int v = 1; v < d.N; ++v) {
             if (vwl[v - 1] > 0) {
-                const float c = ncc_new_direct(d, x, y, v, tp, hyps, taps);
+                const float c = ncc_new_pre(d, x, y, v, tp, colour, li, hyps, taps);
                 if (geom) tc += vwl[v - 1] * (c + gf * geom_cost(d, x, y, v, tp));
                 else tc += vwl[v - 1] * c;
             }
@@ -2030,8 +2194,10 @@ void launch_ransac(const Dev &d, int iter, hipStream_t s) {
     if (d.weak_count > 0) hipLaunchKernelGGL(k_ransac, grid1(d.weak_count, 64), dim3(64), 0, s, d, iter);
 }
 void launch_weak_half(const Dev &d, int colour, int iter, hipStream_t s) {
-    if (d.weak_colour_count[colour] > 0)
+    if (d.weak_colour_count[colour] > 0) {
+        hipLaunchKernelGGL(k_weak_patch, grid1(d.weak_colour_count[colour], 256), dim3(256), 0, s, d, colour);
         hipLaunchKernelGGL(k_weak_update, grid1(d.weak_colour_count[colour], 64), dim3(64), 0, s, d, colour, iter);
+    }
 }
 void launch_depth_and_normal(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_depth_and_normal, grid1(d.P), dim3(256), 0, s, d);

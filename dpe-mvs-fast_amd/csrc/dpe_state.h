@@ -19,7 +19,8 @@ namespace dpe {
 
 constexpr int kStrongSlots = 17;  // 0 = current plane, 1..8 candidates, 9..16 fixed-step candidates
 constexpr int kRefineSlots = 5;
-constexpr int kPostSlots = 73;    // 0..60 DepthToWeak samples, 61 LocalRefine now, 62..72 LocalRefine samples
+constexpr int kPostSlots = 73;
+constexpr int kWeakPatchTaps = 36 + 8 * 9;  // centre window + 8 neighbour windows of ComputeBilateralNCCNew    // 0..60 DepthToWeak samples, 61 LocalRefine now, 62..72 LocalRefine samples
 
 struct Dev {
     int W, H, N, NV, P;
@@ -75,6 +76,9 @@ struct Dev {
     int weak_count;
     const int *weak_colour_list[2];
     int weak_colour_count[2];
+    // reference halves of the weak windows per colour-list entry (k_weak_patch)
+    float2 *weak_wpre[2];   // [kWeakPatchTaps][count]
+    float *weak_wsum[2];    // [3 * kNeighbourNum][count]
     // GenNeighbours constants (DPE.cu:2151-2157, host double math)
     float gn_cos, gn_sin, gn_thr;
     int gn_shift;
