@@ -60,7 +60,9 @@ __device__ __forceinline__ float ncc_adjoint_view(const Dev &d, const SharedPatc
             const float fx = floorf(qx), fy = floorf(qy);
             const float a = qx - fx, b = qy - fy;
             const int ix = f2i_clamp(fx, -1, d.W - 1), iy = f2i_clamp(fy, -1, d.H - 1);
-            const float4 t = d.quads_h ? quad_at<true>(d, v, ix, iy) : quad_at<false>(d, v, ix, iy);
+            const float4 t = d.quad_fmt == QUAD_FP16   ? quad_at<QUAD_FP16>(d, v, ix, iy)
+                             : d.quad_fmt == QUAD_BF16 ? quad_at<QUAD_BF16>(d, v, ix, iy)
+                                                       : quad_at<QUAD_FP32>(d, v, ix, iy);
             const float top = fmaf(a, t.y - t.x, t.x), bot = fmaf(a, t.w - t.z, t.z);
             const float s = fmaf(b, bot - top, top);
             const float s_u = (t.y - t.x) + b * ((t.w - t.z) - (t.y - t.x));

@@ -44,7 +44,7 @@ enum BufId {
     B_POSTBL, B_POSTDISP, B_POSTWN, B_POSTFLAGS, B_PCOST, B_COUNTERS, B_GIVEN_PLANES, B_GIVEN_COSTS,
     B_F_VIEWS, B_F_DEPTH, B_F_NORMAL, B_F_STATES, B_F_COLOR, B_F_MASK, B_F_CLAIM, B_F_USED0, B_F_USED1, B_F_ACC0,
     B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_POSTPLANES, B_WEAKLIST, B_ADJ_WEIGHTS,
-    B_ADJ_GRADS, B_QUADS_H, B_FLAG, B_WPRE, B_WSUM, B_COUNT
+    B_ADJ_GRADS, B_QUADS_H, B_FLAG, B_WPRE, B_WSUM, B_VC64, B_COUNT
 };
 
 struct Fail {
@@ -123,6 +123,29 @@ ViewConst view_const(const dpe_camera &ref, const dpe_camera &src) {
     return v;
 }
 
+// The same constants in double from the fp32 cameras (precision form).
+ViewConst64 view_const64(const dpe_camera &ref, const dpe_camera &src) {
+    ViewConst64 v{};
+    double rC[3], sC[3];
+    for (int j = 0; j < 3; ++j) {
+        rC[j] = -((double)ref.R[j] * ref.t[0] + (double)ref.R[3 + j] * ref.t[1] + (double)ref.R[6 + j] * ref.t[2]);
+        sC[j] = -((double)src.R[j] * src.t[0] + (double)src.R[3 + j] * src.t[1] + (double)src.R[6 + j] * src.t[2]);
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            v.Rr[3 * r + c] = (double)src.R[3 * r] * ref.R[3 * c] + (double)src.R[3 * r + 1] * ref.R[3 * c + 1] +
+                              (double)src.R[3 * r + 2] * ref.R[3 * c + 2];
+    for (int r = 0; r < 3; ++r)
+        v.tr[r] = (double)src.R[3 * r] * (rC[0] - sC[0]) + (double)src.R[3 * r + 1] * (rC[1] - sC[1]) +
+                  (double)src.R[3 * r + 2] * (rC[2] - sC[2]);
+    for (int i = 0; i < 9; ++i) v.Ks[i] = src.K[i];
+    v.Kr[0] = ref.K[0];
+    v.Kr[1] = ref.K[2];
+    v.Kr[2] = ref.K[4];
+    v.Kr[3] = ref.K[5];
+    return v;
+}
+
 struct Upload {
     dpe_ctx *ctx;
     bool device_in;
@@ -171,7 +194,7 @@ dpe_status validate(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in) {
 
 // Common set-up of images, cameras and constants.  Returns the Dev with the
 // image/camera pointers filled.
-Dev setup_views(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in) {
+Dev setup_views(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in, int precision = DPE_PRECISION_FP32) {
     Dev d{};
     d.W = in->width;
     d.H = in->height;
@@ -216,26 +239,39 @@ Dev setup_views(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in) {
     d.spatial = dsp;
     // copies above are stream-ordered; the host vectors must outlive them
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    // quad layout: fp16 when every texel of the call is fp16-exact (same samples, half the bytes)
-    int *flag = (int *)buffer(ctx, B_FLAG, sizeof(int));
-    HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
-    launch_check_fp16_exact(imgs, P * d.N, flag, ctx->stream);
-    int not_exact = 1;
-    HIPCHK(hipMemcpyAsync(&not_exact, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-#ifdef DPE_NO_FP16_QUADS
-    not_exact = 1;
-#endif
-    if (not_exact) {
+    // quad layout: the precision form's 16-bit format; else fp16 when every texel
+    // of the call is fp16-exact (the same samples in half the bytes), else fp32
+    int fmt = precision == DPE_PRECISION_FP16 ? QUAD_FP16 : precision == DPE_PRECISION_BF16 ? QUAD_BF16 : QUAD_FP32;
+    if (precision == DPE_PRECISION_FP32) {
+        int *flag = (int *)buffer(ctx, B_FLAG, sizeof(int));
+        HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+        launch_check_fp16_exact(imgs, P * d.N, flag, ctx->stream);
+        int not_exact = 1;
+        HIPCHK(hipMemcpyAsync(&not_exact, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        fmt = not_exact ? QUAD_FP32 : QUAD_FP16;
+    }
+    d.quad_fmt = fmt;
+    d.prec = precision;
+    if (fmt == QUAD_FP32) {
         float4 *quads = (float4 *)buffer(ctx, B_QUADS, sizeof(float4) * d.qs * d.N);
         for (int i = 0; i < d.N; ++i) launch_make_quads(d, imgs + P * i, quads + d.qs * i, ctx->stream);
         d.quads = quads;
         d.quads_h = nullptr;
     } else {
         uint2 *quads = (uint2 *)buffer(ctx, B_QUADS_H, sizeof(uint2) * d.qs * d.N);
-        for (int i = 0; i < d.N; ++i) launch_make_quads_h(d, imgs + P * i, quads + d.qs * i, ctx->stream);
+        for (int i = 0; i < d.N; ++i) launch_make_quads_h(d, imgs + P * i, quads + d.qs * i, fmt, ctx->stream);
         d.quads = nullptr;
         d.quads_h = quads;
+    }
+    d.vc64 = nullptr;
+    if (precision != DPE_PRECISION_FP32) {
+        std::vector<ViewConst64> v64(d.N);
+        for (int i = 0; i < d.N; ++i) v64[i] = view_const64(cams[0], cams[i]);
+        ViewConst64 *dv64 = (ViewConst64 *)buffer(ctx, B_VC64, sizeof(ViewConst64) * d.N);
+        HIPCHK(hipMemcpyAsync(dv64, v64.data(), sizeof(ViewConst64) * d.N, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        d.vc64 = dv64;
     }
     HIPCHK(hipGetLastError());
     return d;
@@ -301,7 +337,10 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
     const dpe_form form = form_in ? *form_in : dpe_form{};
     if (form.gauss_newton_steps < 0 || form.gauss_newton_steps > 2)
         return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "gauss_newton_steps must be 0, 1 or 2");
-    if (form.precision != DPE_PRECISION_FP32) return fail(ctx, DPE_ERR_UNSUPPORTED, "unknown precision");
+    if (form.precision < DPE_PRECISION_FP32 || form.precision > DPE_PRECISION_BF16)
+        return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "unknown precision");
+    if (form.precision != DPE_PRECISION_FP32 && 2 * prm->strong_radius / prm->strong_increment + 1 != 6)
+        return fail(ctx, DPE_ERR_UNSUPPORTED, "the precision form needs the 11x11 step-2 window");
     for (int r : form.reserved)
         if (r != 0) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "reserved form fields must be 0");
     if (form.gauss_newton_steps > 0 && 2 * prm->strong_radius / prm->strong_increment + 1 != 6)
@@ -311,7 +350,7 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
     HIPCHK(hipSetDevice(ctx->device));
     EventTimer tm{ctx};
     hipEvent_t ev_start = tm.get();
-    Dev d = setup_views(ctx, prm, in);
+    Dev d = setup_views(ctx, prm, in, form.precision);
     const size_t P = d.P, Q = d.Q;
     const int N = d.N, NV = d.NV;
     Upload up{ctx, in->memory == DPE_MEM_DEVICE};
@@ -552,17 +591,22 @@ dpe_status ncc_adjoint_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_input
     return DPE_OK;
 }
 
-dpe_status ncc_cost_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in, int num_hyps, const float *planes,
-                         float *costs) {
+dpe_status ncc_cost_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *form, const dpe_inputs *in, int num_hyps,
+                         const float *planes, float *costs) {
     dpe_params p = *prm;
     p.use_APD = p.use_edge = p.use_limit = p.use_label = p.geom_consistency = 0;
     p.state = DPE_FIRST_INIT;
     dpe_status st = validate(ctx, &p, in);
     if (st != DPE_OK) return st;
     if (num_hyps < 1 || !planes || !costs) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad hypotheses");
+    const int precision = form ? form->precision : DPE_PRECISION_FP32;
+    if (precision < DPE_PRECISION_FP32 || precision > DPE_PRECISION_BF16)
+        return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "unknown precision");
+    if (precision != DPE_PRECISION_FP32 && 2 * p.strong_radius / p.strong_increment + 1 != 6)
+        return fail(ctx, DPE_ERR_UNSUPPORTED, "the precision form needs the 11x11 step-2 window");
     HIPCHK(hipSetDevice(ctx->device));
     EventTimer tm{ctx};
-    Dev d = setup_views(ctx, &p, in);
+    Dev d = setup_views(ctx, &p, in, precision);
     const size_t P = d.P;
     const bool dev = in->memory == DPE_MEM_DEVICE;
     const float4 *pl = (const float4 *)planes;
@@ -768,9 +812,14 @@ dpe_status dpe_last_stats(const dpe_ctx *ctx, dpe_stats *stats) {
 
 dpe_status dpe_ncc_cost(dpe_ctx *ctx, const dpe_params *params, const dpe_inputs *in, int num_hyps,
                         const float *planes, float *costs) {
+    return dpe_ncc_cost_ex(ctx, params, nullptr, in, num_hyps, planes, costs);
+}
+
+dpe_status dpe_ncc_cost_ex(dpe_ctx *ctx, const dpe_params *params, const dpe_form *form, const dpe_inputs *in,
+                           int num_hyps, const float *planes, float *costs) {
     if (!ctx) return DPE_ERR_INVALID_ARGUMENT;
     try {
-        return ncc_cost_impl(ctx, params, in, num_hyps, planes, costs);
+        return ncc_cost_impl(ctx, params, form, in, num_hyps, planes, costs);
     } catch (const Fail &f) {
         return f.st;
     }

@@ -31,17 +31,24 @@ __global__ void k_make_quads(const float *__restrict__ img, float4 *__restrict__
                              texel(img, W, H, ix + 1, iy + 1));
 }
 
-__device__ __forceinline__ uint32_t half_pair(float lo, float hi) {
-    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)lo) | ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)hi) << 16);
+// fp32 -> fp16 (round to nearest even, as the conversion instruction does) or
+// bf16 (round to nearest even on the upper 16 bits; NaN stays NaN)
+__device__ __forceinline__ uint32_t to16(float v, int fmt) {
+    if (fmt == QUAD_BF16) {
+        const uint32_t u = __builtin_bit_cast(uint32_t, v);
+        if (v != v) return (u >> 16) | 0x40u;
+        return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+    }
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v);
 }
-__global__ void k_make_quads_h(const float *__restrict__ img, uint2 *__restrict__ quads, int W, int H) {
+__global__ void k_make_quads_h(const float *__restrict__ img, uint2 *__restrict__ quads, int W, int H, int fmt) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = (W + 1) * (H + 1);
     if (idx >= n) return;
     const int iy = idx / (W + 1) - 1;
     const int ix = idx % (W + 1) - 1;
-    quads[idx] = make_uint2(half_pair(texel(img, W, H, ix, iy), texel(img, W, H, ix + 1, iy)),
-                            half_pair(texel(img, W, H, ix, iy + 1), texel(img, W, H, ix + 1, iy + 1)));
+    quads[idx] = make_uint2(to16(texel(img, W, H, ix, iy), fmt) | (to16(texel(img, W, H, ix + 1, iy), fmt) << 16),
+                            to16(texel(img, W, H, ix, iy + 1), fmt) | (to16(texel(img, W, H, ix + 1, iy + 1), fmt) << 16));
 }
 __global__ void k_check_fp16_exact(const float *__restrict__ img, size_t n, int *__restrict__ flag) {
     bool bad = false;
@@ -64,7 +71,7 @@ __device__ __forceinline__ float ncc_window_sums(const Dev &d, int v, const floa
             const int rx = cx + i, ry = cy + j;
             const float ref = texel(d.ref, d.W, d.H, rx, ry);
             const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
-            const float src = d.quads_h ? sample_src<true>(d, v, sp.x, sp.y) : sample_src<false>(d, v, sp.x, sp.y);
+            const float src = sample_src_any(d, v, sp.x, sp.y);
             const float fi = (float)i, fj = (float)j;
             const float spatial = sqrtf(fi * fi + fj * fj);
             const float color = fabsf(ref - rc);
@@ -121,7 +128,7 @@ __device__ __forceinline__ void weak_window_shape(const Dev &d, int center, int 
     }
 }
 
-template <bool REFINED, bool HALF>
+template <bool REFINED, int FMT>
 __device__ __forceinline__ float ncc_window_pre(const Dev &d, int v, const float H[9], int cx, int cy, int radius,
                                                 int inc, const float2 *__restrict__ tw, int n,
                                                 const float *__restrict__ ws, unsigned long long &taps) {
@@ -132,7 +139,7 @@ __device__ __forceinline__ float ncc_window_pre(const Dev &d, int v, const float
         for (int j = -radius; j <= radius; j += inc) {
             const int rx = cx + i, ry = cy + j;
             const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
-            const float src = sample_src<HALF>(d, v, sp.x, sp.y);
+            const float src = sample_src<FMT>(d, v, sp.x, sp.y);
             const float2 e = tw[(size_t)t * n];
             r_src = fmaf(e.x, src, r_src);
             r_ss = fmaf(e.x * src, src, r_ss);
@@ -152,12 +159,16 @@ __device__ float ncc_window_pre_any(const Dev &d, int v, const float H[9], int c
                                     unsigned long long &taps) {
     const int last = -radius + ((2 * radius) / inc) * inc;
     const bool refined = refined_division_ok(H, cx - radius, cx + last, cy - radius, cy + last);
-    if (d.quads_h) {
-        if (refined) return ncc_window_pre<true, true>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
-        return ncc_window_pre<false, true>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
+    if (d.quad_fmt == QUAD_FP16) {
+        if (refined) return ncc_window_pre<true, QUAD_FP16>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
+        return ncc_window_pre<false, QUAD_FP16>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
     }
-    if (refined) return ncc_window_pre<true, false>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
-    return ncc_window_pre<false, false>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
+    if (d.quad_fmt == QUAD_BF16) {
+        if (refined) return ncc_window_pre<true, QUAD_BF16>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
+        return ncc_window_pre<false, QUAD_BF16>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
+    }
+    if (refined) return ncc_window_pre<true, QUAD_FP32>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
+    return ncc_window_pre<false, QUAD_FP32>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
 }
 
 __device__ float ncc_old_direct(const Dev &d, int x, int y, int v, const float4 &pl, unsigned long long &hyps,
@@ -385,7 +396,7 @@ __device__ float geom_cost(const Dev &d, int x, int y, int v, const float4 &pl) 
 // entry skips the slot), else plane_src[slot*Qn + q].  With view_mask (indexed
 // by pixel, Qn == P) only the views whose bit is set are evaluated.  One
 // workgroup = 64 pixels x 4 waves; wave w evaluates slots w, w+4, ...
-template <int NT>
+template <int NT, bool PREC>
 __global__ __launch_bounds__(kLanes * kWaves) void k_ncc_full(Dev d, int colour, int Qn, int nslot,
                                                               const int *__restrict__ cand_pos,
                                                               const float4 *__restrict__ plane_src,
@@ -421,7 +432,7 @@ __global__ __launch_bounds__(kLanes * kWaves) void k_ncc_full(Dev d, int colour,
             const uint32_t mask = view_mask ? view_mask[q] : 0xFFFFFFFFu;
             for (int v = 1; v < d.N; ++v) {
                 if (!isSet(mask, v - 1)) continue;
-                out[((size_t)slot * d.NV + (v - 1)) * Qn + q] = ncc_shared<NT>(d, S, lane, x, y, v, pl, taps);
+                out[((size_t)slot * d.NV + (v - 1)) * Qn + q] = ncc_shared<NT, PREC>(d, S, lane, x, y, v, pl, taps);
                 hyps++;
             }
         }
@@ -489,7 +500,7 @@ __device__ __forceinline__ bool weighted_slot(const Dev &d, int Qn, int q, int p
 //     DPE.cu:1101-1113): sum over views with weight > 0 of w*cost, / weight_norm.
 //   MODE_POST: DepthToWeak samples (DPE.cu:2667-2690) and LocalRefine
 //     (DPE.cu:2776-2835), all over the pixel's selected views.
-template <int NT, int MODE>
+template <int NT, int MODE, bool PREC>
 __device__ __forceinline__ float weighted_sum(const Dev &d, const void *Sv, int lane, int x, int y,
                                               int slot, const uint8_t *vw, uint32_t sel, const float4 &pl, float wn,
                                               unsigned long long &hyps, unsigned long long &taps) {
@@ -502,7 +513,7 @@ __device__ __forceinline__ float weighted_sum(const Dev &d, const void *Sv, int 
         if (!use) continue;
         float c;
         if constexpr (NT > 0) {
-            c = ncc_shared<NT>(d, *(const SharedPatch<NT> *)Sv, lane, x, y, v, pl, taps);
+            c = ncc_shared<NT, PREC>(d, *(const SharedPatch<NT> *)Sv, lane, x, y, v, pl, taps);
             hyps++;
         } else {
             c = ncc_old_direct(d, x, y, v, pl, hyps, taps);
@@ -524,7 +535,7 @@ __device__ __forceinline__ float weighted_sum(const Dev &d, const void *Sv, int 
 }
 
 // One workgroup = 64 pixels x 4 waves; wave w evaluates slot 4*blockIdx.y + w.
-template <int NT, int MODE>
+template <int NT, int MODE, bool PREC>
 __global__ __launch_bounds__(kLanes * kWaves) void k_ncc_weighted(Dev d, int colour, int Qn, int nslot,
                                                                   float *__restrict__ out) {
     __shared__ SharedPatch<NT> S;
@@ -544,7 +555,7 @@ __global__ __launch_bounds__(kLanes * kWaves) void k_ncc_weighted(Dev d, int col
     uint32_t sel = 0;
     if (any && slot < nslot && weighted_slot<MODE>(d, Qn, q, p, slot, pl, wn, sel)) {
         const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
-        out[(size_t)slot * Qn + q] = weighted_sum<NT, MODE>(d, &S, lane, x, y, slot, vw, sel, pl, wn, hyps, taps);
+        out[(size_t)slot * Qn + q] = weighted_sum<NT, MODE, PREC>(d, &S, lane, x, y, slot, vw, sel, pl, wn, hyps, taps);
     }
     count_add(d.counters, (MODE == MODE_REFINE ? kFamRefine : kFamPost), hyps, taps);
 }
@@ -563,7 +574,7 @@ __global__ __launch_bounds__(256) void k_ncc_weighted_direct(Dev d, int colour, 
     if (active) active = weighted_slot<MODE>(d, Qn, q, p, slot, pl, wn, sel);
     if (active) {
         const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
-        out[(size_t)slot * Qn + q] = weighted_sum<0, MODE>(d, nullptr, 0, x, y, slot, vw, sel, pl, wn, hyps, taps);
+        out[(size_t)slot * Qn + q] = weighted_sum<0, MODE, false>(d, nullptr, 0, x, y, slot, vw, sel, pl, wn, hyps, taps);
     }
     count_add(d.counters, (MODE == MODE_REFINE ? kFamRefine : kFamPost), hyps, taps);
 }
@@ -2120,9 +2131,9 @@ void launch_make_quads(const Dev &d, const float *img, float4 *quads, hipStream_
     const int n = (d.W + 1) * (d.H + 1);
     hipLaunchKernelGGL(k_make_quads, grid1(n), dim3(256), 0, s, img, quads, d.W, d.H);
 }
-void launch_make_quads_h(const Dev &d, const float *img, uint2 *quads, hipStream_t s) {
+void launch_make_quads_h(const Dev &d, const float *img, uint2 *quads, int fmt, hipStream_t s) {
     const int n = (d.W + 1) * (d.H + 1);
-    hipLaunchKernelGGL(k_make_quads_h, grid1(n), dim3(256), 0, s, img, quads, d.W, d.H);
+    hipLaunchKernelGGL(k_make_quads_h, grid1(n), dim3(256), 0, s, img, quads, d.W, d.H, fmt);
 }
 void launch_check_fp16_exact(const float *img, size_t n, int *flag, hipStream_t s) {
     hipLaunchKernelGGL(k_check_fp16_exact, dim3(1024), dim3(256), 0, s, img, n, flag);
@@ -2154,8 +2165,8 @@ void launch_ncc_current(const Dev &d, hipStream_t s) {
     // prior-selected ones (ComputeMultiViewInitialCost, DPE.cu:832-861)
     const uint32_t *mask = d.prm.state == DPE_FIRST_INIT ? nullptr : d.selected;
     if (d.ntap == 6)
-        hipLaunchKernelGGL(k_ncc_full<6>, grid_shared(d.P, 1), dim3(kLanes * kWaves), 0, s, d, -1, d.P, 1, nullptr,
-                           d.planes, mask, d.cost_mat);
+        hipLaunchKernelGGL((d.prec ? k_ncc_full<6, true> : k_ncc_full<6, false>), grid_shared(d.P, 1),
+                           dim3(kLanes * kWaves), 0, s, d, -1, d.P, 1, nullptr, d.planes, mask, d.cost_mat);
     else
         hipLaunchKernelGGL(k_ncc_full_direct, grid_direct(d.P, 1), dim3(256), 0, s, d, -1, d.P, nullptr, d.planes,
                            mask, d.cost_mat);
@@ -2169,8 +2180,9 @@ void launch_strong_candidates(const Dev &d, int colour, int iter, hipStream_t s)
 void launch_ncc_strong(const Dev &d, int colour, hipStream_t s) {
     const int nslot = d.prm.use_edge ? kStrongSlots : 9;
     if (d.ntap == 6)
-        hipLaunchKernelGGL(k_ncc_full<6>, grid_shared(d.Q, nslot), dim3(kLanes * kWaves), 0, s, d, colour, d.Q, nslot,
-                           d.cand_pos, d.planes, (const uint32_t *)nullptr, d.cost_mat);
+        hipLaunchKernelGGL((d.prec ? k_ncc_full<6, true> : k_ncc_full<6, false>), grid_shared(d.Q, nslot),
+                           dim3(kLanes * kWaves), 0, s, d, colour, d.Q, nslot, d.cand_pos, d.planes,
+                           (const uint32_t *)nullptr, d.cost_mat);
     else
         hipLaunchKernelGGL(k_ncc_full_direct, grid_direct(d.Q, nslot), dim3(256), 0, s, d, colour, d.Q, d.cand_pos,
                            d.planes, (const uint32_t *)nullptr, d.cost_mat);
@@ -2180,8 +2192,9 @@ void launch_strong_select(const Dev &d, int colour, int iter, hipStream_t s) {
 }
 void launch_ncc_refine(const Dev &d, int colour, hipStream_t s) {
     if (d.ntap == 6)
-        hipLaunchKernelGGL((k_ncc_weighted<6, MODE_REFINE>), grid_shared_slots(d.Q, kRefineSlots), dim3(kLanes * kWaves), 0,
-                           s, d, colour, d.Q, kRefineSlots, d.rcost);
+        hipLaunchKernelGGL((d.prec ? k_ncc_weighted<6, MODE_REFINE, true> : k_ncc_weighted<6, MODE_REFINE, false>),
+                           grid_shared_slots(d.Q, kRefineSlots), dim3(kLanes * kWaves), 0, s, d, colour, d.Q,
+                           kRefineSlots, d.rcost);
     else
         hipLaunchKernelGGL(k_ncc_weighted_direct<MODE_REFINE>, grid_direct(d.Q, kRefineSlots), dim3(256), 0, s, d,
                            colour, d.Q, d.rcost);
@@ -2213,8 +2226,9 @@ void launch_post_planes(const Dev &d, hipStream_t s) {
 }
 void launch_ncc_post(const Dev &d, hipStream_t s) {
     if (d.ntap == 6)
-        hipLaunchKernelGGL((k_ncc_weighted<6, MODE_POST>), grid_shared_slots(d.P, kPostSlots), dim3(kLanes * kWaves), 0, s,
-                           d, -1, d.P, kPostSlots, d.pcost);
+        hipLaunchKernelGGL((d.prec ? k_ncc_weighted<6, MODE_POST, true> : k_ncc_weighted<6, MODE_POST, false>),
+                           grid_shared_slots(d.P, kPostSlots), dim3(kLanes * kWaves), 0, s, d, -1, d.P, kPostSlots,
+                           d.pcost);
     else
         hipLaunchKernelGGL(k_ncc_weighted_direct<MODE_POST>, grid_direct(d.P, kPostSlots), dim3(256), 0, s, d, -1, d.P,
                            d.pcost);
@@ -2224,8 +2238,9 @@ void launch_post_final(const Dev &d, hipStream_t s) {
 }
 void launch_ncc_given(const Dev &d, int num_hyps, const float4 *planes, float *costs, hipStream_t s) {
     if (d.ntap == 6)
-        hipLaunchKernelGGL(k_ncc_full<6>, grid_shared(d.P, num_hyps), dim3(kLanes * kWaves), 0, s, d, -1, d.P,
-                           num_hyps, nullptr, planes, (const uint32_t *)nullptr, costs);
+        hipLaunchKernelGGL((d.prec ? k_ncc_full<6, true> : k_ncc_full<6, false>), grid_shared(d.P, num_hyps),
+                           dim3(kLanes * kWaves), 0, s, d, -1, d.P, num_hyps, nullptr, planes,
+                           (const uint32_t *)nullptr, costs);
     else
         hipLaunchKernelGGL(k_ncc_full_direct, grid_direct(d.P, num_hyps), dim3(256), 0, s, d, -1, d.P, nullptr, planes,
                            (const uint32_t *)nullptr, costs);

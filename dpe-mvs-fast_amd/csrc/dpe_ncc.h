@@ -53,29 +53,37 @@ __device__ __forceinline__ void count_add(unsigned long long *counters, int fami
 // and their factor-2 pyramid levels, do), so the samples are bit-identical.
 // (Measured at 1600x1200 with 10 sources: two 8-B pairs from the plain fp32
 // image, a quarter of the footprint, ran 4-15 % slower than the fp32 quads.)
-__device__ __forceinline__ float half_bits(uint32_t h) {
-    return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
+// Quad formats (Dev::quad_fmt, QuadFormat): 16-bit formats hold 2 texels per 32-bit word.
+template <int FMT>
+__device__ __forceinline__ float texel16(uint32_t h) {
+    if constexpr (FMT == QUAD_BF16) return __builtin_bit_cast(float, h << 16);
+    else return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
 }
-template <bool HALF>
+template <int FMT>
 __device__ __forceinline__ float4 quad_at(const Dev &d, int v, int ix, int iy) {
     // 32-bit byte offsets: (W + 1) * (H + 1) * 16 < 2^32 is checked at set-up
     const uint32_t cell = (uint32_t)(iy + 1) * (uint32_t)(d.W + 1) + (uint32_t)(ix + 1);
-    if constexpr (HALF) {
+    if constexpr (FMT != QUAD_FP32) {
         const uint2 h = *(const uint2 *)((const char *)(d.quads_h + (size_t)v * d.qs) + cell * 8u);
-        return make_float4(half_bits(h.x & 0xFFFFu), half_bits(h.x >> 16), half_bits(h.y & 0xFFFFu),
-                           half_bits(h.y >> 16));
+        return make_float4(texel16<FMT>(h.x & 0xFFFFu), texel16<FMT>(h.x >> 16), texel16<FMT>(h.y & 0xFFFFu),
+                           texel16<FMT>(h.y >> 16));
     } else {
         return *(const float4 *)((const char *)(d.quads + (size_t)v * d.qs) + cell * 16u);
     }
 }
-template <bool HALF>
+template <int FMT>
 __device__ __forceinline__ float sample_src(const Dev &d, int v, float u, float w) {
     const float fx = floorf(u), fy = floorf(w);
     const float a = u - fx, b = w - fy;
-    const float4 t = quad_at<HALF>(d, v, f2i_clamp(fx, -1, d.W - 1), f2i_clamp(fy, -1, d.H - 1));
+    const float4 t = quad_at<FMT>(d, v, f2i_clamp(fx, -1, d.W - 1), f2i_clamp(fy, -1, d.H - 1));
     const float top = fmaf(a, t.y - t.x, t.x);
     const float bot = fmaf(a, t.w - t.z, t.z);
     return fmaf(b, bot - top, top);
+}
+__device__ __forceinline__ float sample_src_any(const Dev &d, int v, float u, float w) {
+    if (d.quad_fmt == QUAD_FP16) return sample_src<QUAD_FP16>(d, v, u, w);
+    if (d.quad_fmt == QUAD_BF16) return sample_src<QUAD_BF16>(d, v, u, w);
+    return sample_src<QUAD_FP32>(d, v, u, w);
 }
 
 constexpr int kLanes = 64;
@@ -85,6 +93,8 @@ template <int NT>
 struct SharedPatch {
     float2 wwr[NT * NT][kLanes];  // (w, w*ref) per tap
     float sum_ref[kLanes], sum_ref_ref[kLanes], wsum[kLanes];
+    // precision form: sums of w (ref - rc) and w (ref - rc)^2, rc = the centre texel
+    float sum_refc[kLanes], sum_refc2[kLanes], rc[kLanes];
 };
 
 template <int NT>
@@ -107,7 +117,7 @@ __device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int 
     __syncthreads();
     if (wave == 0 && valid) {
         // hypothesis-independent sums in the reference's row-partial order
-        float s_r = 0.0f, s_rr = 0.0f, s_w = 0.0f;
+        float s_r = 0.0f, s_rr = 0.0f, s_w = 0.0f, s_c = 0.0f, s_c2 = 0.0f;
         for (int ki = 0; ki < NT; ++ki) {
             float r_r = 0.0f, r_rr = 0.0f, r_w = 0.0f;
 #pragma unroll
@@ -117,6 +127,11 @@ __device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int 
                 r_r = fmaf(e.x, ref, r_r);
                 r_rr = fmaf(e.y, ref, r_rr);
                 r_w = r_w + e.x;
+                if (d.prec) {
+                    const float rd = ref - rc;
+                    s_c = fmaf(e.x, rd, s_c);
+                    s_c2 = fmaf(e.x * rd, rd, s_c2);
+                }
             }
             s_r += r_r;
             s_rr += r_rr;
@@ -125,6 +140,9 @@ __device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int 
         S.sum_ref[lane] = s_r;
         S.sum_ref_ref[lane] = s_rr;
         S.wsum[lane] = s_w;
+        S.sum_refc[lane] = s_c;
+        S.sum_refc2[lane] = s_c2;
+        S.rc[lane] = rc;
     }
     __syncthreads();
 }
@@ -132,7 +150,7 @@ __device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int 
 // Source sums of one window (DPE.cu:728-758) with the reference weights from
 // LDS; REFINED: the projection uses the refined division (exact, see
 // refined_division_ok).
-template <int NT, bool REFINED, bool HALF>
+template <int NT, bool REFINED, int FMT>
 __device__ __forceinline__ void shared_window_sums(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y,
                                                    int v, const float H[9], float &s_s, float &s_ss, float &s_rs) {
 #pragma unroll 1
@@ -143,11 +161,40 @@ __device__ __forceinline__ void shared_window_sums(const Dev &d, const SharedPat
         for (int kj = 0; kj < NT; ++kj) {
             const int ry = y + d.tap_off0 + kj * d.tap_inc;
             const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
-            const float src = sample_src<HALF>(d, v, sp.x, sp.y);
+            const float src = sample_src<FMT>(d, v, sp.x, sp.y);
             const float2 e = S.wwr[ki * NT + kj][lane];
             r_s = fmaf(e.x, src, r_s);
             r_ss = fmaf(e.x * src, src, r_ss);
             r_rs = fmaf(e.y, src, r_rs);
+        }
+        s_s += r_s;
+        s_ss += r_ss;
+        s_rs += r_rs;
+    }
+}
+
+// Precision form: the source sums shifted by the sample at the window centre's
+// projection (s' = s - sc) and the reference by the centre texel (w (r - rc) =
+// w r - w rc in one rounding), so the one-pass variances E[x^2] - E[x]^2 act on
+// centred values and do not cancel; 16-bit quads, fp32 accumulation.
+template <int NT, bool REFINED, int FMT>
+__device__ __forceinline__ void shifted_window_sums(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y,
+                                                    int v, const float H[9], float sc, float &s_s, float &s_ss,
+                                                    float &s_rs) {
+    const float rc = S.rc[lane];
+#pragma unroll 1
+    for (int ki = 0; ki < NT; ++ki) {
+        float r_s = 0.0f, r_ss = 0.0f, r_rs = 0.0f;
+        const int rx = x + d.tap_off0 + ki * d.tap_inc;
+#pragma unroll DPE_TAP_UNROLL
+        for (int kj = 0; kj < NT; ++kj) {
+            const int ry = y + d.tap_off0 + kj * d.tap_inc;
+            const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
+            const float src = sample_src<FMT>(d, v, sp.x, sp.y) - sc;
+            const float2 e = S.wwr[ki * NT + kj][lane];
+            r_s = fmaf(e.x, src, r_s);
+            r_ss = fmaf(e.x * src, src, r_ss);
+            r_rs = fmaf(fmaf(-e.x, rc, e.y), src, r_rs);
         }
         s_s += r_s;
         s_ss += r_ss;
@@ -161,19 +208,48 @@ __device__ __forceinline__ void window_sums_any(const Dev &d, const SharedPatch<
                                                 const float H[9], float &s_s, float &s_ss, float &s_rs) {
     const int x0 = x + d.tap_off0, y0 = y + d.tap_off0, span = (NT - 1) * d.tap_inc;
     const bool refined = refined_division_ok(H, x0, x0 + span, y0, y0 + span);
-    if (d.quads_h) {
-        if (refined) shared_window_sums<NT, true, true>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
-        else shared_window_sums<NT, false, true>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+    if (d.quad_fmt == QUAD_FP16) {
+        if (refined) shared_window_sums<NT, true, QUAD_FP16>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+        else shared_window_sums<NT, false, QUAD_FP16>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+    } else if (d.quad_fmt == QUAD_BF16) {
+        if (refined) shared_window_sums<NT, true, QUAD_BF16>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+        else shared_window_sums<NT, false, QUAD_BF16>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
     } else {
-        if (refined) shared_window_sums<NT, true, false>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
-        else shared_window_sums<NT, false, false>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+        if (refined) shared_window_sums<NT, true, QUAD_FP32>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+        else shared_window_sums<NT, false, QUAD_FP32>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
     }
 }
 
-// ComputeBilateralNCCOld (DPE.cu:696-782) with the reference half from LDS.
+// The precision form's NCC of one window: FP64 homography, shifted sums.
 template <int NT>
+__device__ __forceinline__ float ncc_precise(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y, int v,
+                                          const float4 &pl, unsigned long long &taps) {
+    const ViewConst &vc = d.vc[v];
+    float H[9];
+    homography64(d.vc64[v], pl, H);
+    const float2 pt = project(H, x, y);
+    if (pt.x >= (float)vc.width || pt.x < 0.0f || pt.y >= (float)vc.height || pt.y < 0.0f) return 2.0f;
+    const float sc = sample_src_any(d, v, pt.x, pt.y);
+    const int x0 = x + d.tap_off0, y0 = y + d.tap_off0, span = (NT - 1) * d.tap_inc;
+    const bool refined = refined_division_ok(H, x0, x0 + span, y0, y0 + span);
+    float s_s = 0.0f, s_ss = 0.0f, s_rs = 0.0f;
+    if (d.quad_fmt == QUAD_BF16) {
+        if (refined) shifted_window_sums<NT, true, QUAD_BF16>(d, S, lane, x, y, v, H, sc, s_s, s_ss, s_rs);
+        else shifted_window_sums<NT, false, QUAD_BF16>(d, S, lane, x, y, v, H, sc, s_s, s_ss, s_rs);
+    } else {
+        if (refined) shifted_window_sums<NT, true, QUAD_FP16>(d, S, lane, x, y, v, H, sc, s_s, s_ss, s_rs);
+        else shifted_window_sums<NT, false, QUAD_FP16>(d, S, lane, x, y, v, H, sc, s_s, s_ss, s_rs);
+    }
+    taps += NT * NT;
+    return ncc_from_sums(S.sum_refc[lane], S.sum_refc2[lane], s_s, s_ss, s_rs, S.wsum[lane]);
+}
+
+// ComputeBilateralNCCOld (DPE.cu:696-782) with the reference half from LDS;
+// PREC: the precision form's window (the kernels are instantiated per form).
+template <int NT, bool PREC>
 __device__ __forceinline__ float ncc_shared(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y, int v,
                                             const float4 &pl, unsigned long long &taps) {
+    if constexpr (PREC) return ncc_precise<NT>(d, S, lane, x, y, v, pl, taps);
     const ViewConst &vc = d.vc[v];
     float H[9];
     homography(vc, d.cams[0], pl, H);

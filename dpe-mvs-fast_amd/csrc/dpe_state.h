@@ -32,8 +32,11 @@ struct Dev {
     const ViewConst *vc;
     const float *spatial;   // [ntap*ntap] spatial weight exponents
     const float *ref;
-    const float4 *quads;     // fp32 quads, or null when quads_h holds them
-    const uint2 *quads_h;    // fp16 quads (4 halves per cell) when every texel is fp16-exact
+    const float4 *quads;     // fp32 quads (quad_fmt 0)
+    const uint2 *quads_h;    // 16-bit quads, 4 texels per cell: fp16 (quad_fmt 1) or bf16 (2)
+    int quad_fmt;
+    int prec;                // 0 plain; else the precision form (dpe_form.precision)
+    const struct ViewConst64 *vc64;  // double-precision view constants (precision form)
     const float *gdepth;
     float4 *planes;
     float *costs;
@@ -85,6 +88,10 @@ struct Dev {
     unsigned long long *counters;  // [2*family] NCC hyps, [2*family+1] taps; family = NccFamily
 };
 
+// Source quad formats (Dev::quad_fmt): fp32 float4 per bilinear cell, or two
+// 16-bit texels per 32-bit word (uint2 per cell).
+enum QuadFormat { QUAD_FP32 = 0, QUAD_FP16 = 1, QUAD_BF16 = 2 };
+
 // NCC kernel families, each counted and timed on its own (dpe_stats.family_*).
 enum NccFamily {
     kFamFull = 0,    // k_ncc_full: initial cost vectors and propagation candidates (8x/16x per pixel)
@@ -97,7 +104,7 @@ enum NccFamily {
 // Launch interface (dpe_kernels.hip); every function enqueues on `stream`.
 // Kernels named ncc_* evaluate bilateral NCC windows (the timed hot path).
 void launch_make_quads(const Dev &d, const float *img, float4 *quads, hipStream_t s);
-void launch_make_quads_h(const Dev &d, const float *img, uint2 *quads, hipStream_t s);
+void launch_make_quads_h(const Dev &d, const float *img, uint2 *quads, int fmt, hipStream_t s);
 // sets *flag (int, zeroed by the caller) when a value of img[0..n) is not fp16-exact
 void launch_check_fp16_exact(const float *img, size_t n, int *flag, hipStream_t s);
 void launch_gen_edge_inform(const Dev &d, hipStream_t s);

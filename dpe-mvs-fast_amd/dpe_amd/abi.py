@@ -52,7 +52,7 @@ class Outputs(C.Structure):
 
 NCC_FAMILIES = 4
 FAMILY_FULL, FAMILY_REFINE, FAMILY_POST, FAMILY_WEAK = range(NCC_FAMILIES)
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 
 class Stats(C.Structure):
@@ -62,7 +62,7 @@ class Stats(C.Structure):
                 ("family_launches", C.c_int32 * NCC_FAMILIES)]
 
 
-PRECISION_FP32 = 0
+PRECISION_FP32, PRECISION_FP16, PRECISION_BF16 = 0, 1, 2
 
 
 class Form(C.Structure):
@@ -150,6 +150,8 @@ def declare(lib):
     lib.dpe_last_stats.restype = C.c_int
     lib.dpe_ncc_cost.argtypes = [C.c_void_p, P(Params), P(Inputs), C.c_int, C.c_void_p, C.c_void_p]
     lib.dpe_ncc_cost.restype = C.c_int
+    lib.dpe_ncc_cost_ex.argtypes = [C.c_void_p, P(Params), C.c_void_p, P(Inputs), C.c_int, C.c_void_p, C.c_void_p]
+    lib.dpe_ncc_cost_ex.restype = C.c_int
     lib.dpe_ncc_cost_adjoint.argtypes = [C.c_void_p, P(Params), P(Inputs), C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p]
     lib.dpe_ncc_cost_adjoint.restype = C.c_int
@@ -161,4 +163,4 @@ def declare(lib):
 
 # Every symbol include/dpe_hip.h declares.
 EXPORTS = ["dpe_create", "dpe_destroy", "dpe_last_error", "dpe_abi_version", "dpe_process", "dpe_process_ex",
-           "dpe_last_stats", "dpe_ncc_cost", "dpe_ncc_cost_adjoint", "dpe_fuse"]
+           "dpe_last_stats", "dpe_ncc_cost", "dpe_ncc_cost_ex", "dpe_ncc_cost_adjoint", "dpe_fuse"]

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DPE_ABI_VERSION 4
+#define DPE_ABI_VERSION 5
 /* main.h:41-44 */
 #define DPE_MAX_IMAGES 32
 #define DPE_NEIGHBOUR_NUM 9
@@ -163,14 +163,23 @@ dpe_status dpe_last_stats(const dpe_ctx *ctx, dpe_stats *stats);
 /* Forms of the process call beyond the plain path (no counterpart in the
  * reference; results are not those of the reference's algorithm and are
  * judged against ground truth, not the oracle). */
-#define DPE_PRECISION_FP32 0
+#define DPE_PRECISION_FP32 0 /* plain */
+#define DPE_PRECISION_FP16 1 /* precision form: fp16 source gathers ... */
+#define DPE_PRECISION_BF16 2 /* ... or bf16; both with FP64 homography set-up and shifted (centred) NCC sums */
 typedef struct dpe_form {
     int32_t gauss_newton_steps; /* 0: plain.  1 or 2: of the 5 hypotheses of
                                    PlaneHypothesisRefinementStrong (DPE.cu:1080-1104), the perturbed
                                    normal (4th) and then the random depth + normal (3rd) become
                                    Gauss-Newton steps on the per-view costs as residuals (gradients
                                    from the adjoint): from the current plane, then from step 1 */
-    int32_t precision;          /* DPE_PRECISION_FP32 */
+    int32_t precision;          /* DPE_PRECISION_*.  FP16 / BF16: the source quads are stored in that
+                                   format (8 B per bilinear cell instead of 16), the homography is
+                                   composed in FP64 and rounded to fp32, and the NCC sums of the
+                                   shared-patch kernels (propagation, refinement, DepthToWeak,
+                                   LocalRefine) are taken on values centred at the window's centre
+                                   (reference texel, source sample at its projection), accumulated
+                                   in fp32.  The weak-texture windows read the same 16-bit quads with
+                                   the plain sums. */
     int32_t reserved[6];        /* 0 */
 } dpe_form;
 
@@ -185,6 +194,10 @@ dpe_status dpe_process_ex(dpe_ctx *ctx, const dpe_params *params, const dpe_form
  * cost against source view v+1.  num_hyps >= 1. */
 dpe_status dpe_ncc_cost(dpe_ctx *ctx, const dpe_params *params, const dpe_inputs *in,
                         int num_hyps, const float *planes, float *costs);
+
+/* dpe_ncc_cost with a form (its precision field; NULL = plain).  No reference counterpart. */
+dpe_status dpe_ncc_cost_ex(dpe_ctx *ctx, const dpe_params *params, const dpe_form *form, const dpe_inputs *in,
+                           int num_hyps, const float *planes, float *costs);
 
 /* Adjoint form (no counterpart in the reference, which has no derivatives):
  * the aggregated multi-view NCC cost C = sum_v w_v c_v / sum_v w_v of every

@@ -9,6 +9,8 @@ for plain and Gauss-Newton, and the iteration count each needs to reach plain's
 4-iteration accuracy.  Bar: within 1 point of plain at every iteration count
 (the random draws differ), and ahead of plain at 0.2 % after 2 iterations (the
 steps converge faster on the planes they start near)."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -50,5 +52,107 @@ def test_gauss_newton_form_against_ground_truth():
         pin = host.first_call_inputs(s, ref, s.pairs[ref], max_iterations=1)
         with pytest.raises(host.DPEError):
             host.run_hip(ctx, pin, abi.Form(gauss_newton_steps=3))
+        with pytest.raises(host.DPEError):
+            host.run_hip(ctx, pin, abi.Form(precision=3))
+    finally:
+        ctx.close()
+
+
+PRECISIONS = {"plain": abi.PRECISION_FP32, "fp16": abi.PRECISION_FP16, "bf16": abi.PRECISION_BF16}
+
+
+def _cost_inputs(seed, continuous):
+    s = scene.make_scene(320, 240, 300.0, 11, 30.0, seed=seed, n_src=10)
+    ref = s.meta["ref_mid"]
+    ids = [ref] + s.pairs[ref]
+    imgs = [s.views[i].image.astype(np.float32) for i in ids]
+    if continuous:
+        # intensities that are not 8-bit (not fp16-exact): a seeded sub-integer offset
+        rng = np.random.default_rng(seed + 1)
+        imgs = [(im + rng.uniform(-0.5, 0.5, im.shape)).astype(np.float32) for im in imgs]
+    cams = host.scene_cameras(s, ids)
+    prm = host.call_params(s.depth_min, s.depth_max, use_APD=0, use_edge=0, use_limit=0, use_label=0)
+    return s, ref, imgs, cams, host.ProblemInputs(imgs, cams, prm)
+
+
+def test_precision_form_cost_error_against_float64():
+    """Precision form (dpe_form.precision = FP16 / BF16) against the float64 cost.
+
+    Per-view NCC costs of perturbed ground-truth planes (dpe_ncc_cost_ex) on
+    320x240 inputs with 10 source views, 8-bit and continuous intensities,
+    compared with the float64 restatement tests/ncc64.py on 10^4 seeded pixels
+    (entries where the float64 cost is clamped, floored or the centre projects
+    outside are left out).  Reported per form: median, p99 and max absolute
+    error, on textured and weak-texture pixels.  Bars: on 8-bit images, where
+    the 16-bit quads are exact, both 16-bit forms are below plain at the median
+    and the p99 of weak-texture pixels (the one-pass fp32 variance of the
+    plain sums cancels there) and within 2x of plain's p99 on textured ones.
+    On continuous images the fp16 / bf16 rounding of the texels (2^-11 /
+    2^-8 relative) is the error and is reported; bars: fp16 p99 < 2e-3 on
+    textured pixels, bf16 p99 < 2e-2."""
+    from ncc64 import view_costs64
+    ctx = host.Context(0)
+    try:
+        for continuous in (False, True):
+            s, ref, imgs, cams, pin = _cost_inputs(0x5EED0004, continuous)
+            H, W = s.height, s.width
+            v = s.views[ref]
+            rng = np.random.default_rng(31)
+            n = v.normal @ v.R.T
+            depth = v.depth * (1 + 0.005 * rng.standard_normal(v.depth.shape))
+            ys, xs = np.mgrid[0:H, 0:W]
+            X = np.stack([depth * (xs - v.K[0, 2]) / v.K[0, 0], depth * (ys - v.K[1, 2]) / v.K[1, 1], depth], -1)
+            planes = np.concatenate([n, -np.sum(n * X, -1)[..., None]], -1).astype(np.float32)[None]
+            NV = len(imgs) - 1
+            ny, nx = np.mgrid[8:H - 8, 8:W - 8]
+            cand = np.stack([nx.ravel(), ny.ravel()], -1)
+            pix = cand[rng.choice(len(cand), 10000, replace=False)]
+            pl = planes[0, pix[:, 1], pix[:, 0]].astype(np.float64)
+            c64, sig = view_costs64(pix, pl[:, :3], pl[:, 3], imgs, cams)
+            c64 = np.stack(c64)
+            keep = np.stack([g[2] & ~g[3] & ~g[4] & ~g[5] for g in sig])
+            weak = np.broadcast_to(v.weak[pix[:, 1], pix[:, 0]], keep.shape)
+            stats = {}
+            for name, prec in PRECISIONS.items():
+                form = abi.Form(precision=prec)
+                out = np.zeros((1, NV, H, W), np.float32)
+                ctx.check(ctx.lib.dpe_ncc_cost_ex(ctx.h, C.byref(pin.params), C.byref(form), C.byref(pin.struct), 1,
+                                                  planes.ctypes.data, out.ctypes.data))
+                ms = ctx.stats().gpu_ms
+                err = np.abs(out[0][:, pix[:, 1], pix[:, 0]].astype(np.float64) - c64)
+                e_t, e_w = err[keep & ~weak], err[keep & weak]
+                stats[name] = dict(t50=np.median(e_t), t99=np.percentile(e_t, 99), tmax=e_t.max(),
+                                   w50=np.median(e_w), w99=np.percentile(e_w, 99), wmax=e_w.max())
+                print(f"{'continuous' if continuous else '8-bit'} {name}: textured median {stats[name]['t50']:.2e} "
+                      f"p99 {stats[name]['t99']:.2e} max {stats[name]['tmax']:.2e}; weak-texture median "
+                      f"{stats[name]['w50']:.2e} p99 {stats[name]['w99']:.2e} max {stats[name]['wmax']:.2e} "
+                      f"({ms:.2f} ms)")
+            p = stats["plain"]
+            if not continuous:
+                for name in ("fp16", "bf16"):
+                    q = stats[name]
+                    assert q["w50"] < p["w50"] and q["w99"] < p["w99"], name
+                    assert q["t99"] < 2 * p["t99"], name
+            else:
+                assert stats["fp16"]["t99"] < 2e-3
+                assert stats["bf16"]["t99"] < 2e-2
+    finally:
+        ctx.close()
+
+
+def test_precision_form_depth_maps():
+    """The precision form through dpe_process_ex on config-1 inputs: ground-truth
+    accuracy within 1 point of plain (|dd|/d < 1 % and < 0.2 %), and its call
+    time next to plain's."""
+    s = scene.config1()
+    ref = s.meta["ref_mid"]
+    ctx = host.Context(0)
+    try:
+        res = {}
+        for name, prec in PRECISIONS.items():
+            res[name] = _accuracy(ctx, s, ref, 4, abi.Form(precision=prec))
+            print(f"{name}: {res[name][0]:.4f} @1% / {res[name][1]:.4f} @0.2% ({res[name][2]:.1f} ms)")
+        for name in ("fp16", "bf16"):
+            assert res[name][0] >= res["plain"][0] - 0.01 and res[name][1] >= res["plain"][1] - 0.01, name
     finally:
         ctx.close()

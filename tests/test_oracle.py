@@ -335,6 +335,46 @@ def test_gt_planes_give_low_cost_on_synthetic_scene(orc):
     assert np.median(c_wrong[textured]) > np.median(c_gt[textured]) + 0.3
 
 
+def test_vectorised_float64_cost_matches_oracle(orc):
+    """tests/ncc64.py (the float64 multi-view cost the GPU precision and adjoint
+    tests measure against) agrees with the oracle on a 3-view scene, on
+    perturbed ground-truth planes, and with the per-pixel loop restatement."""
+    from ncc64 import view_costs64
+    s = scene.make_scene(160, 120, 150.0, 3, 20.0, seed=22)
+    ref = 1
+    ids = [ref] + s.pairs[ref]
+    v = s.views[ref]
+    cams = host.scene_cameras(s, ids)
+    prm = host.call_params(2, 6, use_APD=0, use_edge=0, use_limit=0, use_label=0)
+    imgs = [s.views[i].image.astype(np.float32) for i in ids]
+    pin = host.ProblemInputs(imgs, cams, prm)
+    rng = np.random.default_rng(23)
+    n = v.normal @ v.R.T
+    depth = v.depth * (1 + 0.01 * rng.standard_normal(v.depth.shape))
+    ys, xs = np.mgrid[0:120, 0:160]
+    X = np.stack([depth * (xs - v.K[0, 2]) / v.K[0, 0], depth * (ys - v.K[1, 2]) / v.K[1, 1], depth], -1)
+    planes = np.concatenate([n, -np.sum(n * X, -1)[..., None]], -1).astype(np.float32)[None]
+    got = _ncc(orc, pin, planes)[0]
+    iy, ix = np.mgrid[6:114, 6:154]
+    pix = np.stack([ix.ravel(), iy.ravel()], -1)
+    pl = planes[0, pix[:, 1], pix[:, 0]].astype(np.float64)
+    c64, _ = view_costs64(pix, pl[:, :3], pl[:, 3], imgs, cams)
+    textured = ~v.weak[pix[:, 1], pix[:, 0]]
+    for k in range(len(ids) - 1):
+        err = np.abs(got[k][pix[:, 1], pix[:, 0]] - c64[k])
+        assert np.median(err) < 1e-4
+        assert (err[textured] < 2e-3).mean() >= 0.99
+    # the loop restatement (independent tap and weight code; no bounds test) on a
+    # few pixels whose centre projects into the source
+    Hm = np.zeros(9, np.float32)
+    inside = np.flatnonzero(c64[0] < 1.9)
+    for t in inside[np.linspace(0, len(inside) - 1, 6).astype(int)]:
+        x, y = pix[t]
+        orc.oracle_homography(C.byref(cams[0]), C.byref(cams[1]), planes[0, y, x].ctypes.data, Hm.ctypes.data)
+        exp = _np_bilateral_ncc(imgs[0], imgs[1], x, y, Hm.reshape(3, 3).astype(np.float64))
+        assert abs(exp - c64[0][t]) < 1e-3, (x, y, exp, c64[0][t])
+
+
 # ------------------------------------------------------------ process calls
 def test_oracle_process_status_codes(orc):
     s = scene.make_scene(48, 40, 45.0, 3, 20.0, seed=5, supersample=1)
