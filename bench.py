@@ -25,7 +25,8 @@ sys.path.insert(0, os.path.join(ROOT, "dpe-mvs-fast_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-BYTES_PER_TAP = 20     # one 4-B reference texel + one 16-B bilinear source quad
+REF_BYTES_PER_TAP = 4  # one fp32 reference texel per tap; plus one bilinear source quad (dpe_stats.quad_bytes:
+                       # 16 B fp32, 8 B when the quads are stored in 16 bits)
 FAMILY_NAMES = ["k_ncc_full<6>", "k_ncc_weighted<6,REFINE>", "k_ncc_weighted<6,POST>", "k_weak_update"]
 # HBM traffic of one k_ncc_full<6> launch on this workload, from two rocprofv3
 # --pmc passes (FETCH_SIZE, WRITE_SIZE) over `python bench.py --no-cpu-baseline`,
@@ -45,6 +46,9 @@ def parse():
     ap.add_argument("--iters", type=int, default=8)
     ap.add_argument("--geom-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=["plain", "fp16", "bf16"], default="plain",
+                    help="dpe_form.precision of every call (plain = the reference's fp32 arithmetic)")
+    ap.add_argument("--gauss-newton", type=int, default=0, help="dpe_form.gauss_newton_steps of every call")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device-to-device 3 times (the known-size kernel that scales the "
@@ -108,6 +112,10 @@ def main():
     def ptr(t):
         return t.data_ptr()
 
+    form = abi.Form(gauss_newton_steps=args.gauss_newton,
+                    precision={"plain": abi.PRECISION_FP32, "fp16": abi.PRECISION_FP16,
+                               "bf16": abi.PRECISION_BF16}[args.precision])
+
     def call(v, first):
         srcs = sc.pairs[v]
         ids = [v] + srcs
@@ -137,7 +145,7 @@ def main():
         s.labels = ptr(labels[v])
         out = abi.Outputs(ptr(planes[v]), ptr(costs[v]), ptr(states[v]), ptr(selected[v]))
         torch.cuda.synchronize()
-        ctx.check(lib.dpe_process(ctx.h, C.byref(prm), C.byref(s), C.byref(out)))
+        ctx.check(lib.dpe_process_ex(ctx.h, C.byref(prm), C.byref(form), C.byref(s), C.byref(out)))
         st = ctx.stats()
         # result assembly (main.cpp:264-280) on the device
         d = planes[v][..., 3]
@@ -167,6 +175,7 @@ def main():
                 tot["hyps"] += st.ncc_hyps
                 tot["taps"] += st.ncc_taps
                 tot["gpu_ms"] += st.gpu_ms
+                tot["quad_bytes"] = st.quad_bytes
                 for f in range(4):
                     tot["fam_ms"][f] += st.family_ms[f]
                     tot["fam_hyps"][f] += st.family_hyps[f]
@@ -198,11 +207,13 @@ def main():
         rel = (depth[v] - gt).abs() / gt
         acc.append(float((rel < 0.01).float().mean()))
     # roofline of the dominant kernel, k_ncc_full<6> (candidate cost vectors):
-    # algorithmic bytes per launch (20 B per tap) / its average launch time,
-    # both from the hipEvent pairs and exact tap counters of the timed steps
+    # algorithmic bytes per launch (reference texel + source quad per tap) / its
+    # average launch time, both from the hipEvent pairs and exact tap counters
+    # of the timed steps
+    bytes_per_tap = REF_BYTES_PER_TAP + totals[-1]["quad_bytes"]
     full_launches = max(fam["fam_launches"][0], 1)
     full_avg_ms = fam["fam_ms"][0] / full_launches
-    full_bytes = fam["fam_taps"][0] * BYTES_PER_TAP / full_launches
+    full_bytes = fam["fam_taps"][0] * bytes_per_tap / full_launches
     achieved = full_bytes / (full_avg_ms * 1e-3) / 1e9
     families = {FAMILY_NAMES[f]: {"ms": fam["fam_ms"][f], "launches": fam["fam_launches"][f],
                                   "hyps": fam["fam_hyps"][f], "taps": fam["fam_taps"][f],
@@ -233,7 +244,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": NCC_FULL_TRAFFIC_BYTES,
                      "kernel": "k_ncc_full<6>", "launch_avg_ms": full_avg_ms, "bytes_per_launch": full_bytes,
-                     "launches": fam["fam_launches"][0]},
+                     "bytes_per_tap": bytes_per_tap, "launches": fam["fam_launches"][0]},
+        "form": {"precision": args.precision, "gauss_newton_steps": args.gauss_newton},
         "ncc_families": families,
     }
     if not args.no_cpu_baseline:
@@ -244,7 +256,7 @@ def main():
 
 def cpu_baseline(sc, threads):
     """The CPU oracle on a bounded sample of the same workload: one FIRST_INIT
-    call with 1 iteration on a 480x360 centre crop of reference view 10 and its
+    call with 2 iterations on a 960x720 centre crop of reference view 10 and its
     10 sources (principal points shifted), timed with `threads` host threads."""
     import numpy as np
     import oracle_lib
@@ -252,7 +264,7 @@ def cpu_baseline(sc, threads):
     nthreads = threads or min(16, os.cpu_count() or 1)
     ref = min(10, len(sc.views) - 1)
     ids = [ref] + sc.pairs[ref]
-    cw, ch = 480, 360
+    cw, ch = 960, 720
     x0, y0 = (sc.width - cw) // 2, (sc.height - ch) // 2
     imgs, cams = [], []
     for i in ids:
@@ -262,7 +274,7 @@ def cpu_baseline(sc, threads):
         K[0, 2] -= x0
         K[1, 2] -= y0
         cams.append(abi.camera_from_arrays(K, v.R, v.t, cw, ch, sc.depth_min, sc.depth_max))
-    prm = host.call_params(sc.depth_min, sc.depth_max, max_iterations=1, state=abi.FIRST_INIT, use_APD=0,
+    prm = host.call_params(sc.depth_min, sc.depth_max, max_iterations=2, state=abi.FIRST_INIT, use_APD=0,
                            use_edge=0, use_limit=0, use_label=0, geom_consistency=0)
     pin = host.ProblemInputs(imgs, cams, prm)
     lib = oracle_lib.load_oracle()
@@ -273,7 +285,7 @@ def cpu_baseline(sc, threads):
     # the GPU's unit: every family but the evaluations the reference never reads
     hyps = sum(h for h, _ in fam[:abi.NCC_FAMILIES])
     return {"value": hyps / dt, "unit": "NCC plane-hyp/s", "cores": nthreads, "kind": "port",
-            "sample": f"oracle FIRST_INIT call, 1 iteration, {cw}x{ch} centre crop of view {ref} + 10 sources, "
+            "sample": f"oracle FIRST_INIT call, 2 iterations, {cw}x{ch} centre crop of view {ref} + 10 sources, "
                       f"{hyps} NCC-hyps in {dt:.1f}s"}
 
 

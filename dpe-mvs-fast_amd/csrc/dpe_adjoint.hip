@@ -283,7 +283,10 @@ __global__ __launch_bounds__(kLanes * kWaves) void k_gn_step(Dev d, int colour, 
 }
 
 void launch_gn_step(const Dev &d, int colour, const float4 *src, float4 *dst, hipStream_t s) {
-    hipLaunchKernelGGL(k_gn_step<6>, dim3((d.Q + kLanes - 1) / kLanes), dim3(kLanes * kWaves), 0, s, d, colour, src,
+    // the steps use the plain window sums (on the call's quads) in every form
+    Dev dp = d;
+    dp.prec = 0;
+    hipLaunchKernelGGL(k_gn_step<6>, dim3((d.Q + kLanes - 1) / kLanes), dim3(kLanes * kWaves), 0, s, dp, colour, src,
                        dst);
 }
 

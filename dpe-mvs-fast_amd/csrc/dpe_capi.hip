@@ -123,26 +123,39 @@ ViewConst view_const(const dpe_camera &ref, const dpe_camera &src) {
     return v;
 }
 
-// The same constants in double from the fp32 cameras (precision form).
+// The precision form's view constants in double from the fp32 cameras:
+// A = Ks Rrel Kr^-1, b = Ks trel (Rrel = Rs Rr^T, trel = Rs (Cr - Cs)).
 ViewConst64 view_const64(const dpe_camera &ref, const dpe_camera &src) {
-    ViewConst64 v{};
-    double rC[3], sC[3];
+    double rC[3], sC[3], Rrel[9], trel[3], KsR[9];
     for (int j = 0; j < 3; ++j) {
         rC[j] = -((double)ref.R[j] * ref.t[0] + (double)ref.R[3 + j] * ref.t[1] + (double)ref.R[6 + j] * ref.t[2]);
         sC[j] = -((double)src.R[j] * src.t[0] + (double)src.R[3 + j] * src.t[1] + (double)src.R[6 + j] * src.t[2]);
     }
     for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 3; ++c)
-            v.Rr[3 * r + c] = (double)src.R[3 * r] * ref.R[3 * c] + (double)src.R[3 * r + 1] * ref.R[3 * c + 1] +
+            Rrel[3 * r + c] = (double)src.R[3 * r] * ref.R[3 * c] + (double)src.R[3 * r + 1] * ref.R[3 * c + 1] +
                               (double)src.R[3 * r + 2] * ref.R[3 * c + 2];
     for (int r = 0; r < 3; ++r)
-        v.tr[r] = (double)src.R[3 * r] * (rC[0] - sC[0]) + (double)src.R[3 * r + 1] * (rC[1] - sC[1]) +
+        trel[r] = (double)src.R[3 * r] * (rC[0] - sC[0]) + (double)src.R[3 * r + 1] * (rC[1] - sC[1]) +
                   (double)src.R[3 * r + 2] * (rC[2] - sC[2]);
-    for (int i = 0; i < 9; ++i) v.Ks[i] = src.K[i];
-    v.Kr[0] = ref.K[0];
-    v.Kr[1] = ref.K[2];
-    v.Kr[2] = ref.K[4];
-    v.Kr[3] = ref.K[5];
+    ViewConst64 v{};
+    for (int r = 0; r < 3; ++r) {
+        v.b[r] = (double)src.K[3 * r] * trel[0] + (double)src.K[3 * r + 1] * trel[1] + (double)src.K[3 * r + 2] * trel[2];
+        for (int c = 0; c < 3; ++c)
+            KsR[3 * r + c] = (double)src.K[3 * r] * Rrel[c] + (double)src.K[3 * r + 1] * Rrel[3 + c] +
+                             (double)src.K[3 * r + 2] * Rrel[6 + c];
+    }
+    // Kr^-1 = [[1/fx, 0, -cx/fx], [0, 1/fy, -cy/fy], [0, 0, 1]]
+    const double fx = ref.K[0], cx = ref.K[2], fy = ref.K[4], cy = ref.K[5];
+    for (int r = 0; r < 3; ++r) {
+        v.A[3 * r + 0] = KsR[3 * r + 0] / fx;
+        v.A[3 * r + 1] = KsR[3 * r + 1] / fy;
+        v.A[3 * r + 2] = KsR[3 * r + 2] - KsR[3 * r + 0] * cx / fx - KsR[3 * r + 1] * cy / fy;
+    }
+    v.kinv[0] = 1.0 / fx;
+    v.kinv[1] = 1.0 / fy;
+    v.kinv[2] = cx / fx;
+    v.kinv[3] = cy / fy;
     return v;
 }
 
@@ -539,6 +552,7 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
     dpe_stats stt{};
     HIPCHK(hipEventElapsedTime(&stt.gpu_ms, ev_start, ev_stop));
     tm.fill(stt, cnt);
+    stt.quad_bytes = d.quad_fmt == QUAD_FP32 ? 16 : 8;
     ctx->stats = stt;
     return DPE_OK;
 }
@@ -586,6 +600,7 @@ dpe_status ncc_adjoint_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_input
     HIPCHK(hipStreamSynchronize(ctx->stream));
     dpe_stats stt{};
     tm.fill(stt, cnt);
+    stt.quad_bytes = d.quad_fmt == QUAD_FP32 ? 16 : 8;
     stt.gpu_ms = stt.ncc_ms;
     ctx->stats = stt;
     return DPE_OK;
@@ -627,6 +642,7 @@ dpe_status ncc_cost_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *fo
     HIPCHK(hipStreamSynchronize(ctx->stream));
     dpe_stats stt{};
     tm.fill(stt, cnt);
+    stt.quad_bytes = d.quad_fmt == QUAD_FP32 ? 16 : 8;
     stt.gpu_ms = stt.ncc_ms;
     ctx->stats = stt;
     return DPE_OK;

@@ -264,36 +264,23 @@ __device__ __forceinline__ void homography(const ViewConst &v, const dpe_camera 
     H[8] = v.Ks[8] * t[8];
 }
 
-// Double-precision view constants and homography (precision form): the same
-// composition as homography() in FP64, rounded to fp32 at the end.
+// Double-precision view constants and homography (precision form).  The
+// plane-induced homography H = Ks (Rr - tr n^T / d) Kr^-1 is factored as
+// A - b c^T with A = Ks Rr Kr^-1 and b = Ks tr per view (host, double) and
+// c = Kr^-T n / d per plane: 9 FP64 FMAs per view, each entry rounded to fp32 once.
 struct ViewConst64 {
-    double Rr[9], tr[3], Ks[9];
-    double Kr[4];  // reference fx, cx, fy, cy
+    double A[9], b[3];
+    double kinv[4];  // reference 1/fx, 1/fy, cx/fx, cy/fy
 };
 __device__ __forceinline__ void homography64(const ViewConst64 &v, const float4 &pl, float H[9]) {
-    const double nx = pl.x, ny = pl.y, nz = pl.z, dd = pl.w;
-    double G[9];
-    for (int r = 0; r < 3; ++r) {
-        G[3 * r + 0] = v.Rr[3 * r + 0] - v.tr[r] * nx / dd;
-        G[3 * r + 1] = v.Rr[3 * r + 1] - v.tr[r] * ny / dd;
-        G[3 * r + 2] = v.Rr[3 * r + 2] - v.tr[r] * nz / dd;
-    }
-    const double fx = v.Kr[0], cx = v.Kr[1], fy = v.Kr[2], cy = v.Kr[3];
-    double t[9];
-    for (int r = 0; r < 3; ++r) {
-        t[3 * r + 0] = G[3 * r + 0] / fx;
-        t[3 * r + 1] = G[3 * r + 1] / fy;
-        t[3 * r + 2] = -G[3 * r + 0] * cx / fx - G[3 * r + 1] * cy / fy + G[3 * r + 2];
-    }
-    H[0] = (float)(v.Ks[0] * t[0] + v.Ks[2] * t[6]);
-    H[1] = (float)(v.Ks[0] * t[1] + v.Ks[2] * t[7]);
-    H[2] = (float)(v.Ks[0] * t[2] + v.Ks[2] * t[8]);
-    H[3] = (float)(v.Ks[4] * t[3] + v.Ks[5] * t[6]);
-    H[4] = (float)(v.Ks[4] * t[4] + v.Ks[5] * t[7]);
-    H[5] = (float)(v.Ks[4] * t[5] + v.Ks[5] * t[8]);
-    H[6] = (float)(v.Ks[8] * t[6]);
-    H[7] = (float)(v.Ks[8] * t[7]);
-    H[8] = (float)(v.Ks[8] * t[8]);
+    const double inv_d = 1.0 / (double)pl.w;
+    const double nx = pl.x, ny = pl.y, nz = pl.z;
+    const double c[3] = {nx * v.kinv[0] * inv_d, ny * v.kinv[1] * inv_d,
+                         fma(-ny, v.kinv[3], fma(-nx, v.kinv[2], nz)) * inv_d};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) H[3 * r + k] = (float)fma(-v.b[r], c[k], v.A[3 * r + k]);
 }
 
 // ComputeCorrespondingPoint, DPE.cu:519-526 (projection written with fmaf)

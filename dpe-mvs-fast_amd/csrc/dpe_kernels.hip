@@ -1303,6 +1303,16 @@ __global__ void k_find_nearest_strong(Dev d, const int16_t *__restrict__ g) {
     d.nearest[p] = result;
 }
 
+// A pixel (x, y) packed into a float's bits (moved, never computed on).
+__device__ __forceinline__ float pack_pixel(short2 s) {
+    return __builtin_bit_cast(float, (uint32_t)(uint16_t)s.x | ((uint32_t)(uint16_t)s.y << 16));
+}
+__device__ __forceinline__ short2 unpack_pixel(float f) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return make_short2((short)(u & 0xFFFFu), (short)(u >> 16));
+}
+__device__ __forceinline__ bool pixel_valid(float f) { return __builtin_bit_cast(uint32_t, f) != 0xFFFFFFFFu; }
+
 // GenNeighbours, DPE.cu:2107-2467
 __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1319,12 +1329,13 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
     Rng rng(d.prm.seed, p, PH_GEN_NEIGHBOURS, d.prm.call_index);
     for (int i = 0; i < kNeighbourNum; ++i) neighbours[i] = make_short2(-1, -1);
     neighbours[0] = make_short2((short)px, (short)py);
-    short2 strong_points[max_pt_num];
-    bool dir_valid[max_pt_num];
-    for (int i = 0; i < max_pt_num; ++i) {
-        strong_points[i] = make_short2(-1, -1);
-        dir_valid[i] = false;
-    }
+    // One 16-B record per candidate strong point (private memory): first the
+    // point only (.w = packed pixel, (-1, -1) = none), after compaction also the
+    // iteration-invariant terms of the RANSAC residual: fx = (x - cx) / fx_K,
+    // fy likewise, and the point's depth (the reference recomputes them per
+    // iteration and keeps 3D points and normals in arrays; the values are the same).
+    float4 pts[max_pt_num];
+    for (int i = 0; i < max_pt_num; ++i) pts[i] = make_float4(0.0f, 0.0f, 0.0f, pack_pixel(make_short2(-1, -1)));
     int odi = -1;
     int strong_point_size = 0;
     const int rotate_time = d.prm.rotate_time;
@@ -1374,13 +1385,12 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
                         normalize2(&td);
                         const float ca = td.x * od.x + td.y * od.y;
                         if (ca > d.gn_thr && (!edge_limit || !bresenham(d, px, py, np.x, np.y))) {
-                            strong_points[di] = np;
-                            dir_valid[di] = true;
+                            pts[di].w = pack_pixel(np);
                             strong_point_size++;
                             break;
                         }
                     }
-                    if (dir_valid[di]) break;
+                    if (pixel_valid(pts[di].w)) break;
                 }
                 float2 rd;
                 rd.x = od.x * d.gn_cos - od.y * d.gn_sin;
@@ -1434,8 +1444,7 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
                 if (d.label[nc] != 0 && d.label[nc] != d.label[p]) continue;
                 if (extend_index >= max_pt_num - 1) continue;
                 extend_index++;
-                strong_points[extend_index] = np;
-                dir_valid[extend_index] = true;
+                pts[extend_index].w = pack_pixel(np);
                 strong_point_size++;
             }
         }
@@ -1446,30 +1455,40 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
     }
     float4 best_plane = make_float4(0, 0, 0, 0);
     bool has_valid_plane = false;
-    short2 sv[max_pt_num];
-    float3 s3[max_pt_num];
-    float3 sn[max_pt_num];
+    float residuals[max_pt_num];
     int vc = 0;
     float X[3];
     get3d(cam, px, py, d.planes[p].w, X);
     const float cpz = X[2];
+    // compact the valid points to the front in index order (the rest: none)
     for (int i = 0; i < max_pt_num; ++i) {
-        sv[i] = make_short2(-1, -1);
-        if (dir_valid[i]) {
-            const short2 sp = strong_points[i];
-            const int sc = sp.x + sp.y * W;
-            sv[vc] = sp;
-            get3d(cam, sp.x, sp.y, d.planes[sc].w, X);
-            s3[vc] = make_float3(X[0], X[1], X[2]);
-            const float4 n4 = to_camera(cam, d.planes[sc]);
-            sn[vc] = make_float3(n4.x, n4.y, n4.z);
+        const float pw = pts[i].w;
+        pts[i].w = pack_pixel(make_short2(-1, -1));
+        if (pixel_valid(pw)) {
+            const short2 sp = unpack_pixel(pw);
+            const float fx = ((float)sp.x - cam.K[2]) / cam.K[0];
+            const float fy = ((float)sp.y - cam.K[5]) / cam.K[4];
+            pts[vc] = make_float4(fx, fy, d.planes[sp.x + sp.y * W].w, pw);
             vc++;
         }
     }
+    // the 3D point (s3) and camera-frame normal (sn) of a compacted point
+    auto point3 = [&](int k) {
+        const short2 sp = unpack_pixel(pts[k].w);
+        float Y[3];
+        get3d(cam, sp.x, sp.y, pts[k].z, Y);
+        return make_float3(Y[0], Y[1], Y[2]);
+    };
+    auto normal3 = [&](int k) {
+        const short2 sp = unpack_pixel(pts[k].w);
+        const float4 n4 = to_camera(cam, d.planes[sp.x + sp.y * W]);
+        return make_float3(n4.x, n4.y, n4.z);
+    };
     {
         int iteration = 50, max_iter = d.prm.high_res_img ? 200 : 125, max_count = 3;
         float min_cost = 3.402823466e+38f;
-        float residuals[max_pt_num];
+        // residuals of the current plane; only read where the threshold is
+        // tightened, so they are written there (entries >= vc stay 0)
         for (int i = 0; i < max_pt_num; ++i) residuals[i] = 0.0f;
         float temp_thr = ransac_threshold;
         bool has_consist = false;
@@ -1480,23 +1499,25 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
             const int b = (int)(rng.next() % (uint32_t)vc);
             const int c = (int)(rng.next() % (uint32_t)vc);
             if (a == b || b == c || a == c) continue;
-            if (must_in_triangle && !point_in_triangle(sv[a], sv[b], sv[c], px, py)) continue;
+            const short2 sa = unpack_pixel(pts[a].w), sb = unpack_pixel(pts[b].w), sc3 = unpack_pixel(pts[c].w);
+            if (must_in_triangle && !point_in_triangle(sa, sb, sc3, px, py)) continue;
             if (edge_limit) {
-                if (bresenham(d, sv[a].x, sv[a].y, sv[b].x, sv[b].y) || bresenham(d, sv[b].x, sv[b].y, sv[c].x, sv[c].y) ||
-                    bresenham(d, sv[c].x, sv[c].y, sv[a].x, sv[a].y))
+                if (bresenham(d, sa.x, sa.y, sb.x, sb.y) || bresenham(d, sb.x, sb.y, sc3.x, sc3.y) ||
+                    bresenham(d, sc3.x, sc3.y, sa.x, sa.y))
                     continue;
             }
             bool normal_consistency = false;
             if (d.prm.geom_consistency && edge_limit) {
                 normal_consistency = true;
-                const float ab = sn[a].x * sn[b].x + sn[a].y * sn[b].y + sn[a].z * sn[b].z;
-                const float ac = sn[a].x * sn[c].x + sn[a].y * sn[c].y + sn[a].z * sn[c].z;
-                const float bc = sn[b].x * sn[c].x + sn[b].y * sn[c].y + sn[b].z * sn[c].z;
+                const float3 na = normal3(a), nb = normal3(b), nc = normal3(c);
+                const float ab = na.x * nb.x + na.y * nb.y + na.z * nb.z;
+                const float ac = na.x * nc.x + na.y * nc.y + na.z * nc.z;
+                const float bc = nb.x * nc.x + nb.y * nc.y + nb.z * nc.z;
                 if ((double)ab < 0.8660254 || (double)ac < 0.8660254 || (double)bc < 0.8660254) normal_consistency = false;
                 if (has_consist && !normal_consistency) continue;
             }
             iteration--;
-            const float3 A = s3[a], B = s3[b], Cc = s3[c];
+            const float3 A = point3(a), B = point3(b), Cc = point3(c);
             const float3 AC = make_float3(A.x - Cc.x, A.y - Cc.y, A.z - Cc.z);
             const float3 BC = make_float3(B.x - Cc.x, B.y - Cc.y, B.z - Cc.z);
             float4 cr;
@@ -1508,12 +1529,13 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
             cr.w = -(cr.x * A.x + cr.y * A.y + cr.z * A.z);
             int temp_count = 0;
             float strong_dist = 0.0f;
+            auto residual = [&](int si) {
+                const float4 q = pts[si];
+                const float fit_depth = -cr.w / (cr.x * q.x + cr.y * q.y + cr.z);
+                return fabsf(fit_depth - q.z);
+            };
             for (int si = 0; si < vc; ++si) {
-                const float fx = ((float)sv[si].x - cam.K[2]) / cam.K[0];
-                const float fy = ((float)sv[si].y - cam.K[5]) / cam.K[4];
-                const float fit_depth = -cr.w / (cr.x * fx + cr.y * fy + cr.z);
-                const float distance = fabsf(fit_depth - s3[si].z);
-                residuals[si] = distance;
+                const float distance = residual(si);
                 if (distance < temp_thr) {
                     temp_count++;
                     strong_dist += distance;
@@ -1521,7 +1543,7 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
             }
             if (temp_count < 6) continue;
             if (temp_count > max_count) {
-                if (!must_in_triangle && point_in_triangle(sv[a], sv[b], sv[c], px, py)) must_in_triangle = true;
+                if (!must_in_triangle && point_in_triangle(sa, sb, sc3, px, py)) must_in_triangle = true;
                 if (!has_consist && normal_consistency) has_consist = true;
                 const float fx = ((float)px - cam.K[2]) / cam.K[0];
                 const float fy = ((float)py - cam.K[5]) / cam.K[4];
@@ -1533,6 +1555,7 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
                 min_cost = cd;
                 has_valid_plane = true;
                 if ((double)temp_thr > (d.prm.high_res_img ? 0.05 : 0.005)) {
+                    for (int si = 0; si < vc; ++si) residuals[si] = residual(si);
                     int j;
                     for (int i = 1; i < vc; i++) {
                         const float tmp = residuals[i];
@@ -1549,7 +1572,7 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
                     max_count = temp_count;
                 }
             } else if (temp_count == max_count) {
-                if (!must_in_triangle && point_in_triangle(sv[a], sv[b], sv[c], px, py)) must_in_triangle = true;
+                if (!must_in_triangle && point_in_triangle(sa, sb, sc3, px, py)) must_in_triangle = true;
                 const float fx = ((float)px - cam.K[2]) / cam.K[0];
                 const float fy = ((float)py - cam.K[5]) / cam.K[4];
                 const float fit_depth = -cr.w / (cr.x * fx + cr.y * fy + cr.z);
@@ -1567,14 +1590,13 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
         d.weak_rel[p] = 0;
         return;
     }
-    float weight[max_pt_num];
+    float *weight = residuals;  // no longer needed as residuals
     for (int i = 0; i < vc; ++i) {
-        const float fx = ((float)sv[i].x - cam.K[2]) / cam.K[0];
-        const float fy = ((float)sv[i].y - cam.K[5]) / cam.K[4];
-        const float fit_depth = -best_plane.w / (best_plane.x * fx + best_plane.y * fy + best_plane.z);
-        const float distance = fabsf(fit_depth - s3[i].z);
+        const float4 q = pts[i];
+        const float fit_depth = -best_plane.w / (best_plane.x * q.x + best_plane.y * q.y + best_plane.z);
+        const float distance = fabsf(fit_depth - q.z);
         if (distance >= ransac_threshold) {
-            sv[i] = make_short2(-1, -1);
+            pts[i].w = pack_pixel(make_short2(-1, -1));
             weight[i] = 3.402823466e+38f;
             continue;
         }
@@ -1583,17 +1605,17 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
     {
         int j;  // sort_small_weighted, DPE.cu:20-33
         for (int i = 1; i < vc; i++) {
-            const short2 tmp = sv[i];
+            const float tmp = pts[i].w;
             const float tw = weight[i];
             for (j = i; j >= 1 && tw < weight[j - 1]; j--) {
-                sv[j] = sv[j - 1];
+                pts[j].w = pts[j - 1].w;
                 weight[j] = weight[j - 1];
             }
-            sv[j] = tmp;
+            pts[j].w = tmp;
             weight[j] = tw;
         }
     }
-    for (int i = 1; i < kNeighbourNum; ++i) neighbours[i] = sv[i - 1];
+    for (int i = 1; i < kNeighbourNum; ++i) neighbours[i] = unpack_pixel(pts[i - 1].w);
     d.weak_rel[p] = 1;
 }
 

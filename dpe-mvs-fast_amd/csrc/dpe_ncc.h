@@ -94,7 +94,7 @@ struct SharedPatch {
     float2 wwr[NT * NT][kLanes];  // (w, w*ref) per tap
     float sum_ref[kLanes], sum_ref_ref[kLanes], wsum[kLanes];
     // precision form: sums of w (ref - rc) and w (ref - rc)^2, rc = the centre texel
-    float sum_refc[kLanes], sum_refc2[kLanes], rc[kLanes];
+    float sum_refc[kLanes], sum_refc2[kLanes];
 };
 
 template <int NT>
@@ -110,7 +110,8 @@ __device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int 
         if (valid) {
             const float ref = texel(d.ref, d.W, d.H, x + d.tap_off0 + ki * d.tap_inc, y + d.tap_off0 + kj * d.tap_inc);
             const float w = d_expf(d.spatial[k] - fabsf(ref - rc) / cden);
-            e = make_float2(w, w * ref);
+            // precision form: w (ref - rc), the reference centred at its centre texel
+            e = make_float2(w, w * (d.prec ? ref - rc : ref));
         }
         S.wwr[k][lane] = e;
     }
@@ -124,14 +125,14 @@ __device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int 
             for (int kj = 0; kj < NT; ++kj) {
                 const float2 e = S.wwr[ki * NT + kj][lane];
                 const float ref = texel(d.ref, d.W, d.H, x + d.tap_off0 + ki * d.tap_inc, y + d.tap_off0 + kj * d.tap_inc);
-                r_r = fmaf(e.x, ref, r_r);
-                r_rr = fmaf(e.y, ref, r_rr);
-                r_w = r_w + e.x;
                 if (d.prec) {
-                    const float rd = ref - rc;
-                    s_c = fmaf(e.x, rd, s_c);
-                    s_c2 = fmaf(e.x * rd, rd, s_c2);
+                    s_c += e.y;
+                    s_c2 = fmaf(e.y, ref - rc, s_c2);
+                } else {
+                    r_r = fmaf(e.x, ref, r_r);
+                    r_rr = fmaf(e.y, ref, r_rr);
                 }
+                r_w = r_w + e.x;
             }
             s_r += r_r;
             s_rr += r_rr;
@@ -142,7 +143,6 @@ __device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int 
         S.wsum[lane] = s_w;
         S.sum_refc[lane] = s_c;
         S.sum_refc2[lane] = s_c2;
-        S.rc[lane] = rc;
     }
     __syncthreads();
 }
@@ -174,14 +174,13 @@ __device__ __forceinline__ void shared_window_sums(const Dev &d, const SharedPat
 }
 
 // Precision form: the source sums shifted by the sample at the window centre's
-// projection (s' = s - sc) and the reference by the centre texel (w (r - rc) =
-// w r - w rc in one rounding), so the one-pass variances E[x^2] - E[x]^2 act on
-// centred values and do not cancel; 16-bit quads, fp32 accumulation.
+// projection (s' = s - sc) and the reference by the centre texel (LDS holds
+// w (r - rc)), so the one-pass variances E[x^2] - E[x]^2 act on centred values
+// and do not cancel; 16-bit quads, fp32 accumulation.
 template <int NT, bool REFINED, int FMT>
 __device__ __forceinline__ void shifted_window_sums(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y,
                                                     int v, const float H[9], float sc, float &s_s, float &s_ss,
                                                     float &s_rs) {
-    const float rc = S.rc[lane];
 #pragma unroll 1
     for (int ki = 0; ki < NT; ++ki) {
         float r_s = 0.0f, r_ss = 0.0f, r_rs = 0.0f;
@@ -194,7 +193,7 @@ __device__ __forceinline__ void shifted_window_sums(const Dev &d, const SharedPa
             const float2 e = S.wwr[ki * NT + kj][lane];
             r_s = fmaf(e.x, src, r_s);
             r_ss = fmaf(e.x * src, src, r_ss);
-            r_rs = fmaf(fmaf(-e.x, rc, e.y), src, r_rs);
+            r_rs = fmaf(e.y, src, r_rs);
         }
         s_s += r_s;
         s_ss += r_ss;

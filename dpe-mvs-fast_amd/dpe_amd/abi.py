@@ -52,14 +52,14 @@ class Outputs(C.Structure):
 
 NCC_FAMILIES = 4
 FAMILY_FULL, FAMILY_REFINE, FAMILY_POST, FAMILY_WEAK = range(NCC_FAMILIES)
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 
 class Stats(C.Structure):
     _fields_ = [("gpu_ms", C.c_float), ("ncc_ms", C.c_float), ("ncc_hyps", C.c_int64), ("ncc_taps", C.c_int64),
                 ("ncc_launches", C.c_int32), ("family_ms", C.c_float * NCC_FAMILIES),
                 ("family_hyps", C.c_int64 * NCC_FAMILIES), ("family_taps", C.c_int64 * NCC_FAMILIES),
-                ("family_launches", C.c_int32 * NCC_FAMILIES)]
+                ("family_launches", C.c_int32 * NCC_FAMILIES), ("quad_bytes", C.c_int32)]
 
 
 PRECISION_FP32, PRECISION_FP16, PRECISION_BF16 = 0, 1, 2

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DPE_ABI_VERSION 5
+#define DPE_ABI_VERSION 6
 /* main.h:41-44 */
 #define DPE_MAX_IMAGES 32
 #define DPE_NEIGHBOUR_NUM 9
@@ -140,6 +140,7 @@ typedef struct dpe_stats {
     int64_t family_hyps[DPE_NCC_FAMILIES];
     int64_t family_taps[DPE_NCC_FAMILIES];
     int32_t family_launches[DPE_NCC_FAMILIES];
+    int32_t quad_bytes;       /* bytes of one bilinear source quad (one per tap): 16 fp32, 8 fp16 / bf16 */
 } dpe_stats;
 
 typedef struct dpe_ctx dpe_ctx;

@@ -152,7 +152,11 @@ def test_precision_form_depth_maps():
         for name, prec in PRECISIONS.items():
             res[name] = _accuracy(ctx, s, ref, 4, abi.Form(precision=prec))
             print(f"{name}: {res[name][0]:.4f} @1% / {res[name][1]:.4f} @0.2% ({res[name][2]:.1f} ms)")
-        for name in ("fp16", "bf16"):
+        # with the Gauss-Newton form (its steps use the plain sums on the 16-bit quads)
+        res["fp16+gn2"] = _accuracy(ctx, s, ref, 4, abi.Form(gauss_newton_steps=2, precision=abi.PRECISION_FP16))
+        print(f"fp16 + 2 Gauss-Newton steps: {res['fp16+gn2'][0]:.4f} @1% / {res['fp16+gn2'][1]:.4f} @0.2% "
+              f"({res['fp16+gn2'][2]:.1f} ms)")
+        for name in ("fp16", "bf16", "fp16+gn2"):
             assert res[name][0] >= res["plain"][0] - 0.01 and res[name][1] >= res["plain"][1] - 0.01, name
     finally:
         ctx.close()
