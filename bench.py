@@ -49,6 +49,10 @@ def parse():
     ap.add_argument("--precision", choices=["plain", "fp16", "bf16"], default="plain",
                     help="dpe_form.precision of every call (plain = the reference's fp32 arithmetic)")
     ap.add_argument("--gauss-newton", type=int, default=0, help="dpe_form.gauss_newton_steps of every call")
+    ap.add_argument("--workload", choices=["c2", "batch"], default="c2",
+                    help="c2: the 1600x1200 20-view job (default); batch: 32 independent FIRST_INIT problems of "
+                         "uneven size and source count, sequential plain calls vs the batched form")
+    ap.add_argument("--lanes", type=int, default=4, help="lanes of the batched form (--workload batch)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device-to-device 3 times (the known-size kernel that scales the "
@@ -77,6 +81,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        batch_bench(args)
+        return
     import numpy as np
     import torch
     from dpe_amd import abi, host, scene, support
@@ -252,6 +259,130 @@ def main():
         result["cpu_baseline"] = cpu_baseline(sc, args.cpu_threads)
     print(json.dumps(result), flush=True)
     ctx.close()
+
+
+def batch_bench(args):
+    """32 FIRST_INIT problems (8 checkerboard iterations) with sizes drawn
+    uniformly from 640x480, 1024x768, 1600x1200 and 2048x1536 and 5-10 source
+    views each (seeded), inputs and outputs resident in HBM.  Timed: the
+    problems as sequential plain calls on one context, then as one
+    dpe_batch_process call (autotuner + graph form, `--lanes` lanes) after an
+    untimed batch that lets the autotuner decide.  The outputs of both are
+    compared bit for bit."""
+    import numpy as np
+    import torch
+    from dpe_amd import abi, host, scene
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(0x5EED00C5)
+    sizes = [(640, 480), (1024, 768), (1600, 1200), (2048, 1536)]
+    t0 = time.time()
+    scenes = [scene.make_scene(w, h, 1500.0 * w / 1600, 12, 60.0, seed=0x5EED0050 + k, n_src=10, supersample=1)
+              for k, (w, h) in enumerate(sizes)]
+    print(f"[bench] scenes ready in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    keep = []
+
+    def on_device(a):
+        t = torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        keep.append(t)
+        return t.data_ptr()
+
+    pins, structs, outs, meta = [], [], [], []
+    for i in range(32):
+        k = int(rng.integers(0, 4))
+        sc = scenes[k]
+        ref = int(rng.integers(0, len(sc.views)))
+        nsrc = int(rng.integers(5, 11))
+        pin = host.first_call_inputs(sc, ref, sc.pairs[ref][:nsrc], max_iterations=8)
+        s = abi.Inputs()
+        C.memmove(C.addressof(s), C.addressof(pin.struct), C.sizeof(abi.Inputs))
+        s.memory = abi.MEM_DEVICE
+        for j, im in enumerate(pin.images):
+            s.images[j] = on_device(im)
+        s.edges = on_device(pin.edges)
+        s.edges_low = on_device(pin.edges_low)
+        s.labels = on_device(pin.labels)
+        W, H = pin.width, pin.height
+        o = [torch.zeros((H, W, 4), dtype=torch.float32, device=dev), torch.zeros((H, W), dtype=torch.float32, device=dev),
+             torch.zeros((H, W), dtype=torch.uint8, device=dev), torch.zeros((H, W), dtype=torch.int32, device=dev)]
+        pins.append(pin)
+        structs.append(s)
+        outs.append(o)
+        meta.append((k, ref, nsrc))
+    torch.cuda.synchronize()
+
+    def outputs_struct(o):
+        return abi.Outputs(o[0].data_ptr(), o[1].data_ptr(), o[2].data_ptr(), o[3].data_ptr())
+
+    ctx = host.Context(0)
+    # sequential plain calls (one warm-up call first: the device pool grows to the largest problem)
+    ctx.check(ctx.lib.dpe_process(ctx.h, C.byref(pins[0].params), C.byref(structs[0]),
+                                  C.byref(outputs_struct(outs[0]))))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    hyps_seq = 0
+    for i in range(32):
+        ctx.check(ctx.lib.dpe_process(ctx.h, C.byref(pins[i].params), C.byref(structs[i]),
+                                      C.byref(outputs_struct(outs[i]))))
+        hyps_seq += ctx.stats().ncc_hyps
+    torch.cuda.synchronize()
+    seq_s = time.perf_counter() - t1
+    ref_out = [[t.clone() for t in o] for o in outs]
+    ctx.close()
+    print(f"[bench] sequential: {seq_s:.2f}s", file=sys.stderr, flush=True)
+    # batched form
+    b = host.Batch(0, lanes=args.lanes)
+    form = abi.Form(autotune=1, graph=1)
+    params = (abi.Params * 32)(*[p.params for p in pins])
+    inputs = (abi.Inputs * 32)(*structs)
+    outputs = (abi.Outputs * 32)(*[outputs_struct(o) for o in outs])
+    stats = (abi.Stats * 32)()
+    wall = C.c_float()
+    for rep in range(2):
+        for o in outs:
+            for t in o:
+                t.zero_()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        st = b.lib.dpe_batch_process(b.h, 32, params, C.byref(form), inputs, outputs, stats, C.byref(wall))
+        torch.cuda.synchronize()
+        bat_s = time.perf_counter() - t2
+        if st != abi.OK:
+            raise host.DPEError(st, b.lib.dpe_batch_last_error(b.h).decode())
+        print(f"[bench] batch {rep}: {bat_s:.2f}s", file=sys.stderr, flush=True)
+    hyps_bat = sum(stats[i].ncc_hyps for i in range(32))
+    tiles = sorted({(stats[i].tile_full, stats[i].tile_weighted) for i in range(32)})
+    b.close()
+    same = all(torch.equal(a.view(torch.uint8), r.view(torch.uint8)) for o, ro in zip(outs, ref_out)
+               for a, r in zip(o, ro))
+    acc = []
+    for (k, ref, nsrc), o in zip(meta, outs):
+        gt = torch.from_numpy(scenes[k].views[ref].depth).to(dev)
+        d = o[0][..., 3]
+        acc.append(float(((d - gt).abs() / gt < 0.01).float().mean()))
+    result = {
+        "metric": "NCC plane-hyp/s + s/depth-map, batched form vs sequential plain calls",
+        "value": hyps_bat / bat_s,
+        "unit": "NCC plane-hyp/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": bat_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded 8-plane scenes, one per size)",
+        "config": {"workload": "32 FIRST_INIT problems (8 iterations), sizes uniform in 640x480/1024x768/"
+                               "1600x1200/2048x1536, 5-10 source views", "lanes": args.lanes,
+                   "problems": [{"size": f"{sizes[k][0]}x{sizes[k][1]}", "sources": n} for k, _, n in meta]},
+        "depth_maps_per_s": {"sequential": 32 / seq_s, "batched": 32 / bat_s},
+        "hyp_per_s": {"sequential": hyps_seq / seq_s, "batched": hyps_bat / bat_s},
+        "speedup": seq_s / bat_s,
+        "bit_identical": same,
+        "tiles": tiles,
+        "accuracy_px_lt_1pct_vs_gt": float(np.mean(acc)),
+    }
+    print(json.dumps(result), flush=True)
 
 
 def cpu_baseline(sc, threads):

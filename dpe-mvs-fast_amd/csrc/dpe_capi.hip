@@ -8,14 +8,27 @@
 // CudaSpaceInitialization (DPE.cpp:934-1185); every exit() there is a status.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "dpe_state.h"
 #include "dpe_fuse.h"
+#include "dpe_ncc.h"
+
+// Autotuner record of one problem class (image size, source count, window
+// slots, precision): launch times of every tile, then the chosen tiles.
+struct dpe_tune {
+    int W = 0, H = 0, NV = 0, use_edge = 0, prec = 0;
+    bool decided = false;
+    int full = 0, wtd = 0;                 // chosen tile indices
+    std::vector<std::vector<float>> ms[2];  // [0] k_ncc_full, [1] k_ncc_weighted: per tile, per launch
+};
 
 struct dpe_ctx {
     int device = 0;
@@ -29,6 +42,10 @@ struct dpe_ctx {
     std::vector<hipEvent_t> events;
     dpe_stats stats{};
     size_t reserved = 0;
+    int lds_per_cu = 0, lds_per_block = 0;
+    std::vector<dpe_tune> tunes;
+    hipGraphExec_t graph = nullptr;  // the last captured schedule (graph form), updated in place when it fits
+    int graph_rebuilds = 0, graph_updates = 0;
 };
 
 static thread_local std::string g_create_error;
@@ -297,8 +314,9 @@ void copy_out(dpe_ctx *ctx, void *dst, const void *src, size_t bytes, bool devic
 struct EventTimer {
     dpe_ctx *ctx;
     int next = 0;
+    bool record = true;  // false inside a graph capture: the launches are not timed one by one
     struct Span {
-        int a, b, family;
+        int a, b, family, tile;  // tile: index of the tile timed for the autotuner, or -1
     };
     std::vector<Span> ncc;
     hipEvent_t get() {
@@ -310,14 +328,28 @@ struct EventTimer {
         return ctx->events[next++];
     }
     template <class F>
-    void ncc_launch(int family, F f) {
+    void ncc_launch(int family, F f, int tile = -1) {
+        if (!record) {
+            f();
+            HIPCHK(hipGetLastError());
+            return;
+        }
         const int a = next;
         HIPCHK(hipEventRecord(get(), ctx->stream));
         f();
         HIPCHK(hipGetLastError());
         const int b = next;
         HIPCHK(hipEventRecord(get(), ctx->stream));
-        ncc.push_back({a, b, family});
+        ncc.push_back({a, b, family, tile});
+    }
+    // launch times of the tuned tiles into the tuner record (stream synchronised)
+    void tune_samples(dpe_tune &t) {
+        for (const Span &sp : ncc) {
+            if (sp.tile < 0) continue;
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ctx->events[sp.a], ctx->events[sp.b]));
+            t.ms[sp.family == kFamFull ? 0 : 1][sp.tile].push_back(ms);
+        }
     }
     // Per-family launch times and the counters of the NCC kernels into `stt`;
     // gpu_ms is left to the caller.  Needs the stream synchronised.
@@ -343,6 +375,89 @@ struct EventTimer {
 
 constexpr int kNumCounters = 2 * kNumFamilies;
 
+// The autotuner's tiles (index = dpe_stats.tile_full / tile_weighted):
+//   0: 4 waves            1: 8 waves            2: 2 waves
+//   3: 4 waves, <= 3 workgroups per CU          4: 4 waves, <= 2 workgroups per CU
+// The caps come from dynamic LDS padding (the CU's LDS shared by fewer
+// workgroups); a cap the device cannot express is left out.
+constexpr int kTiles = 5;
+constexpr int kSharedPatchBytes = (int)sizeof(SharedPatch<6>);
+bool tile_config(const dpe_ctx *ctx, int i, NccTile &t) {
+    static const int waves[kTiles] = {4, 8, 2, 4, 4};
+    static const int cap[kTiles] = {0, 0, 0, 3, 2};
+    t.waves = waves[i];
+    t.lds_pad = 0;
+    if (cap[i] > 0) {
+        // LDS per workgroup such that exactly cap[i] workgroups fit in a CU
+        const int total = std::min(ctx->lds_per_cu / cap[i], ctx->lds_per_block) / 256 * 256;
+        if (total <= kSharedPatchBytes || ctx->lds_per_cu / total != cap[i]) return false;
+        t.lds_pad = total - kSharedPatchBytes;
+    }
+    return true;
+}
+
+dpe_tune *tune_record(dpe_ctx *ctx, const Dev &d) {
+    for (dpe_tune &t : ctx->tunes)
+        if (t.W == d.W && t.H == d.H && t.NV == d.NV && t.use_edge == d.prm.use_edge && t.prec == d.prec) return &t;
+    dpe_tune t;
+    t.W = d.W;
+    t.H = d.H;
+    t.NV = d.NV;
+    t.use_edge = d.prm.use_edge;
+    t.prec = d.prec;
+    t.ms[0].resize(kTiles);
+    t.ms[1].resize(kTiles);
+    ctx->tunes.push_back(std::move(t));
+    return &ctx->tunes.back();
+}
+
+// Decide once every available tile has 2 launches of each kernel timed: the
+// lowest median per kernel.
+void tune_decide(const dpe_ctx *ctx, dpe_tune &t) {
+    int best[2] = {0, 0};
+    float best_ms[2] = {1e30f, 1e30f};
+    for (int k = 0; k < 2; ++k)
+        for (int i = 0; i < kTiles; ++i) {
+            NccTile tc;
+            if (!tile_config(ctx, i, tc)) continue;
+            std::vector<float> v = t.ms[k][i];
+            if (v.size() < 2) return;
+            std::sort(v.begin(), v.end());
+            const float med = v[v.size() / 2];
+            if (med < best_ms[k]) {
+                best_ms[k] = med;
+                best[k] = i;
+            }
+        }
+    t.full = best[0];
+    t.wtd = best[1];
+    t.decided = true;
+}
+
+// Launch a captured schedule: update the context's executable graph in place
+// when the topology matches (kernel arguments and grids change), else
+// instantiate it anew.
+void launch_schedule_graph(dpe_ctx *ctx, hipGraph_t g) {
+    bool updated = false;
+    if (ctx->graph) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(ctx->graph, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+            updated = true;
+            ctx->graph_updates++;
+        } else {
+            (void)hipGetLastError();
+            HIPCHK(hipGraphExecDestroy(ctx->graph));
+            ctx->graph = nullptr;
+        }
+    }
+    if (!updated) {
+        HIPCHK(hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0));
+        ctx->graph_rebuilds++;
+    }
+    HIPCHK(hipGraphLaunch(ctx->graph, ctx->stream));
+}
+
 dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *form_in, const dpe_inputs *in,
                         dpe_outputs *out) {
     dpe_status st = validate(ctx, prm, in);
@@ -354,6 +469,8 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
         return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "unknown precision");
     if (form.precision != DPE_PRECISION_FP32 && 2 * prm->strong_radius / prm->strong_increment + 1 != 6)
         return fail(ctx, DPE_ERR_UNSUPPORTED, "the precision form needs the 11x11 step-2 window");
+    if (form.autotune < 0 || form.autotune > 1) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "autotune must be 0 or 1");
+    if (form.graph < 0 || form.graph > 1) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "graph must be 0 or 1");
     for (int r : form.reserved)
         if (r != 0) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "reserved form fields must be 0");
     if (form.gauss_newton_steps > 0 && 2 * prm->strong_radius / prm->strong_increment + 1 != 6)
@@ -499,10 +616,42 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
         d.gn_thr = (float)std::cos((angle / 2.0f) * M_PI / 180.0f);
         d.gn_shift = std::max((int)(std::tan((angle / 2.0f) * M_PI / 180.0f) * 20), 1);
     }
+    // tiles: the tuned ones of this problem class, or (while the tuner samples)
+    // each half-iteration's candidate and refinement launches in the next tile
+    dpe_tune *tune = form.autotune && d.ntap == 6 ? tune_record(ctx, d) : nullptr;
+    if (tune && tune->decided) {
+        tile_config(ctx, tune->full, d.full_tile);
+        tile_config(ctx, tune->wtd, d.wtd_tile);
+    }
+    std::vector<int> tiles;
+    for (int i = 0; i < kTiles; ++i) {
+        NccTile t;
+        if (tile_config(ctx, i, t)) tiles.push_back(i);
+    }
+    int half = 0;
     hipStream_t s = ctx->stream;
     HIPCHK(hipStreamSynchronize(s));
+    // graph form: the schedule is captured and launched as one graph (not while
+    // the tuner times single launches); its launches are not timed one by one
+    const bool use_graph = form.graph && !(tune && !tune->decided);
     // ---- RunPatchMatch schedule (DPE.cu:3157-3234)
     HIPCHK(hipEventRecord(ev_start, s));
+    // ends a capture that an error left open, so the stream stays usable
+    struct CaptureGuard {
+        hipStream_t s;
+        bool active = false;
+        ~CaptureGuard() {
+            if (!active) return;
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } capture{s};
+    if (use_graph) {
+        tm.record = false;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        capture.active = true;
+    }
     launch_gen_edge_inform(d, s);
     launch_find_nearest_strong(d, s);
     launch_gen_neighbours(d, s);
@@ -510,10 +659,18 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
     launch_random_init_planes(d, s);
     tm.ncc_launch(kFamFull, [&] { launch_ncc_current(d, s); });
     launch_random_init_costs(d, s);
+    const NccTile full_tile = d.full_tile, wtd_tile = d.wtd_tile;
     for (int it = 0; it < prm->max_iterations; ++it) {
         for (int c = 0; c < 2; ++c) {
+            int tile = -1;
+            if (tune && !tune->decided) {
+                tile = tiles[half % tiles.size()];
+                tile_config(ctx, tile, d.full_tile);
+                d.wtd_tile = d.full_tile;
+            }
+            half++;
             launch_strong_candidates(d, c, it, s);
-            tm.ncc_launch(kFamFull, [&] { launch_ncc_strong(d, c, s); });
+            tm.ncc_launch(kFamFull, [&] { launch_ncc_strong(d, c, s); }, tile);
             launch_strong_select(d, c, it, s);
             // Gauss-Newton form: step 1 (from the current plane) replaces the perturbed-normal
             // hypothesis (slot 3), step 2 (from step 1) the random depth + normal one (slot 2)
@@ -521,7 +678,7 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
                 tm.ncc_launch(kFamRefine, [&] { launch_gn_step(d, c, d.now_plane, d.rplane + 3 * Q, s); });
             if (form.gauss_newton_steps >= 2)
                 tm.ncc_launch(kFamRefine, [&] { launch_gn_step(d, c, d.rplane + 3 * Q, d.rplane + 2 * Q, s); });
-            tm.ncc_launch(kFamRefine, [&] { launch_ncc_refine(d, c, s); });
+            tm.ncc_launch(kFamRefine, [&] { launch_ncc_refine(d, c, s); }, tile);
             launch_strong_accept(d, c, s);
         }
         launch_ransac(d, it, s);
@@ -530,6 +687,8 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
                 if (d.weak_colour_count[c] > 0) tm.ncc_launch(kFamWeak, [&] { launch_weak_half(d, c, it, s); });
         }
     }
+    d.full_tile = full_tile;
+    d.wtd_tile = wtd_tile;
     launch_depth_and_normal(d, s);
     launch_filter(d, 0, s);
     launch_filter(d, 1, s);
@@ -538,6 +697,16 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
     tm.ncc_launch(kFamPost, [&] { launch_ncc_post(d, s); });
     launch_post_final(d, s);
     HIPCHK(hipGetLastError());
+    if (use_graph) {
+        hipGraph_t g = nullptr;
+        capture.active = false;
+        HIPCHK(hipStreamEndCapture(s, &g));
+        struct GraphGuard {
+            hipGraph_t g;
+            ~GraphGuard() { (void)hipGraphDestroy(g); }
+        } graph_guard{g};
+        launch_schedule_graph(ctx, g);
+    }
     hipEvent_t ev_stop = tm.get();
     HIPCHK(hipEventRecord(ev_stop, s));
     // ---- downloads (DPE.cu:3258-3260, plus the cost map)
@@ -553,6 +722,18 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
     HIPCHK(hipEventElapsedTime(&stt.gpu_ms, ev_start, ev_stop));
     tm.fill(stt, cnt);
     stt.quad_bytes = d.quad_fmt == QUAD_FP32 ? 16 : 8;
+    stt.tile_full = stt.tile_weighted = 0;
+    stt.graph = use_graph ? 1 : 0;
+    if (tune) {
+        if (tune->decided) {
+            stt.tile_full = tune->full;
+            stt.tile_weighted = tune->wtd;
+        } else {
+            tm.tune_samples(*tune);
+            tune_decide(ctx, *tune);
+            stt.tile_full = stt.tile_weighted = -1;
+        }
+    }
     ctx->stats = stt;
     return DPE_OK;
 }
@@ -788,6 +969,14 @@ dpe_status dpe_create(int device, dpe_ctx **out) {
         delete ctx;
         return DPE_ERR_HIP;
     }
+    if (hipDeviceGetAttribute(&ctx->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&ctx->lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) {
+        g_create_error = "cannot query the LDS size";
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return DPE_ERR_HIP;
+    }
     *out = ctx;
     return DPE_OK;
 }
@@ -797,6 +986,7 @@ void dpe_destroy(dpe_ctx *ctx) {
     // teardown: errors cannot be reported any more, so they are ignored
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     for (auto &b : ctx->bufs)
         if (b.ptr) (void)hipFree(b.ptr);
     for (auto e : ctx->events) (void)hipEventDestroy(e);
@@ -818,6 +1008,122 @@ dpe_status dpe_process_ex(dpe_ctx *ctx, const dpe_params *params, const dpe_form
     } catch (const Fail &f) {
         return f.st;
     }
+}
+
+// ---- batched form: independent problems over concurrent lanes
+struct dpe_batch {
+    int device = 0;
+    std::vector<dpe_ctx *> lanes;
+    std::string err;
+};
+
+dpe_status dpe_batch_create(int device, int lanes, dpe_batch **out) {
+    if (!out) return DPE_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (lanes < 1 || lanes > DPE_MAX_LANES) {
+        g_create_error = "lanes must be in [1, DPE_MAX_LANES]";
+        return DPE_ERR_INVALID_ARGUMENT;
+    }
+    dpe_batch *b = new dpe_batch();
+    b->device = device;
+    for (int i = 0; i < lanes; ++i) {
+        dpe_ctx *c = nullptr;
+        const dpe_status st = dpe_create(device, &c);
+        if (st != DPE_OK) {
+            dpe_batch_destroy(b);
+            return st;
+        }
+        b->lanes.push_back(c);
+    }
+    *out = b;
+    return DPE_OK;
+}
+
+void dpe_batch_destroy(dpe_batch *b) {
+    if (!b) return;
+    for (dpe_ctx *c : b->lanes) dpe_destroy(c);
+    delete b;
+}
+
+const char *dpe_batch_last_error(const dpe_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+dpe_status dpe_batch_process(dpe_batch *b, int n, const dpe_params *params, const dpe_form *form,
+                             const dpe_inputs *inputs, dpe_outputs *outputs, dpe_stats *stats, float *wall_ms) {
+    if (!b) return DPE_ERR_INVALID_ARGUMENT;
+    b->err.clear();
+    if (n < 0 || (n > 0 && (!params || !inputs || !outputs))) {
+        b->err = "bad batch arrays";
+        return DPE_ERR_INVALID_ARGUMENT;
+    }
+    // longest-processing-time-first assignment on the estimated NCC work
+    // (pixels x source views x (iterations + 1)) to the least loaded lane
+    std::vector<int> order(n);
+    std::vector<double> cost(n);
+    for (int i = 0; i < n; ++i) {
+        order[i] = i;
+        cost[i] = (double)inputs[i].width * inputs[i].height * std::max(inputs[i].num_images - 1, 1) *
+                  (std::max(params[i].max_iterations, 0) + 1);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+    const int L = (int)b->lanes.size();
+    std::vector<std::vector<int>> work(L);
+    std::vector<double> load(L, 0.0);
+    for (int i : order) {
+        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        work[l].push_back(i);
+        load[l] += cost[i];
+    }
+    // the lanes share what the autotuner has decided
+    for (dpe_ctx *c : b->lanes)
+        for (const dpe_tune &t : b->lanes[0]->tunes)
+            if (t.decided) {
+                bool have = false;
+                for (dpe_tune &u : c->tunes)
+                    if (u.W == t.W && u.H == t.H && u.NV == t.NV && u.use_edge == t.use_edge && u.prec == t.prec) {
+                        if (!u.decided) u = t;
+                        have = true;
+                    }
+                if (!have) c->tunes.push_back(t);
+            }
+    std::vector<dpe_status> status(n, DPE_OK);
+    std::vector<std::string> msg(n);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> threads;
+    for (int l = 0; l < L; ++l) {
+        if (work[l].empty()) continue;
+        threads.emplace_back([&, l] {
+            dpe_ctx *c = b->lanes[l];
+            for (int i : work[l]) {
+                status[i] = dpe_process_ex(c, &params[i], form, &inputs[i], &outputs[i]);
+                if (status[i] != DPE_OK) {
+                    msg[i] = c->err;
+                    break;
+                }
+                if (stats) stats[i] = c->stats;
+            }
+        });
+    }
+    for (std::thread &t : threads) t.join();
+    const auto t1 = std::chrono::steady_clock::now();
+    if (wall_ms) *wall_ms = (float)std::chrono::duration<double, std::milli>(t1 - t0).count();
+    // decided tuner records flow back to lane 0 for the next batch
+    for (int l = 1; l < L; ++l)
+        for (const dpe_tune &t : b->lanes[l]->tunes)
+            if (t.decided) {
+                bool have = false;
+                for (dpe_tune &u : b->lanes[0]->tunes)
+                    if (u.W == t.W && u.H == t.H && u.NV == t.NV && u.use_edge == t.use_edge && u.prec == t.prec) {
+                        if (!u.decided) u = t;
+                        have = true;
+                    }
+                if (!have) b->lanes[0]->tunes.push_back(t);
+            }
+    for (int i = 0; i < n; ++i)
+        if (status[i] != DPE_OK) {
+            b->err = "problem " + std::to_string(i) + ": " + msg[i];
+            return status[i];
+        }
+    return DPE_OK;
 }
 
 dpe_status dpe_last_stats(const dpe_ctx *ctx, dpe_stats *stats) {

@@ -395,9 +395,9 @@ __device__ float geom_cost(const Dev &d, int x, int y, int v, const float4 &pl) 
 // (pixel, slot): plane_src[cand_pos[slot*Qn+q]] if cand_pos is given (a -1
 // entry skips the slot), else plane_src[slot*Qn + q].  With view_mask (indexed
 // by pixel, Qn == P) only the views whose bit is set are evaluated.  One
-// workgroup = 64 pixels x 4 waves; wave w evaluates slots w, w+4, ...
-template <int NT, bool PREC>
-__global__ __launch_bounds__(kLanes * kWaves) void k_ncc_full(Dev d, int colour, int Qn, int nslot,
+// workgroup = 64 pixels x WAVES waves; wave w evaluates slots w, w+WAVES, ...
+template <int NT, bool PREC, int WAVES>
+__global__ __launch_bounds__(kLanes * WAVES) void k_ncc_full(Dev d, int colour, int Qn, int nslot,
                                                               const int *__restrict__ cand_pos,
                                                               const float4 *__restrict__ plane_src,
                                                               const uint32_t *__restrict__ view_mask,
@@ -414,9 +414,9 @@ __global__ __launch_bounds__(kLanes * kWaves) void k_ncc_full(Dev d, int colour,
     } else {
         any_slot = valid;
     }
-    fill_shared_patch<NT>(d, any_slot, x, y, S);
+    fill_shared_patch<NT, WAVES>(d, any_slot, x, y, S);
     unsigned long long hyps = 0, taps = 0;
-    for (int slot = wave; slot < nslot; slot += kWaves) {
+    for (int slot = wave; slot < nslot; slot += WAVES) {
         bool active = any_slot;
         float4 pl = make_float4(0, 0, 0, 0);
         if (active) {
@@ -534,21 +534,21 @@ __device__ __forceinline__ float weighted_sum(const Dev &d, const void *Sv, int 
     return acc;
 }
 
-// One workgroup = 64 pixels x 4 waves; wave w evaluates slot 4*blockIdx.y + w.
-template <int NT, int MODE, bool PREC>
-__global__ __launch_bounds__(kLanes * kWaves) void k_ncc_weighted(Dev d, int colour, int Qn, int nslot,
+// One workgroup = 64 pixels x WAVES waves; wave w evaluates slot WAVES*blockIdx.y + w.
+template <int NT, int MODE, bool PREC, int WAVES>
+__global__ __launch_bounds__(kLanes * WAVES) void k_ncc_weighted(Dev d, int colour, int Qn, int nslot,
                                                                   float *__restrict__ out) {
     __shared__ SharedPatch<NT> S;
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = threadIdx.x / kLanes;
     const int q = blockIdx.x * kLanes + lane;
-    const int slot = blockIdx.y * kWaves + wave;
+    const int slot = blockIdx.y * WAVES + wave;
     int x = 0, y = 0;
     const bool valid = q < Qn && colour_pixel(d, colour, q, x, y);
     const int p = x + y * d.W;
     bool any = false;
     if (valid) any = MODE == MODE_REFINE ? d.cand_pos[q] >= 0 : d.post_flags[p] != 0;
-    fill_shared_patch<NT>(d, any, x, y, S);
+    fill_shared_patch<NT, WAVES>(d, any, x, y, S);
     unsigned long long hyps = 0, taps = 0;
     float4 pl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float wn = 1.0f;
@@ -2178,17 +2178,56 @@ void launch_random_init_planes(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_random_init_planes, grid1(d.P), dim3(256), 0, s, d);
 }
 
-static inline dim3 grid_shared(int n, int nslot) { return dim3((n + kLanes - 1) / kLanes); }
-static inline dim3 grid_shared_slots(int n, int nslot) { return dim3((n + kLanes - 1) / kLanes, (nslot + kWaves - 1) / kWaves); }
 static inline dim3 grid_direct(int n, int nslot) { return dim3((n + 255) / 256, nslot); }
+
+// Shared-patch NCC launches with the call's tile (Dev::full_tile / wtd_tile:
+// waves per workgroup and extra dynamic LDS that caps workgroups per CU).
+// Every tile gives bit-identical results; only the schedule differs.
+template <bool PREC, int WAVES>
+static void full_t(const Dev &d, hipStream_t s, int colour, int Qn, int nslot, const int *cand, const float4 *src,
+                   const uint32_t *mask, float *out) {
+    hipLaunchKernelGGL((k_ncc_full<6, PREC, WAVES>), dim3((Qn + kLanes - 1) / kLanes), dim3(kLanes * WAVES),
+                       d.full_tile.lds_pad, s, d, colour, Qn, nslot, cand, src, mask, out);
+}
+static void launch_full(const Dev &d, hipStream_t s, int colour, int Qn, int nslot, const int *cand,
+                        const float4 *src, const uint32_t *mask, float *out) {
+    const int w = d.full_tile.waves;
+    if (d.prec) {
+        if (w == 2) full_t<true, 2>(d, s, colour, Qn, nslot, cand, src, mask, out);
+        else if (w == 8) full_t<true, 8>(d, s, colour, Qn, nslot, cand, src, mask, out);
+        else full_t<true, 4>(d, s, colour, Qn, nslot, cand, src, mask, out);
+    } else {
+        if (w == 2) full_t<false, 2>(d, s, colour, Qn, nslot, cand, src, mask, out);
+        else if (w == 8) full_t<false, 8>(d, s, colour, Qn, nslot, cand, src, mask, out);
+        else full_t<false, 4>(d, s, colour, Qn, nslot, cand, src, mask, out);
+    }
+}
+template <int MODE, bool PREC, int WAVES>
+static void wtd_t(const Dev &d, hipStream_t s, int colour, int Qn, int nslot, float *out) {
+    hipLaunchKernelGGL((k_ncc_weighted<6, MODE, PREC, WAVES>),
+                       dim3((Qn + kLanes - 1) / kLanes, (nslot + WAVES - 1) / WAVES), dim3(kLanes * WAVES),
+                       d.wtd_tile.lds_pad, s, d, colour, Qn, nslot, out);
+}
+template <int MODE>
+static void launch_wtd(const Dev &d, hipStream_t s, int colour, int Qn, int nslot, float *out) {
+    const int w = d.wtd_tile.waves;
+    if (d.prec) {
+        if (w == 2) wtd_t<MODE, true, 2>(d, s, colour, Qn, nslot, out);
+        else if (w == 8) wtd_t<MODE, true, 8>(d, s, colour, Qn, nslot, out);
+        else wtd_t<MODE, true, 4>(d, s, colour, Qn, nslot, out);
+    } else {
+        if (w == 2) wtd_t<MODE, false, 2>(d, s, colour, Qn, nslot, out);
+        else if (w == 8) wtd_t<MODE, false, 8>(d, s, colour, Qn, nslot, out);
+        else wtd_t<MODE, false, 4>(d, s, colour, Qn, nslot, out);
+    }
+}
 
 void launch_ncc_current(const Dev &d, hipStream_t s) {
     // FIRST_INIT needs every view (top-k selection); the REFINE states only the
     // prior-selected ones (ComputeMultiViewInitialCost, DPE.cu:832-861)
     const uint32_t *mask = d.prm.state == DPE_FIRST_INIT ? nullptr : d.selected;
     if (d.ntap == 6)
-        hipLaunchKernelGGL((d.prec ? k_ncc_full<6, true> : k_ncc_full<6, false>), grid_shared(d.P, 1),
-                           dim3(kLanes * kWaves), 0, s, d, -1, d.P, 1, nullptr, d.planes, mask, d.cost_mat);
+        launch_full(d, s, -1, d.P, 1, nullptr, d.planes, mask, d.cost_mat);
     else
         hipLaunchKernelGGL(k_ncc_full_direct, grid_direct(d.P, 1), dim3(256), 0, s, d, -1, d.P, nullptr, d.planes,
                            mask, d.cost_mat);
@@ -2202,9 +2241,7 @@ void launch_strong_candidates(const Dev &d, int colour, int iter, hipStream_t s)
 void launch_ncc_strong(const Dev &d, int colour, hipStream_t s) {
     const int nslot = d.prm.use_edge ? kStrongSlots : 9;
     if (d.ntap == 6)
-        hipLaunchKernelGGL((d.prec ? k_ncc_full<6, true> : k_ncc_full<6, false>), grid_shared(d.Q, nslot),
-                           dim3(kLanes * kWaves), 0, s, d, colour, d.Q, nslot, d.cand_pos, d.planes,
-                           (const uint32_t *)nullptr, d.cost_mat);
+        launch_full(d, s, colour, d.Q, nslot, d.cand_pos, d.planes, nullptr, d.cost_mat);
     else
         hipLaunchKernelGGL(k_ncc_full_direct, grid_direct(d.Q, nslot), dim3(256), 0, s, d, colour, d.Q, d.cand_pos,
                            d.planes, (const uint32_t *)nullptr, d.cost_mat);
@@ -2214,9 +2251,7 @@ void launch_strong_select(const Dev &d, int colour, int iter, hipStream_t s) {
 }
 void launch_ncc_refine(const Dev &d, int colour, hipStream_t s) {
     if (d.ntap == 6)
-        hipLaunchKernelGGL((d.prec ? k_ncc_weighted<6, MODE_REFINE, true> : k_ncc_weighted<6, MODE_REFINE, false>),
-                           grid_shared_slots(d.Q, kRefineSlots), dim3(kLanes * kWaves), 0, s, d, colour, d.Q,
-                           kRefineSlots, d.rcost);
+        launch_wtd<MODE_REFINE>(d, s, colour, d.Q, kRefineSlots, d.rcost);
     else
         hipLaunchKernelGGL(k_ncc_weighted_direct<MODE_REFINE>, grid_direct(d.Q, kRefineSlots), dim3(256), 0, s, d,
                            colour, d.Q, d.rcost);
@@ -2248,9 +2283,7 @@ void launch_post_planes(const Dev &d, hipStream_t s) {
 }
 void launch_ncc_post(const Dev &d, hipStream_t s) {
     if (d.ntap == 6)
-        hipLaunchKernelGGL((d.prec ? k_ncc_weighted<6, MODE_POST, true> : k_ncc_weighted<6, MODE_POST, false>),
-                           grid_shared_slots(d.P, kPostSlots), dim3(kLanes * kWaves), 0, s, d, -1, d.P, kPostSlots,
-                           d.pcost);
+        launch_wtd<MODE_POST>(d, s, -1, d.P, kPostSlots, d.pcost);
     else
         hipLaunchKernelGGL(k_ncc_weighted_direct<MODE_POST>, grid_direct(d.P, kPostSlots), dim3(256), 0, s, d, -1, d.P,
                            d.pcost);
@@ -2260,9 +2293,7 @@ void launch_post_final(const Dev &d, hipStream_t s) {
 }
 void launch_ncc_given(const Dev &d, int num_hyps, const float4 *planes, float *costs, hipStream_t s) {
     if (d.ntap == 6)
-        hipLaunchKernelGGL((d.prec ? k_ncc_full<6, true> : k_ncc_full<6, false>), grid_shared(d.P, num_hyps),
-                           dim3(kLanes * kWaves), 0, s, d, -1, d.P, num_hyps, nullptr, planes,
-                           (const uint32_t *)nullptr, costs);
+        launch_full(d, s, -1, d.P, num_hyps, nullptr, planes, nullptr, costs);
     else
         hipLaunchKernelGGL(k_ncc_full_direct, grid_direct(d.P, num_hyps), dim3(256), 0, s, d, -1, d.P, nullptr, planes,
                            (const uint32_t *)nullptr, costs);

@@ -97,14 +97,15 @@ struct SharedPatch {
     float sum_refc[kLanes], sum_refc2[kLanes];
 };
 
-template <int NT>
+// WAVES = waves of the workgroup (the tile's slot width); they fill the taps together.
+template <int NT, int WAVES = kWaves>
 __device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int x, int y, SharedPatch<NT> &S) {
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = threadIdx.x / kLanes;
     const float sc = d.prm.sigma_color;
     const float cden = 2.0f * sc * sc;
     const float rc = valid ? texel(d.ref, d.W, d.H, x, y) : 0.0f;
-    for (int k = wave; k < NT * NT; k += kWaves) {
+    for (int k = wave; k < NT * NT; k += WAVES) {
         const int ki = k / NT, kj = k - ki * NT;
         float2 e = make_float2(0.0f, 0.0f);
         if (valid) {

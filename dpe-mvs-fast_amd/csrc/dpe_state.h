@@ -22,6 +22,15 @@ constexpr int kRefineSlots = 5;
 constexpr int kPostSlots = 73;
 constexpr int kWeakPatchTaps = 36 + 8 * 9;  // centre window + 8 neighbour windows of ComputeBilateralNCCNew    // 0..60 DepthToWeak samples, 61 LocalRefine now, 62..72 LocalRefine samples
 
+// Tile of a shared-patch NCC launch: 64 pixels x `waves` waves per
+// workgroup, plus `lds_pad` bytes of dynamic LDS per workgroup that caps the
+// workgroups resident per CU (the waves-per-CU knob).  Chosen per problem
+// class by the autotuner; the default is 4 waves and no pad.
+struct NccTile {
+    int waves = 4;
+    int lds_pad = 0;
+};
+
 struct Dev {
     int W, H, N, NV, P;
     int half_rows, Wh, Q;   // colour indexing
@@ -86,6 +95,9 @@ struct Dev {
     float gn_cos, gn_sin, gn_thr;
     int gn_shift;
     unsigned long long *counters;  // [2*family] NCC hyps, [2*family+1] taps; family = NccFamily
+    // tiles of the shared-patch NCC launches: k_ncc_full (candidates, initial
+    // costs) and k_ncc_weighted (refinement, post-pass)
+    NccTile full_tile, wtd_tile;
 };
 
 // Source quad formats (Dev::quad_fmt): fp32 float4 per bilinear cell, or two

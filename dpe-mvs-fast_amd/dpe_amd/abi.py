@@ -52,21 +52,23 @@ class Outputs(C.Structure):
 
 NCC_FAMILIES = 4
 FAMILY_FULL, FAMILY_REFINE, FAMILY_POST, FAMILY_WEAK = range(NCC_FAMILIES)
-ABI_VERSION = 6
+ABI_VERSION = 8
 
 
 class Stats(C.Structure):
     _fields_ = [("gpu_ms", C.c_float), ("ncc_ms", C.c_float), ("ncc_hyps", C.c_int64), ("ncc_taps", C.c_int64),
                 ("ncc_launches", C.c_int32), ("family_ms", C.c_float * NCC_FAMILIES),
                 ("family_hyps", C.c_int64 * NCC_FAMILIES), ("family_taps", C.c_int64 * NCC_FAMILIES),
-                ("family_launches", C.c_int32 * NCC_FAMILIES), ("quad_bytes", C.c_int32)]
+                ("family_launches", C.c_int32 * NCC_FAMILIES), ("quad_bytes", C.c_int32),
+                ("tile_full", C.c_int32), ("tile_weighted", C.c_int32), ("graph", C.c_int32)]
 
 
 PRECISION_FP32, PRECISION_FP16, PRECISION_BF16 = 0, 1, 2
 
 
 class Form(C.Structure):
-    _fields_ = [("gauss_newton_steps", C.c_int32), ("precision", C.c_int32), ("reserved", C.c_int32 * 6)]
+    _fields_ = [("gauss_newton_steps", C.c_int32), ("precision", C.c_int32), ("autotune", C.c_int32),
+                ("graph", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class FuseView(C.Structure):
@@ -158,9 +160,20 @@ def declare(lib):
     lib.dpe_fuse.argtypes = [C.c_void_p, P(FuseView), C.c_int, P(FuseParams), C.c_void_p, C.c_int64,
                              P(C.c_int64)]
     lib.dpe_fuse.restype = C.c_int
+    lib.dpe_batch_create.argtypes = [C.c_int, C.c_int, P(C.c_void_p)]
+    lib.dpe_batch_create.restype = C.c_int
+    lib.dpe_batch_destroy.argtypes = [C.c_void_p]
+    lib.dpe_batch_destroy.restype = None
+    lib.dpe_batch_last_error.argtypes = [C.c_void_p]
+    lib.dpe_batch_last_error.restype = C.c_char_p
+    lib.dpe_batch_process.argtypes = [C.c_void_p, C.c_int, P(Params), C.c_void_p, P(Inputs), P(Outputs),
+                                      C.c_void_p, P(C.c_float)]
+    lib.dpe_batch_process.restype = C.c_int
     return lib
 
 
 # Every symbol include/dpe_hip.h declares.
 EXPORTS = ["dpe_create", "dpe_destroy", "dpe_last_error", "dpe_abi_version", "dpe_process", "dpe_process_ex",
-           "dpe_last_stats", "dpe_ncc_cost", "dpe_ncc_cost_ex", "dpe_ncc_cost_adjoint", "dpe_fuse"]
+           "dpe_last_stats", "dpe_ncc_cost", "dpe_ncc_cost_ex", "dpe_ncc_cost_adjoint", "dpe_fuse",
+           "dpe_batch_create", "dpe_batch_destroy", "dpe_batch_last_error", "dpe_batch_process"]
+MAX_LANES = 16

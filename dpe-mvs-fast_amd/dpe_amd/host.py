@@ -72,6 +72,45 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p).value if a is not None else None
 
 
+class Batch:
+    """A dpe_batch: `lanes` contexts on one GPU processing independent problems
+    concurrently (the batched form)."""
+
+    def __init__(self, device=0, lanes=4, lib=None):
+        self.lib = lib or load_library()
+        h = C.c_void_p()
+        st = self.lib.dpe_batch_create(device, lanes, C.byref(h))
+        if st != abi.OK:
+            raise DPEError(st, self.lib.dpe_batch_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.dpe_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, pins, form=None):
+        """Process the ProblemInputs `pins`; returns (outputs, stats, wall ms)."""
+        n = len(pins)
+        params = (abi.Params * n)(*[p.params for p in pins])
+        inputs = (abi.Inputs * n)(*[p.struct for p in pins])
+        outs = [ProblemOutputs(p.width, p.height) for p in pins]
+        outputs = (abi.Outputs * n)(*[o.struct for o in outs])
+        stats = (abi.Stats * n)()
+        wall = C.c_float()
+        st = self.lib.dpe_batch_process(self.h, n, params, C.byref(form) if form is not None else None, inputs,
+                                        outputs, stats, C.byref(wall))
+        if st != abi.OK:
+            raise DPEError(st, self.lib.dpe_batch_last_error(self.h).decode())
+        return outs, list(stats), wall.value
+
+
 class ProblemInputs:
     """Host arrays of one process call plus the dpe_inputs struct pointing at them."""
 

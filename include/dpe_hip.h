@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DPE_ABI_VERSION 6
+#define DPE_ABI_VERSION 8
 /* main.h:41-44 */
 #define DPE_MAX_IMAGES 32
 #define DPE_NEIGHBOUR_NUM 9
@@ -141,6 +141,11 @@ typedef struct dpe_stats {
     int64_t family_taps[DPE_NCC_FAMILIES];
     int32_t family_launches[DPE_NCC_FAMILIES];
     int32_t quad_bytes;       /* bytes of one bilinear source quad (one per tap): 16 fp32, 8 fp16 / bf16 */
+    int32_t tile_full;        /* tile of the candidate-cost and refinement/post-pass NCC launches: */
+    int32_t tile_weighted;    /* 0 = 4 waves per workgroup (default), 1 = 8 waves, 2 = 2 waves,
+                                 3 = 4 waves with <= 3 workgroups per CU, 4 = 4 waves with <= 2;
+                                 -1 while the autotuner samples (dpe_form.autotune) */
+    int32_t graph;            /* 1: the schedule ran as one captured hipGraph (dpe_form.graph) */
 } dpe_stats;
 
 typedef struct dpe_ctx dpe_ctx;
@@ -181,7 +186,19 @@ typedef struct dpe_form {
                                    (reference texel, source sample at its projection), accumulated
                                    in fp32.  The weak-texture windows read the same 16-bit quads with
                                    the plain sums. */
-    int32_t reserved[6];        /* 0 */
+    int32_t autotune;           /* 1: the shared-patch NCC launches use the tile (waves per workgroup,
+                                   workgroups per CU) the context's autotuner chose for this problem
+                                   class (image size, source count, window slots, precision).  The
+                                   first calls of a class time every tile on their half-iteration
+                                   launches, one tile per half-iteration, then the fastest is kept.
+                                   Results are bit-identical for every tile. */
+    int32_t graph;              /* 1: the call's launch schedule is captured into a hipGraph on the
+                                   context's stream and launched as one graph; the context keeps
+                                   the executable graph and updates it in place when the next
+                                   call's schedule has the same shape.  Per-family launch times
+                                   (dpe_stats.family_ms) are not taken in this form.  Calls in
+                                   which the autotuner still samples run without the graph. */
+    int32_t reserved[4];        /* 0 */
 } dpe_form;
 
 /* dpe_process with a form; form == NULL or all zero is dpe_process. */
@@ -250,6 +267,26 @@ typedef struct dpe_point {
 dpe_status dpe_fuse(dpe_ctx *ctx, const dpe_fuse_view *views, int num_views,
                     const dpe_fuse_params *params, dpe_point *points, int64_t capacity,
                     int64_t *num_points);
+
+/* Batched form (no counterpart in the reference, whose main.cpp processes one
+ * problem at a time): many independent problems of uneven image size and
+ * source count in one call.  A batch owns `lanes` contexts on one GPU, each
+ * with its own stream and buffer pool, driven by one host thread each; the
+ * problems go to the lanes longest-first (estimated NCC work), so small
+ * problems run concurrently and fill the GPU.  Each problem is one
+ * dpe_process_ex call with `form` (typically autotune = 1, graph = 1); the
+ * lanes share the autotuner's decisions between batches.  Results equal the
+ * problems' single calls bit for bit. */
+#define DPE_MAX_LANES 16
+typedef struct dpe_batch dpe_batch;
+dpe_status dpe_batch_create(int device, int lanes, dpe_batch **out);
+void dpe_batch_destroy(dpe_batch *batch);
+const char *dpe_batch_last_error(const dpe_batch *batch);
+/* params, inputs, outputs: n each; stats: n or NULL; wall_ms: host wall time
+ * of the whole batch (all lanes synchronised), or NULL. */
+dpe_status dpe_batch_process(dpe_batch *batch, int n, const dpe_params *params, const dpe_form *form,
+                             const dpe_inputs *inputs, dpe_outputs *outputs, dpe_stats *stats,
+                             float *wall_ms);
 
 #ifdef __cplusplus
 }

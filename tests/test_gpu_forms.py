@@ -160,3 +160,107 @@ def test_precision_form_depth_maps():
             assert res[name][0] >= res["plain"][0] - 0.01 and res[name][1] >= res["plain"][1] - 0.01, name
     finally:
         ctx.close()
+
+
+def _same(a, b):
+    """Bit-identical outputs (NaN payloads included)."""
+    return all(np.array_equal(getattr(a, n).view(np.uint8), getattr(b, n).view(np.uint8))
+               for n in ("planes", "costs", "pixel_states", "selected_views"))
+
+
+def test_autotuned_tiles_are_bit_identical():
+    """Autotuner (dpe_form.autotune = 1): the first calls of a problem class
+    time the five tiles of the shared-patch NCC launches (one tile per
+    half-iteration) and report tile -1; once every tile has two launches of
+    each kernel timed, the fastest are kept and reported.  Every call's planes,
+    costs, states and selected views equal the plain call's bit for bit."""
+    s = scene.config1()
+    ref = s.meta["ref_mid"]
+    ctx = host.Context(0)
+    try:
+        pin = host.first_call_inputs(s, ref, s.pairs[ref], max_iterations=3)
+        plain = host.run_hip(ctx, pin)
+        seen = []
+        for call in range(4):
+            out = host.run_hip(ctx, pin, abi.Form(autotune=1))
+            st = ctx.stats()
+            seen.append((st.tile_full, st.tile_weighted))
+            assert _same(out, plain), call
+        print(f"tiles per call (full, weighted): {seen}")
+        assert seen[0] == (-1, -1)
+        assert all(0 <= t <= 4 for t in seen[-1])
+        assert seen[-1] == seen[-2]  # decided once, then kept
+    finally:
+        ctx.close()
+
+
+
+
+def test_graph_form_replays_are_bit_identical():
+    """Graph form (dpe_form.graph = 1): the schedule is captured and launched as
+    one hipGraph; the context updates its executable graph in place for the
+    next call of the same shape.  Graph calls alternate with eager calls and
+    with a problem of another size (which rebuilds the graph); every graph
+    call equals the eager call of the same problem bit for bit."""
+    c1 = scene.config1()
+    small = scene.make_scene(160, 120, 150.0, 4, 20.0, seed=0x5EED0031)
+    ref = c1.meta["ref_mid"]
+    store = host.StateStore()
+    ctx = host.Context(0)
+    try:
+        for i in range(len(c1.views)):
+            pin = host.first_call_inputs(c1, i, c1.pairs[i], max_iterations=3, call_index=0)
+            store.put(i, host.assemble(host.run_hip(ctx, pin), pin.params.depth_min, pin.params.depth_max))
+        problems = [host.first_call_inputs(c1, ref, c1.pairs[ref], max_iterations=3),
+                    host.refine_call_inputs(c1, ref, c1.pairs[ref], store, max_iterations=2, call_index=1),
+                    host.first_call_inputs(small, 1, small.pairs[1], max_iterations=2)]
+        eager = [host.run_hip(ctx, p) for p in problems]
+        graph = abi.Form(graph=1)
+        for rep in range(3):
+            for k, p in enumerate(problems):
+                out = host.run_hip(ctx, p, graph)
+                assert ctx.stats().graph == 1
+                assert _same(out, eager[k]), (rep, k)
+                # eager work between the replays
+                assert _same(host.run_hip(ctx, p), eager[k])
+    finally:
+        ctx.close()
+
+
+def test_batched_form_is_bit_identical_to_single_calls():
+    """Batched form (dpe_batch_process): 9 problems of three image sizes and 3-6
+    source views (FIRST_INIT) plus a REFINE_ITER call with the weak path, on 3
+    lanes with the autotuner and the graph form; run twice (the first batch
+    samples tiles, the second uses the decided tiles and graphs).  Every output
+    equals the problem's single plain call bit for bit."""
+    c1 = scene.config1()
+    scenes = [scene.make_scene(160, 120, 150.0, 7, 25.0, seed=0x5EED0041),
+              scene.make_scene(224, 168, 210.0, 7, 25.0, seed=0x5EED0042), c1]
+    ctx = host.Context(0)
+    try:
+        pins = []
+        for si, sc in enumerate(scenes):
+            for k, ref in enumerate((1, 2, 3)):
+                nsrc = 3 + (si + k) % 4
+                pins.append(host.first_call_inputs(sc, ref, sc.pairs[ref][:nsrc], max_iterations=2))
+        store = host.StateStore()
+        for i in range(len(c1.views)):
+            pin = host.first_call_inputs(c1, i, c1.pairs[i], max_iterations=2, call_index=0)
+            store.put(i, host.assemble(host.run_hip(ctx, pin), pin.params.depth_min, pin.params.depth_max))
+        ref = c1.meta["ref_mid"]
+        pins.append(host.refine_call_inputs(c1, ref, c1.pairs[ref], store, max_iterations=2, call_index=1))
+        single = [host.run_hip(ctx, p) for p in pins]
+    finally:
+        ctx.close()
+    batch = host.Batch(0, lanes=3)
+    try:
+        form = abi.Form(autotune=1, graph=1)
+        for rep in range(2):
+            outs, stats, wall = batch.run(pins, form)
+            print(f"batch {rep}: {len(pins)} problems in {wall:.1f} ms; tiles "
+                  f"{[(s.tile_full, s.tile_weighted) for s in stats]}; graph {[s.graph for s in stats]}")
+            for k, (o, ref_out) in enumerate(zip(outs, single)):
+                assert _same(o, ref_out), (rep, k)
+            assert all(s.ncc_hyps > 0 for s in stats)
+    finally:
+        batch.close()
