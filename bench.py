@@ -32,9 +32,9 @@ FAMILY_NAMES = ["k_ncc_full<6>", "k_ncc_weighted<6,REFINE>", "k_ncc_weighted<6,P
 # --pmc passes (FETCH_SIZE, WRITE_SIZE) over `python bench.py --no-cpu-baseline`,
 # each scaled by the counter's reading for a 256 MiB device copy
 # (`python bench.py --copy-calibration`: FETCH_SIZE x2.000, WRITE_SIZE x1.000);
-# the runs and the arithmetic are in profiles/ncc_kernels_c2_hbm_traffic.md.
+# the runs and the arithmetic are in profiles/ncc_kernels_1600x1200_20views_hbm_traffic.md.
 # Counters cannot be read inside a timed run.
-NCC_FULL_TRAFFIC_BYTES = 3.1004e10
+NCC_FULL_TRAFFIC_BYTES = 8.6173e9
 
 
 def parse():

@@ -2341,5 +2341,43 @@ void oracle_homography(const dpe_camera *ref, const dpe_camera *src, const float
 int oracle_point_in_triangle(const int16_t *A, const int16_t *B, const int16_t *Cc, int px, int py) {
     return PointinTriangle(S2{A[0], A[1]}, S2{B[0], B[1]}, S2{Cc[0], Cc[1]}, px, py) ? 1 : 0;
 }
+void oracle_sort_small(float *d, int n) { sort_small(d, n); }
+void oracle_sort_small_weighted(int16_t *xy, float *w, int n) {
+    std::vector<S2> pts(n);
+    for (int i = 0; i < n; ++i) pts[i] = S2{xy[2 * i], xy[2 * i + 1]};
+    sort_small_weighted(pts.data(), w, n);
+    for (int i = 0; i < n; ++i) {
+        xy[2 * i] = pts[i].x;
+        xy[2 * i + 1] = pts[i].y;
+    }
+}
+int oracle_find_min_cost_index(const float *costs, int n) { return FindMinCostIndex(costs, n); }
+void oracle_pdf_to_cdf(float *probs, int n) { TransformPDFToCDF(probs, n); }
+void oracle_nearest_strong(const uint8_t *weak, int W, int H, int16_t *out) {
+    Ctx C;
+    C.W = W;
+    C.H = H;
+    C.P = W * H;
+    C.weak.assign(weak, weak + (size_t)W * H);
+    C.nearest.assign((size_t)W * H, S2{-1, -1});
+    for (int i = 0; i < W * H; ++i) FindNearestStrongPoint(C, i);
+    for (int i = 0; i < W * H; ++i) {
+        out[2 * i] = C.nearest[i].x;
+        out[2 * i + 1] = C.nearest[i].y;
+    }
+}
+float oracle_filter_strong(const float *depth, const float *costs, const uint8_t *weak, int W, int H, int px,
+                           int py) {
+    Ctx C;
+    C.W = W;
+    C.H = H;
+    C.P = W * H;
+    C.planes.assign((size_t)W * H, F4{0, 0, 0, 0});
+    for (int i = 0; i < W * H; ++i) C.planes[i].w = depth[i];
+    C.costs.assign(costs, costs + (size_t)W * H);
+    C.weak.assign(weak, weak + (size_t)W * H);
+    CheckerboardFilterStrong(C, px, py);
+    return C.planes[(size_t)py * W + px].w;
+}
 
 }  // extern "C"

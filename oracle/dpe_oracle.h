@@ -27,6 +27,16 @@ void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]
 float oracle_bilinear(const float *im, int W, int H, float u, float v);
 void oracle_homography(const dpe_camera *ref, const dpe_camera *src, const float *plane, float *H);
 int oracle_point_in_triangle(const int16_t *A, const int16_t *B, const int16_t *C, int px, int py);
+/* sort_small, sort_small_weighted, FindMinCostIndex, TransformPDFToCDF (DPE.cu:9-61, 297-311) */
+void oracle_sort_small(float *d, int n);
+void oracle_sort_small_weighted(int16_t *xy, float *w, int n);
+int oracle_find_min_cost_index(const float *costs, int n);
+void oracle_pdf_to_cdf(float *probs, int n);
+/* FindNearestStrongPoint (DPE.cu:2859-2893) over a pixel-state map; out: (x, y) per pixel */
+void oracle_nearest_strong(const uint8_t *weak, int W, int H, int16_t *out);
+/* CheckerboardFilterStrong (DPE.cu:1961-2071) of one pixel; returns its filtered depth */
+float oracle_filter_strong(const float *depth, const float *costs, const uint8_t *weak, int W, int H, int px,
+                           int py);
 #ifdef __cplusplus
 }
 #endif

@@ -49,3 +49,36 @@ def fuse_hip(ctx, fs, fp):
                                C.byref(cnt)))
     return np.frombuffer(pts, dtype=np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("b", "u1"), ("g", "u1"),
                                               ("r", "u1"), ("pad", "u1")]))[:cnt.value]
+
+
+class _Views:
+    pass
+
+
+def coincident_pair(depth1_scale_rows=None, W=40, H=30):
+    """Two views with the same camera looking at the plane z = 4 (all pixels
+    strong, normals (0, 0, -1)): every pixel of view 0 reprojects onto itself
+    in view 1.  depth1_scale_rows: (row slice, factor) applied to view 1's
+    depth.  Returns (views holder for fuse_oracle / fuse_hip, K)."""
+    K = np.array([[50.0, 0, (W - 1) / 2], [0, 50.0, (H - 1) / 2], [0, 0, 1]])
+    cam = abi.camera_from_arrays(K, np.eye(3), np.zeros(3), W, H, 2, 6)
+    fs = _Views()
+    fs.keep = []
+    fs.views = (abi.FuseView * 2)()
+    fs.n = 2
+    for i in range(2):
+        d = np.full((H, W), 4.0, np.float32)
+        if i == 1 and depth1_scale_rows is not None:
+            rows, f = depth1_scale_rows
+            d[rows] *= np.float32(f)
+        nrm = np.zeros((H, W, 3), np.float32)
+        nrm[..., 2] = -1.0
+        st = np.full((H, W), abi.STRONG, np.uint8)
+        col = np.full((H, W, 3), 100 + i, np.uint8)
+        fs.keep += [d, nrm, st, col]
+        v = fs.views[i]
+        v.width, v.height, v.camera = W, H, cam
+        v.depth, v.normal, v.pixel_states, v.color = d.ctypes.data, nrm.ctypes.data, st.ctypes.data, col.ctypes.data
+        v.num_src = 1
+        v.src_index[0] = 1 - i
+    return fs, K

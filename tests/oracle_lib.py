@@ -44,6 +44,18 @@ def load_oracle():
     lib.oracle_homography.restype = None
     lib.oracle_point_in_triangle.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
     lib.oracle_point_in_triangle.restype = C.c_int
+    lib.oracle_sort_small.argtypes = [C.c_void_p, C.c_int]
+    lib.oracle_sort_small.restype = None
+    lib.oracle_sort_small_weighted.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+    lib.oracle_sort_small_weighted.restype = None
+    lib.oracle_find_min_cost_index.argtypes = [C.c_void_p, C.c_int]
+    lib.oracle_find_min_cost_index.restype = C.c_int
+    lib.oracle_pdf_to_cdf.argtypes = [C.c_void_p, C.c_int]
+    lib.oracle_pdf_to_cdf.restype = None
+    lib.oracle_nearest_strong.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    lib.oracle_nearest_strong.restype = None
+    lib.oracle_filter_strong.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.oracle_filter_strong.restype = C.c_float
     return lib
 
 

@@ -178,3 +178,17 @@ def test_fusion_matches_oracle(ctx, orc, c1):
     # config 2's rule: at least 3 consistent views
     fp3 = abi.fuse_defaults(min_consistent=3)
     assert abs(len(fuse_hip(ctx, fs, fp3)) - len(fuse_oracle(orc, fs, fp3))) <= 0.005 * len(po)
+
+
+def test_fusion_closed_form_on_gpu(ctx):
+    """The GPU fusion on two coincident views of a plane: one point per pixel of
+    view 0, none from view 1; a 5 %-off band of view-1 rows drops exactly those
+    rows; min_consistent 2 keeps nothing (the closed form the oracle also meets)."""
+    from fuse_util import coincident_pair, fuse_hip
+    W, H = 40, 30
+    fs, _ = coincident_pair(None, W, H)
+    assert len(fuse_hip(ctx, fs, abi.fuse_defaults())) == W * H
+    fs, _ = coincident_pair((slice(10, 17), 1.05), W, H)
+    assert len(fuse_hip(ctx, fs, abi.fuse_defaults())) == W * (H - 7)
+    fs, _ = coincident_pair(None, W, H)
+    assert len(fuse_hip(ctx, fs, abi.fuse_defaults(min_consistent=2))) == 0
