@@ -86,6 +86,40 @@ __device__ __forceinline__ float sample_src_any(const Dev &d, int v, float u, fl
     return sample_src<QUAD_FP32>(d, v, u, w);
 }
 
+// Clamp-free bilinear gather (Dev::fast_gather, a window inside the source):
+// the cell offset comes from the floored floats (exact below 2^24), and the
+// texel differences t.y - t.x, t.w - t.z are taken in packed fp16, exact on the
+// quarter grid, so the sample equals sample_src's bit for bit.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float sample_fast(const char *__restrict__ cell0, float row_cells, float u, float w) {
+    const float fx = floorf(u), fy = floorf(w);
+    const float a = u - fx, b = w - fy;
+    const uint32_t cell = (uint32_t)fmaf(fy, row_cells, fx);
+    const uint2 h = *(const uint2 *)(cell0 + cell * 8u);
+    const half2_t p = __builtin_bit_cast(half2_t, h.x);  // (t.x, t.y)
+    const half2_t q = __builtin_bit_cast(half2_t, h.y);  // (t.z, t.w)
+    const half2_t lo = {p.x, q.x}, hi = {p.y, q.y};
+    const half2_t dd = hi - lo;
+    const float top = fmaf(a, (float)dd.x, (float)p.x);
+    const float bot = fmaf(a, (float)dd.y, (float)q.x);
+    return fmaf(b, bot - top, top);
+}
+// Every tap of the window [x0, x1] x [y0, y1] projects to [1/64, W-1-1/64] x
+// [1/64, H-1-1/64]: the projection is linear-fractional with w of one sign
+// (refined_division_ok), so its extremes over the window are at the corner
+// taps, and the margin covers rounding.  No tap then needs the clamp.
+__device__ __forceinline__ bool window_inside(const Dev &d, const float H[9], int x0, int x1, int y0, int y1) {
+    const float m = 1.0f / 64.0f;
+    const float ux = (float)(d.W - 1) - m, uy = (float)(d.H - 1) - m;
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float2 p = project(H, (c & 1) ? x1 : x0, (c & 2) ? y1 : y0);
+        ok = ok && p.x >= m && p.x <= ux && p.y >= m && p.y <= uy;
+    }
+    return ok;
+}
+
 constexpr int kLanes = 64;
 constexpr int kWaves = 4;
 
@@ -151,9 +185,12 @@ __device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int 
 // Source sums of one window (DPE.cu:728-758) with the reference weights from
 // LDS; REFINED: the projection uses the refined division (exact, see
 // refined_division_ok).
-template <int NT, bool REFINED, int FMT>
+template <int NT, bool REFINED, int FMT, bool FAST = false>
 __device__ __forceinline__ void shared_window_sums(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y,
                                                    int v, const float H[9], float &s_s, float &s_ss, float &s_rs) {
+    // FAST: cell (0, 0) of the fp16 quads and the cells per row
+    const char *cell0 = (const char *)(d.quads_h + (size_t)v * d.qs) + (size_t)(d.W + 2) * 8;
+    const float row_cells = (float)(d.W + 1);
 #pragma unroll 1
     for (int ki = 0; ki < NT; ++ki) {
         float r_s = 0.0f, r_ss = 0.0f, r_rs = 0.0f;
@@ -162,7 +199,7 @@ __device__ __forceinline__ void shared_window_sums(const Dev &d, const SharedPat
         for (int kj = 0; kj < NT; ++kj) {
             const int ry = y + d.tap_off0 + kj * d.tap_inc;
             const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
-            const float src = sample_src<FMT>(d, v, sp.x, sp.y);
+            const float src = FAST ? sample_fast(cell0, row_cells, sp.x, sp.y) : sample_src<FMT>(d, v, sp.x, sp.y);
             const float2 e = S.wwr[ki * NT + kj][lane];
             r_s = fmaf(e.x, src, r_s);
             r_ss = fmaf(e.x * src, src, r_ss);
@@ -208,7 +245,9 @@ __device__ __forceinline__ void window_sums_any(const Dev &d, const SharedPatch<
                                                 const float H[9], float &s_s, float &s_ss, float &s_rs) {
     const int x0 = x + d.tap_off0, y0 = y + d.tap_off0, span = (NT - 1) * d.tap_inc;
     const bool refined = refined_division_ok(H, x0, x0 + span, y0, y0 + span);
-    if (d.quad_fmt == QUAD_FP16) {
+    if (d.fast_gather && refined && window_inside(d, H, x0, x0 + span, y0, y0 + span)) {
+        shared_window_sums<NT, true, QUAD_FP16, true>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+    } else if (d.quad_fmt == QUAD_FP16) {
         if (refined) shared_window_sums<NT, true, QUAD_FP16>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
         else shared_window_sums<NT, false, QUAD_FP16>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
     } else if (d.quad_fmt == QUAD_BF16) {

@@ -95,6 +95,9 @@ struct Dev {
     float gn_cos, gn_sin, gn_thr;
     int gn_shift;
     unsigned long long *counters;  // [2*family] NCC hyps, [2*family+1] taps; family = NccFamily
+    // 1: fp16 quads of quarter-grid texels and (W+1)(H+1) < 2^24, so windows
+    // that project inside the source take the clamp-free gather (gather_fast)
+    int fast_gather;
     // tiles of the shared-patch NCC launches: k_ncc_full (candidates, initial
     // costs) and k_ncc_weighted (refinement, post-pass)
     NccTile full_tile, wtd_tile;
