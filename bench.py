@@ -34,7 +34,7 @@ FAMILY_NAMES = ["k_ncc_full<6>", "k_ncc_weighted<6,REFINE>", "k_ncc_weighted<6,P
 # (`python bench.py --copy-calibration`: FETCH_SIZE x2.000, WRITE_SIZE x1.000);
 # the runs and the arithmetic are in profiles/ncc_kernels_1600x1200_20views_hbm_traffic.md.
 # Counters cannot be read inside a timed run.
-NCC_FULL_TRAFFIC_BYTES = 8.6173e9
+NCC_FULL_TRAFFIC_BYTES = 9.4268e9
 
 
 def parse():
@@ -53,6 +53,9 @@ def parse():
                     help="c2: the 1600x1200 20-view job (default); batch: 32 independent FIRST_INIT problems of "
                          "uneven size and source count, sequential plain calls vs the batched form")
     ap.add_argument("--lanes", type=int, default=4, help="lanes of the batched form (--workload batch)")
+    ap.add_argument("--io", choices=["device", "host"], default="device",
+                    help="device: inputs, state and outputs resident in HBM (the measured value); host: every "
+                         "call takes and returns host buffers (the PCIe-inclusive rate, reported beside it)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device-to-device 3 times (the known-size kernel that scales the "
@@ -83,6 +86,9 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         batch_bench(args)
+        return
+    if args.io == "host":
+        host_io_bench(args)
         return
     import numpy as np
     import torch
@@ -259,6 +265,79 @@ def main():
         result["cpu_baseline"] = cpu_baseline(sc, args.cpu_threads)
     print(json.dumps(result), flush=True)
     ctx.close()
+
+
+def host_io_bench(args):
+    """The 1600x1200 job with host buffers at the boundary: every call uploads
+    its images, support maps, prior state and geometric depths and downloads
+    its outputs (dpe_inputs.memory = DPE_MEM_HOST), the result assembly and the
+    state hand-off run on the host, and fusion reads host views.  Reports the
+    PCIe-inclusive NCC plane-hyp/s of the whole job."""
+    import numpy as np
+    from dpe_amd import abi, host, scene, support
+    t0 = time.time()
+    sc = scene.make_scene(1600, 1200, 1500.0, args.views, 60.0, seed=0x5EED0001, n_src=10, supersample=1)
+    print(f"[bench] scene ready in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    V, H, W = args.views, sc.height, sc.width
+    imgs = [np.ascontiguousarray(v.image, np.float32) for v in sc.views]
+    sup = [support.support_maps(v.image) for v in sc.views]
+    cams = host.scene_cameras(sc, list(range(V)))
+    depth = [np.zeros((H, W), np.float32) for _ in range(V)]
+    normal = [np.zeros((H, W, 3), np.float32) for _ in range(V)]
+    states = [np.zeros((H, W), np.uint8) for _ in range(V)]
+    selected = [np.zeros((H, W), np.uint32) for _ in range(V)]
+    ctx = host.Context(0)
+
+    def call(v, first):
+        ids = [v] + sc.pairs[v]
+        if first:
+            prm = host.call_params(sc.depth_min, sc.depth_max, max_iterations=args.iters, state=abi.FIRST_INIT,
+                                   use_APD=0, use_edge=0, geom_consistency=0, weak_peak_radius=6, call_index=0)
+            pin = host.ProblemInputs([imgs[i] for i in ids], [cams[i] for i in ids], prm, edges=sup[v][0],
+                                     edges_low=sup[v][0], labels=sup[v][1])
+        else:
+            prm = host.call_params(sc.depth_min, sc.depth_max, max_iterations=args.geom_iters,
+                                   state=abi.REFINE_ITER, use_APD=1, use_edge=1, geom_consistency=1,
+                                   weak_peak_radius=4, call_index=1, ransac_threshold=0.01 - 0.00125, rotate_time=2)
+            prior = np.concatenate([normal[v], depth[v][..., None]], -1)
+            pin = host.ProblemInputs([imgs[i] for i in ids], [cams[i] for i in ids], prm,
+                                     geom_depths=[depth[i] for i in ids], weak_info=states[v], prior_planes=prior,
+                                     prior_selected=selected[v], edges=sup[v][0], edges_low=sup[v][0],
+                                     labels=sup[v][1])
+        out = host.run_hip(ctx, pin)
+        r = host.assemble(out, prm.depth_min, prm.depth_max)
+        depth[v], normal[v], states[v], selected[v] = r["depth"], r["normal"], r["pixel_states"], out.selected_views
+        return ctx.stats().ncc_hyps
+
+    def step():
+        hyps = 0
+        for first in (True, False):
+            for v in range(V):
+                hyps += call(v, first)
+        from fuse_util import FuseSet
+        fs = FuseSet(sc, depth, normal, states)
+        n = C.c_int64()
+        ctx.check(ctx.lib.dpe_fuse(ctx.h, fs.views, fs.n, C.byref(abi.fuse_defaults(min_consistent=3)), None, 0,
+                                   C.byref(n)))
+        return hyps, n.value
+
+    for _ in range(args.warmup):
+        step()
+    t1 = time.perf_counter()
+    res = [step() for _ in range(args.steps)]
+    elapsed = time.perf_counter() - t1
+    hyps = sum(h for h, _ in res)
+    acc = float(np.mean([((np.abs(depth[v] - sc.views[v].depth) / sc.views[v].depth) < 0.01).mean()
+                         for v in range(V)]))
+    ctx.close()
+    print(json.dumps({
+        "metric": "NCC plane-hyp/s + s/depth-map @1600x1200,10 src, host buffers at the boundary (PCIe-inclusive)",
+        "value": hyps / elapsed, "unit": "NCC plane-hyp/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (seeded 8-plane scene, value-noise texture, 20% weak texture)",
+        "config": {"workload": f"{V} reference views 1600x1200, 10 sources each, host I/O", "views": V},
+        "s_per_depth_map": elapsed / args.steps / V, "accuracy_px_lt_1pct_vs_gt": acc,
+        "fused_points": res[-1][1]}), flush=True)
 
 
 def batch_bench(args):

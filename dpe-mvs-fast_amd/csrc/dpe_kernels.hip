@@ -133,8 +133,10 @@ __device__ __forceinline__ void weak_window_shape(const Dev &d, int center, int 
     }
 }
 
-// (The clamp-free gather measured slower here: 18.0 -> 24.4 ms per weak launch,
-// the kernel's private arrays grow with the extra variant.)
+// Measured and not kept here: the clamp-free gather (18.0 -> 24.4 ms per weak
+// launch) and fully unrolled 3x3 / 6x6 windows (18.0 -> 18.7-19.2 ms).  This
+// kernel is bound by the fabric traffic of its scattered neighbour-window taps
+// (about 6 TB/s counted), not by instruction latency.
 template <bool REFINED, int FMT>
 __device__ __forceinline__ float ncc_window_pre(const Dev &d, int v, const float H[9], int cx, int cy, int radius,
                                                 int inc, const float2 *__restrict__ tw, int n,
