@@ -18,7 +18,8 @@ _NP = {CV_8UC1: (np.uint8, 1), CV_32SC1: (np.int32, 1), CV_32FC1: (np.float32, 1
 # Every symbol include/dpe_host.h declares.
 EXPORTS = ["dpe_host_type_size", "dpe_host_write_bin_mat", "dpe_host_read_bin_mat", "dpe_host_read_camera",
            "dpe_host_read_pairs", "dpe_host_resize_linear", "dpe_host_rescale_nearest", "dpe_host_support_maps",
-           "dpe_host_write_ply"]
+           "dpe_host_write_ply", "dpe_host_edge_segment", "dpe_host_roberts", "dpe_host_connect", "dpe_host_canny",
+           "dpe_host_hough_lines_p"]
 
 
 def load(path=HOST_LIB_PATH):
@@ -44,7 +45,37 @@ def load(path=HOST_LIB_PATH):
     lib.dpe_host_support_maps.restype = None
     lib.dpe_host_write_ply.argtypes = [C.c_char_p, C.c_void_p, C.c_int64]
     lib.dpe_host_write_ply.restype = C.c_int
+    lib.dpe_host_edge_segment.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          C.c_void_p, C.c_void_p, P(C.c_int), P(C.c_int)]
+    lib.dpe_host_edge_segment.restype = C.c_int
+    lib.dpe_host_roberts.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    lib.dpe_host_roberts.restype = None
+    lib.dpe_host_connect.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+    lib.dpe_host_connect.restype = C.c_int
+    lib.dpe_host_canny.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_void_p]
+    lib.dpe_host_canny.restype = None
+    lib.dpe_host_hough_lines_p.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                           C.c_int, C.c_void_p, C.c_int]
+    lib.dpe_host_hough_lines_p.restype = C.c_int
     return lib
+
+
+def edge_segment(lib, image, scale, mode, use_canny, high_res=False):
+    """EdgeSegment restatement: mode 0 -> edges (u8, image size), mode 1 -> labels
+    (i32 at round(size / 2^scale))."""
+    image = np.ascontiguousarray(image, np.uint8)
+    h, w = image.shape
+    f = 1.0 / (1 << scale) if mode == 1 else 1.0
+    oh, ow = int(round(h * f)), int(round(w * f))
+    edges = np.zeros((oh, ow), np.uint8)
+    labels = np.zeros((oh, ow), np.int32)
+    gw, gh = C.c_int(), C.c_int()
+    st = lib.dpe_host_edge_segment(scale, image.ctypes.data, w, h, mode, int(use_canny), int(high_res),
+                                   edges.ctypes.data, labels.ctypes.data, C.byref(gw), C.byref(gh))
+    if st != 0:
+        raise ValueError("dpe_host_edge_segment rejected its arguments")
+    assert (gw.value, gh.value) == (ow, oh)
+    return edges if mode == 0 else labels
 
 
 def read_bin_mat(path):

@@ -5,7 +5,7 @@
 //
 //   dpe_driver input_folder [gpu_index] [--output=folder] [--vis=none|final|all]
 //              [--checkpoint=none|final|all] [--profile=on|off]
-//              [--schedule=file] [--dry-run]
+//              [--schedule=file] [--support=own|reference] [--dry-run]
 //
 // Input folder: images/%08d.pgm (8-bit gray) or .ppm (8-bit BGR-convertible
 // colour) -- JPEG decoding is not built, convert first --, cams/%08d_cam.txt,
@@ -18,6 +18,9 @@
 //   call scale=S state=FIRST_INIT|REFINE_INIT|REFINE_ITER iters=N apd=0|1 edge=0|1 geom=0|1 peak=P
 //        [ransac=F] [rotate=R]
 //   fusion min_consistent=K
+// --support=reference computes the edge and label maps with the reference's
+// EdgeSegment pipeline (Canny edges, Roberts + Hough + Connect labels;
+// dpe_host_edge_segment), --support=own (default) with dpe_host_support_maps.
 // --dry-run prints the problems and every call of the schedule, then exits
 // without touching a GPU.
 #include <sys/stat.h>
@@ -51,6 +54,7 @@ struct Options {
     Mode checkpoint = Mode::FINAL;
     bool profile = true;
     bool dry_run = false;
+    bool reference_support = false;
     std::string output, schedule;
 };
 
@@ -392,8 +396,28 @@ struct Driver {
         }
         std::vector<uint8_t> edges(u8.size());
         std::vector<int32_t> labels(u8.size());
-        // labels at the call's size with 1/1024 of its area as the small-region limit
-        dpe_host_support_maps(u8.data(), w, h, 0, edges.data(), labels.data());
+        if (opt.reference_support) {
+            // GetProblemEdges (main.cpp:166-221): edges by Canny on the scaled image,
+            // labels from the full-size image reduced to the call's size
+            int fw = 0, fh = 0, ow = 0, oh = 0;
+            const std::vector<float> &full = cache.float_image(p.ref, 1, fw, fh);
+            std::vector<uint8_t> full8(full.size());
+            for (size_t i = 0; i < full.size(); ++i)
+                full8[i] = (uint8_t)std::min(255.0f, std::max(0.0f, std::nearbyint(full[i])));
+            if (dpe_host_edge_segment(s, u8.data(), w, h, 0, 1, 1, edges.data(), nullptr, &ow, &oh) != 0 ||
+                ow != w || oh != h)
+                throw std::runtime_error("EdgeSegment (edges) failed for " + p.result_folder);
+            std::vector<int32_t> lab((size_t)(std::lround(fw / (double)(1 << s)) + 1) *
+                                     (std::lround(fh / (double)(1 << s)) + 1));
+            if (dpe_host_edge_segment(s, full8.data(), fw, fh, 1, 0, 1, nullptr, lab.data(), &ow, &oh) != 0 ||
+                ow != w || oh != h)
+                throw std::runtime_error("EdgeSegment (labels) does not match the call's size for " +
+                                         p.result_folder);
+            std::copy(lab.begin(), lab.begin() + (size_t)w * h, labels.begin());
+        } else {
+            // labels at the call's size with 1/1024 of its area as the small-region limit
+            dpe_host_support_maps(u8.data(), w, h, 0, edges.data(), labels.data());
+        }
         if (dpe_host_write_bin_mat(ep.c_str(), h, w, DPE_HOST_CV_8UC1, edges.data()) != 0 ||
             dpe_host_write_bin_mat(lp.c_str(), h, w, DPE_HOST_CV_32SC1, labels.data()) != 0)
             throw std::runtime_error("Cannot write support maps to " + p.result_folder);
@@ -661,6 +685,8 @@ Options parse_options(int argc, char **argv) {
             o.output = argv[++i];
         } else if (a.rfind("--schedule=", 0) == 0) o.schedule = a.substr(11);
         else if (a == "--dry-run") o.dry_run = true;
+        else if (a == "--support=reference") o.reference_support = true;
+        else if (a == "--support=own") o.reference_support = false;
         else if (!gpu_set && !a.empty() && a[0] != '-') {
             o.gpu = std::atoi(a.c_str());
             gpu_set = true;
@@ -674,7 +700,8 @@ Options parse_options(int argc, char **argv) {
 int main(int argc, char **argv) {
     if (argc < 2) {
         std::cerr << "USAGE: dpe_driver input_folder [gpu_index] [--output=output_folder] [--vis=none|final|all] "
-                     "[--checkpoint=none|final|all] [--profile=on|off] [--schedule=file] [--dry-run]\n";
+                     "[--checkpoint=none|final|all] [--profile=on|off] [--schedule=file] "
+                     "[--support=own|reference] [--dry-run]\n";
         return EXIT_FAILURE;
     }
     Driver d;

@@ -63,6 +63,28 @@ void dpe_host_rescale_nearest(const void *src, int w, int h, void *dst, int dw, 
  * -1 = small region, > 0 = region).  Either output may be NULL. */
 void dpe_host_support_maps(const uint8_t *image, int w, int h, int scale, uint8_t *edges, int32_t *labels);
 
+/* The reference's own segmentation (EdgeSegment, DPE.cpp:136-291) without
+ * OpenCV, for the driver's --support=reference: mode 0 = edges (Canny on the
+ * image with median-derived thresholds, use_canny = 1 in the reference's call,
+ * main.cpp:187), u8 0/255 at w x h; mode 1 = labels (Roberts + threshold +
+ * Connect + HoughLinesP borders of large regions at a quarter of the size,
+ * use_canny = 0, main.cpp:207), i32 (0 edge, -1 small region, > 0 region) at
+ * round(w / 2^scale) x round(h / 2^scale).  *out_w, *out_h receive the output
+ * size.  Returns 0, or -1 for bad arguments.  The OpenCV routines are restated
+ * from their published algorithms; agreement with an OpenCV build is not
+ * checked here. */
+int dpe_host_edge_segment(int scale, const uint8_t *image, int w, int h, int mode, int use_canny, int high_res,
+                          uint8_t *edges, int32_t *labels, int *out_w, int *out_h);
+/* Its building blocks (exposed for tests): Roberts (DPE.cpp:9-25); Connect
+ * (DPE.cpp:27-139, returns the number of labels, counts per label up to
+ * capacity); cv::Canny (aperture 3, L2 gradient); cv::HoughLinesP (lines as
+ * x0 y0 x1 y1, returns the count). */
+void dpe_host_roberts(const uint8_t *src, int w, int h, uint8_t *dst);
+int dpe_host_connect(const uint8_t *bin, int w, int h, int32_t *labels, int32_t *counts, int capacity);
+void dpe_host_canny(const uint8_t *src, int w, int h, double low, double high, uint8_t *edges);
+int dpe_host_hough_lines_p(const uint8_t *img, int w, int h, double rho, double theta, int threshold, int min_len,
+                           int max_gap, int32_t *lines, int max_lines);
+
 /* ExportPointCloud (DPE.cpp:532-572): binary little-endian PLY, xyz float +
  * bgr uchar per vertex.  Returns 0 on success. */
 int dpe_host_write_ply(const char *path, const dpe_point *points, int64_t count);

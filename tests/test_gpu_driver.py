@@ -45,3 +45,42 @@ def test_driver_matches_python_schedule(tmp_path):
     print(f"driver fused {len(verts)} points, python {len(pts)}")
     assert len(verts) == len(pts) > 0
     assert np.array_equal(verts["x"], pts["x"]) and np.array_equal(verts["b"], pts["b"])
+
+
+def test_driver_with_reference_segmentation(tmp_path):
+    """--support=reference: the support maps come from the EdgeSegment
+    restatement (Canny edges, Roberts + Hough + Connect labels); the run
+    completes, writes maps of the call's size and a point cloud whose depths
+    agree with ground truth about as well as with this project's detector."""
+    s = scene.make_scene(320, 240, 300.0, 4, 24.0, seed=0x5EED0005, supersample=1)
+    folder = tmp_path / "scene"
+    scene.write_folder(s, str(folder))
+    # a REFINE_ITER call with APD, edge sampling and the weak path consumes the maps
+    sched = tmp_path / "refine.sched"
+    sched.write_text("call scale=1 state=FIRST_INIT iters=3 apd=0 edge=0 geom=0 peak=6\n"
+                     "call scale=1 state=REFINE_ITER iters=2 apd=1 edge=1 geom=1 peak=4 ransac=0.00875 rotate=2\n"
+                     "fusion min_consistent=1\n")
+    acc, pts = {}, {}
+    for mode in ("own", "reference"):
+        out = tmp_path / ("out_" + mode)
+        print(f"driver --support={mode}: 4 views 320x240", flush=True)
+        r = subprocess.run([hostio.DRIVER_PATH, str(folder), "0", "--output=" + str(out), "--checkpoint=final",
+                            "--schedule=" + str(sched), "--support=" + mode], capture_output=True, text=True,
+                           timeout=300)
+        print(r.stdout[-1500:], r.stderr[-1500:])
+        assert r.returncode == 0
+        labels = hostio.read_bin_mat(str(out / "views" / "00000001" / "labels_0.dmb"))
+        edges = hostio.read_bin_mat(str(out / "views" / "00000001" / "edges_0.dmb"))
+        assert labels.shape == edges.shape == (240, 320)
+        good = []
+        for i in range(4):
+            d = hostio.read_bin_mat(str(out / "views" / ("%08d" % i) / "depths.dmb"))
+            gt = s.views[i].depth
+            good.append(float((np.abs(d - gt) / gt < 0.01).mean()))
+        acc[mode] = float(np.mean(good))
+        _, verts = hostio.read_ply(str(out / "DPE.ply"))
+        pts[mode] = len(verts)
+        print(f"--support={mode}: {len(verts)} points, {acc[mode]:.4f} of pixels within 1 % of ground truth")
+        assert len(verts) > 0
+    assert acc["reference"] >= acc["own"] - 0.05
+    assert pts["reference"] != pts["own"]  # the maps are consumed: the runs differ
