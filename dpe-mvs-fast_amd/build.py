@@ -31,7 +31,8 @@ def hipcc():
 
 HOST_LIB = os.path.join(HERE, "libdpe_host.so")
 DRIVER = os.path.join(HERE, "dpe_driver")
-HOST_SOURCES = [os.path.join(HERE, "host", "dpe_hostio.cpp"), os.path.join(HERE, "host", "dpe_edges.cpp")]
+HOST_SOURCES = [os.path.join(HERE, "host", "dpe_hostio.cpp"), os.path.join(HERE, "host", "dpe_edges.cpp"),
+                os.path.join(HERE, "host", "dpe_jpeg.cpp")]
 DRIVER_SOURCES = [os.path.join(HERE, "host", "dpe_driver.cpp")]
 GXX_FLAGS = ["-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include")]
 

@@ -19,7 +19,7 @@ _NP = {CV_8UC1: (np.uint8, 1), CV_32SC1: (np.int32, 1), CV_32FC1: (np.float32, 1
 EXPORTS = ["dpe_host_type_size", "dpe_host_write_bin_mat", "dpe_host_read_bin_mat", "dpe_host_read_camera",
            "dpe_host_read_pairs", "dpe_host_resize_linear", "dpe_host_rescale_nearest", "dpe_host_support_maps",
            "dpe_host_write_ply", "dpe_host_edge_segment", "dpe_host_roberts", "dpe_host_connect", "dpe_host_canny",
-           "dpe_host_hough_lines_p"]
+           "dpe_host_hough_lines_p", "dpe_host_read_jpeg"]
 
 
 def load(path=HOST_LIB_PATH):
@@ -57,7 +57,22 @@ def load(path=HOST_LIB_PATH):
     lib.dpe_host_hough_lines_p.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                            C.c_int, C.c_void_p, C.c_int]
     lib.dpe_host_hough_lines_p.restype = C.c_int
+    lib.dpe_host_read_jpeg.argtypes = [C.c_char_p, C.c_int, C.c_void_p, C.c_size_t, P(C.c_int), P(C.c_int)]
+    lib.dpe_host_read_jpeg.restype = C.c_int
     return lib
+
+
+def read_jpeg(lib, path, color=False):
+    """Decode a JPEG as the driver does: grayscale (H, W) u8, or colour (H, W, 3) BGR."""
+    w, h = C.c_int(), C.c_int()
+    st = lib.dpe_host_read_jpeg(str(path).encode(), int(color), None, 0, C.byref(w), C.byref(h))
+    if st != 0:
+        raise ValueError(f"cannot decode {path}: status {st}")
+    out = np.zeros((h.value, w.value, 3) if color else (h.value, w.value), np.uint8)
+    st = lib.dpe_host_read_jpeg(str(path).encode(), int(color), out.ctypes.data, out.nbytes, C.byref(w), C.byref(h))
+    if st != 0:
+        raise ValueError(f"cannot decode {path}: status {st}")
+    return out
 
 
 def edge_segment(lib, image, scale, mode, use_canny, high_res=False):

@@ -7,8 +7,8 @@
 //              [--checkpoint=none|final|all] [--profile=on|off]
 //              [--schedule=file] [--support=own|reference] [--dry-run]
 //
-// Input folder: images/%08d.pgm (8-bit gray) or .ppm (8-bit BGR-convertible
-// colour) -- JPEG decoding is not built, convert first --, cams/%08d_cam.txt,
+// Input folder: images/%08d.jpg (sequential 8-bit JPEG, decoded as libjpeg
+// does for cv::imread), .pgm (8-bit gray) or .ppm (8-bit colour), cams/%08d_cam.txt,
 // pair.txt (main.cpp:336-341).  Output: <output>/views/%08d/{depths.dmb,
 // normals.dmb, weak.bin, selected_views.bin, edges_<s>.dmb, labels_<s>.dmb}
 // and <output>/DPE.ply.
@@ -166,11 +166,17 @@ struct SceneCache {
         if (it != gray.end()) return it->second;
         int w = 0, h = 0, ch = 0;
         std::vector<uint8_t> data;
-        if (!read_pnm(image_path(id, ".pgm"), w, h, ch, data) && !read_pnm(image_path(id, ".ppm"), w, h, ch, data)) {
-            if (exists(image_path(id, ".jpg")))
-                throw std::runtime_error("JPEG decoding is not built; convert " + image_path(id, ".jpg") +
-                                         " to PGM/PPM");
-            throw std::runtime_error("Cannot read image: " + image_path(id, ".pgm"));
+        // images/%08d.jpg as cv::imread(IMREAD_GRAYSCALE) reads it (DPE.cpp:713), else PGM / PPM
+        const std::string jpg = image_path(id, ".jpg");
+        if (exists(jpg)) {
+            const int st = dpe_host_read_jpeg(jpg.c_str(), 0, nullptr, 0, &w, &h);
+            data.resize((size_t)w * h);
+            if (st != 0 || dpe_host_read_jpeg(jpg.c_str(), 0, data.data(), data.size(), &w, &h) != 0)
+                throw std::runtime_error("Cannot decode image (sequential 8-bit JPEG only): " + jpg);
+            ch = 1;
+        } else if (!read_pnm(image_path(id, ".pgm"), w, h, ch, data) &&
+                   !read_pnm(image_path(id, ".ppm"), w, h, ch, data)) {
+            throw std::runtime_error("Cannot read image: " + image_path(id, ".jpg") + " / .pgm / .ppm");
         }
         std::vector<uint8_t> g((size_t)w * h);
         if (ch == 1) g = data;
@@ -184,6 +190,12 @@ struct SceneCache {
     std::vector<uint8_t> color_image(int id, int &w, int &h) {
         int ch = 0;
         std::vector<uint8_t> data;
+        // cv::imread(IMREAD_COLOR) (DPE.cpp:1422)
+        const std::string jpg = image_path(id, ".jpg");
+        if (exists(jpg) && dpe_host_read_jpeg(jpg.c_str(), 1, nullptr, 0, &w, &h) == 0) {
+            data.resize((size_t)w * h * 3);
+            if (dpe_host_read_jpeg(jpg.c_str(), 1, data.data(), data.size(), &w, &h) == 0) return data;
+        }
         if (read_pnm(image_path(id, ".ppm"), w, h, ch, data) && ch == 3) return data;
         const std::vector<uint8_t> &g = gray_image(id);
         w = size[id].first;

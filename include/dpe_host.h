@@ -85,6 +85,16 @@ void dpe_host_canny(const uint8_t *src, int w, int h, double low, double high, u
 int dpe_host_hough_lines_p(const uint8_t *img, int w, int h, double rho, double theta, int threshold, int min_len,
                            int max_gap, int32_t *lines, int max_lines);
 
+/* cv::imread of a JPEG (DPE.cpp:713 grayscale, DPE.cpp:1422 colour) without
+ * OpenCV: sequential-Huffman 8-bit JPEG (baseline / extended; 1 or 3
+ * components, any sampling factors, restart intervals) decoded as libjpeg 6b /
+ * libjpeg-turbo does by default: islow integer IDCT; grayscale = the Y
+ * component; colour = triangle-filter chroma upsampling (h2v1, h2v2) and the
+ * 16-bit fixed-point YCbCr -> RGB tables, written as BGR.  out == NULL: only
+ * *width, *height.  Returns 0, -1 (cannot open), -2 (not a supported JPEG),
+ * -3 (capacity below width * height * (color ? 3 : 1)). */
+int dpe_host_read_jpeg(const char *path, int color, uint8_t *out, size_t capacity, int *width, int *height);
+
 /* ExportPointCloud (DPE.cpp:532-572): binary little-endian PLY, xyz float +
  * bgr uchar per vertex.  Returns 0 on success. */
 int dpe_host_write_ply(const char *path, const dpe_point *points, int64_t count);

@@ -84,3 +84,39 @@ def test_driver_with_reference_segmentation(tmp_path):
         assert len(verts) > 0
     assert acc["reference"] >= acc["own"] - 0.05
     assert pts["reference"] != pts["own"]  # the maps are consumed: the runs differ
+
+
+def test_driver_reads_jpeg_folders(tmp_path):
+    """images/%08d.jpg input: the driver decodes the JPEGs itself (as
+    cv::imread does through libjpeg) and its final state equals the Python
+    schedule run on PIL's (libjpeg-turbo's) decode of the same files."""
+    import copy
+    from PIL import Image
+    s = scene.make_scene(160, 120, 150.0, 4, 24.0, seed=0x5EED0006, supersample=1)
+    folder = tmp_path / "scene"
+    scene.write_folder(s, str(folder))
+    s2 = copy.deepcopy(s)
+    for i, v in enumerate(s.views):
+        pgm = folder / "images" / ("%08d.pgm" % i)
+        jpg = folder / "images" / ("%08d.jpg" % i)
+        col = np.stack([v.image, np.roll(v.image, 2, 1), 255 - v.image], -1).astype(np.uint8)
+        Image.fromarray(col).save(jpg, quality=92, subsampling=2)
+        pgm.unlink()
+        im = Image.open(jpg)
+        im.draft("L", im.size)
+        s2.views[i].image = np.array(im.convert("L") if im.mode != "L" else im)
+    out = tmp_path / "out"
+    r = subprocess.run([hostio.DRIVER_PATH, str(folder), "0", "--output=" + str(out), "--checkpoint=final"],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0
+    ctx = host.Context(0)
+    try:
+        store = host.run_schedule_scale1(ctx, s2, host.reference_schedule(1))
+    finally:
+        ctx.close()
+    for i in range(4):
+        f = out / "views" / ("%08d" % i)
+        assert np.array_equal(hostio.read_bin_mat(str(f / "depths.dmb")), store.get(i)["depth"]), i
+    _, verts = hostio.read_ply(str(out / "DPE.ply"))
+    assert len(verts) > 0

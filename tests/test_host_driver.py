@@ -74,6 +74,16 @@ def test_camera_and_pairs(lib, tmp_path):
                                 srcs.ctypes.data)
     assert n == 2 and list(refs[:2]) == [7, 8] and list(counts[:2]) == [1, 1]
     assert srcs[0, 0] == 1 and srcs[1, 0] == 4
+    # more than 10 sources (ETH3D-style pair files): all positive ones are kept in order
+    ids = list(range(10, 30))
+    with open(tmp_path / "pair3.txt", "w") as f:
+        f.write("1
+3
+%d %s
+" % (len(ids) + 2, " ".join("%d %.2f" % (i, 30 - i) for i in ids) + " 40 0 41 -2"))
+    n = lib.dpe_host_read_pairs(str(tmp_path / "pair3.txt").encode(), 4, refs.ctypes.data, counts.ctypes.data,
+                                srcs.ctypes.data)
+    assert n == 1 and counts[0] == 20 and list(srcs[0, :20]) == ids
 
 
 def test_resize_factor_two_is_box_mean_and_nearest_rescale(lib):
