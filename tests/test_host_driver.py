@@ -77,10 +77,8 @@ def test_camera_and_pairs(lib, tmp_path):
     # more than 10 sources (ETH3D-style pair files): all positive ones are kept in order
     ids = list(range(10, 30))
     with open(tmp_path / "pair3.txt", "w") as f:
-        f.write("1
-3
-%d %s
-" % (len(ids) + 2, " ".join("%d %.2f" % (i, 30 - i) for i in ids) + " 40 0 41 -2"))
+        body = " ".join("%d %.2f" % (i, 30 - i) for i in ids) + " 40 0 41 -2"
+        f.write("1" + chr(10) + "3" + chr(10) + "%d %s" % (len(ids) + 2, body) + chr(10))
     n = lib.dpe_host_read_pairs(str(tmp_path / "pair3.txt").encode(), 4, refs.ctypes.data, counts.ctypes.data,
                                 srcs.ctypes.data)
     assert n == 1 and counts[0] == 20 and list(srcs[0, :20]) == ids
