@@ -49,9 +49,11 @@ def parse():
     ap.add_argument("--precision", choices=["plain", "fp16", "bf16"], default="plain",
                     help="dpe_form.precision of every call (plain = the reference's fp32 arithmetic)")
     ap.add_argument("--gauss-newton", type=int, default=0, help="dpe_form.gauss_newton_steps of every call")
-    ap.add_argument("--workload", choices=["c2", "batch"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "batch", "fusion"], default="c2",
                     help="c2: the 1600x1200 20-view job (default); batch: 32 independent FIRST_INIT problems of "
-                         "uneven size and source count, sequential plain calls vs the batched form")
+                         "uneven size and source count, sequential plain calls vs the batched form; fusion: the "
+                         "three fusions (RunFusion, TAT intermediate, TAT advanced) on 20 resident 1600x1200 "
+                         "depth maps")
     ap.add_argument("--lanes", type=int, default=4, help="lanes of the batched form (--workload batch)")
     ap.add_argument("--io", choices=["device", "host"], default="device",
                     help="device: inputs, state and outputs resident in HBM (the measured value); host: every "
@@ -86,6 +88,9 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         batch_bench(args)
+        return
+    if args.workload == "fusion":
+        fusion_bench(args)
         return
     if args.io == "host":
         host_io_bench(args)
@@ -460,6 +465,106 @@ def batch_bench(args):
         "bit_identical": same,
         "tiles": tiles,
         "accuracy_px_lt_1pct_vs_gt": float(np.mean(acc)),
+    }
+    print(json.dumps(result), flush=True)
+
+
+def fusion_bench(args):
+    """The fusions alone: 20 reference views of 1600x1200 with 10 sources each,
+    ground-truth depth with 0.05 % noise and 5 % holes, ground-truth normals and
+    pixel states, colours; everything resident in HBM, points written to HBM.
+    Timed per fusion: `--steps` calls after `--warmup` (at least one) untimed
+    ones.  CPU baseline: the oracle's sequential restatement fusing reference
+    view 0 against its 10 sources (the other views keep their data but list no
+    sources), one thread, as the reference runs it."""
+    import numpy as np
+    import torch
+    import oracle_lib
+    from dpe_amd import abi, host, scene
+    from fuse_util import FuseSet
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(0x5EED00F5)
+    t0 = time.time()
+    sc = scene.make_scene(1600, 1200, 1500.0, args.views, 60.0, seed=0x5EED0001, n_src=10, supersample=1)
+    V, H, W = args.views, sc.height, sc.width
+    depths = []
+    for v in sc.views:
+        d = (v.depth * (1 + 0.0005 * rng.standard_normal(v.depth.shape))).astype(np.float32)
+        d[rng.uniform(size=d.shape) < 0.05] = 0.0
+        depths.append(d)
+    states = [np.where(v.weak, abi.WEAK, abi.STRONG).astype(np.uint8) for v in sc.views]
+    fs = FuseSet(sc, depths, [v.normal for v in sc.views], states)
+    print(f"[bench] fusion inputs ready in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    keep = []
+    views = (abi.FuseView * V)()
+    for i in range(V):
+        C.memmove(C.addressof(views[i]), C.addressof(fs.views[i]), C.sizeof(abi.FuseView))
+        P = W * H
+        for name, ctype, k in (("depth", C.c_float, 1), ("normal", C.c_float, 3), ("pixel_states", C.c_uint8, 1),
+                               ("color", C.c_uint8, 3)):
+            a = np.ctypeslib.as_array(C.cast(getattr(fs.views[i], name), C.POINTER(ctype)), (P * k,))
+            t = torch.from_numpy(a.copy()).to(dev)
+            keep.append(t)
+            setattr(views[i], name, t.data_ptr())
+    points = torch.empty((V * H * W, 4), dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    ctx = host.Context(0)
+    lib = ctx.lib
+    n = C.c_int64()
+    fp = abi.fuse_defaults(min_consistent=3, memory=abi.MEM_DEVICE)
+    runs = {
+        "RunFusion (min_consistent 3)": lambda: lib.dpe_fuse(ctx.h, views, V, C.byref(fp), points.data_ptr(),
+                                                            V * H * W, C.byref(n)),
+    }
+    for name, variant in (("TAT intermediate", abi.TAT_INTERMEDIATE), ("TAT advanced", abi.TAT_ADVANCED)):
+        tp = abi.fuse_tat_defaults(variant, memory=abi.MEM_DEVICE)
+        runs[name] = (lambda tp=tp: lib.dpe_fuse_tat(ctx.h, views, V, None, C.byref(tp), points.data_ptr(),
+                                                     V * H * W, C.byref(n)))
+    res = {}
+    for name, fn in runs.items():
+        for _ in range(max(args.warmup, 1)):
+            ctx.check(fn())
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.check(fn())
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t1) / args.steps
+        res[name] = {"ms": dt * 1e3, "ref_pixels_per_s": V * H * W / dt, "points": n.value}
+        print(f"[bench] {name}: {dt * 1e3:.1f} ms, {n.value} points", file=sys.stderr, flush=True)
+    ctx.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        orc = oracle_lib.load_oracle()
+        from fuse_util import fuse_oracle, fuse_tat_oracle
+        for i in range(1, V):
+            fs.views[i].num_src = 0
+        cpu = {}
+        t2 = time.perf_counter()
+        fuse_oracle(orc, fs, abi.fuse_defaults(min_consistent=3))
+        cpu["RunFusion (min_consistent 3)"] = H * W / (time.perf_counter() - t2)
+        t2 = time.perf_counter()
+        fuse_tat_oracle(orc, fs, abi.fuse_tat_defaults(abi.TAT_INTERMEDIATE))
+        cpu["TAT intermediate"] = H * W / (time.perf_counter() - t2)
+        cpu = {"value": cpu["TAT intermediate"], "unit": "reference pixels/s", "cores": 1, "kind": "port",
+               "sample": "oracle fusing reference view 0 (1600x1200) against its 10 sources; per fusion: " +
+                         ", ".join(f"{k} {v:.3g} px/s" for k, v in cpu.items())}
+    result = {
+        "metric": "reference pixels fused per second (TAT intermediate)",
+        "value": res["TAT intermediate"]["ref_pixels_per_s"],
+        "unit": "reference pixels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": res["TAT intermediate"]["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded 8-plane scene; ground-truth depth + 0.05 % noise, 5 % holes)",
+        "config": {"workload": f"fusion of {V} depth maps of {W}x{H}, 10 sources each"},
+        "fusions": res,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)
 

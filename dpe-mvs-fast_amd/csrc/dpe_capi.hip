@@ -830,6 +830,83 @@ dpe_status ncc_cost_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *fo
     return DPE_OK;
 }
 
+// Validated views of a fusion call, uploaded back to back (FuseViewDev.offset).
+struct FuseUpload {
+    std::vector<FuseViewDev> fv;
+    long total = 0;
+    int maxP = 0;
+    FuseDev f{};
+};
+
+FuseUpload fuse_upload(dpe_ctx *ctx, const dpe_fuse_view *views, int nv, bool need_states, bool dev) {
+    hipStream_t s = ctx->stream;
+    FuseUpload u;
+    u.fv.resize(nv);
+    for (int i = 0; i < nv; ++i) {
+        const dpe_fuse_view &v = views[i];
+        if (v.width <= 0 || v.height <= 0 || !v.depth || !v.normal || (need_states && !v.pixel_states) || !v.color)
+            throw Fail{fail(ctx, DPE_ERR_INVALID_ARGUMENT, "fusion view " + std::to_string(i) + " is incomplete")};
+        if (v.num_src < 0 || v.num_src > DPE_MAX_IMAGES)
+            throw Fail{fail(ctx, DPE_ERR_INVALID_ARGUMENT, "fusion view " + std::to_string(i) + ": bad source count")};
+        FuseViewDev &d = u.fv[i];
+        d.width = v.width;
+        d.height = v.height;
+        d.offset = u.total;
+        d.cam = v.camera;
+        d.num_src = v.num_src;
+        for (int j = 0; j < v.num_src; ++j) {
+            if (v.src_index[j] < 0 || v.src_index[j] >= nv || v.src_index[j] == i)
+                throw Fail{fail(ctx, DPE_ERR_INVALID_ARGUMENT, "fusion view " + std::to_string(i) + ": bad source index")};
+            d.src[j] = v.src_index[j];
+        }
+        u.total += (long)v.width * v.height;
+        u.maxP = std::max(u.maxP, v.width * v.height);
+    }
+    FuseViewDev *dv = (FuseViewDev *)buffer(ctx, B_F_VIEWS, sizeof(FuseViewDev) * nv);
+    float *depth = (float *)buffer(ctx, B_F_DEPTH, sizeof(float) * u.total);
+    float *normal = (float *)buffer(ctx, B_F_NORMAL, sizeof(float) * 3 * u.total);
+    uint8_t *states = need_states ? (uint8_t *)buffer(ctx, B_F_STATES, u.total) : nullptr;
+    uint8_t *color = (uint8_t *)buffer(ctx, B_F_COLOR, 3 * u.total);
+    uint8_t *mask = (uint8_t *)buffer(ctx, B_F_MASK, u.total);
+    HIPCHK(hipMemcpyAsync(dv, u.fv.data(), sizeof(FuseViewDev) * nv, hipMemcpyHostToDevice, s));
+    const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    for (int i = 0; i < nv; ++i) {
+        const size_t P = (size_t)u.fv[i].width * u.fv[i].height;
+        HIPCHK(hipMemcpyAsync(depth + u.fv[i].offset, views[i].depth, sizeof(float) * P, kind, s));
+        HIPCHK(hipMemcpyAsync(normal + 3 * u.fv[i].offset, views[i].normal, sizeof(float) * 3 * P, kind, s));
+        if (need_states) HIPCHK(hipMemcpyAsync(states + u.fv[i].offset, views[i].pixel_states, P, kind, s));
+        HIPCHK(hipMemcpyAsync(color + 3 * u.fv[i].offset, views[i].color, 3 * P, kind, s));
+    }
+    HIPCHK(hipMemsetAsync(mask, 0, u.total, s));
+    u.f.views = dv;
+    u.f.depth = depth;
+    u.f.normal = normal;
+    u.f.states = states;
+    u.f.color = color;
+    u.f.mask = mask;
+    return u;
+}
+
+// Copies n points of one view from the staging buffer to the caller's array.
+void fuse_emit(dpe_ctx *ctx, const dpe_point *stage, int64_t n, dpe_point *points, int64_t capacity, int64_t count,
+               bool dev) {
+    if (points && count < capacity && n > 0) {
+        const int64_t m = std::min<int64_t>(n, capacity - count);
+        HIPCHK(hipMemcpyAsync(points + count, stage, sizeof(dpe_point) * m,
+                              dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+}
+
+// Number of points of a compaction: exclusive sum at P-1 plus the last flag.
+int64_t fuse_count(dpe_ctx *ctx, const int *index, const int *flags, int P) {
+    int last_index = 0, last_flag = 0;
+    HIPCHK(hipMemcpyAsync(&last_index, index + P - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(&last_flag, flags + P - 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return (int64_t)last_index + last_flag;
+}
+
 dpe_status fuse_impl(dpe_ctx *ctx, const dpe_fuse_view *views, int nv, const dpe_fuse_params *fp, dpe_point *points,
                      int64_t capacity, int64_t *num_points) {
     if (!views || nv <= 0 || !fp || !num_points) return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad fusion arguments");
@@ -838,36 +915,11 @@ dpe_status fuse_impl(dpe_ctx *ctx, const dpe_fuse_view *views, int nv, const dpe
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     const bool dev = fp->memory == DPE_MEM_DEVICE;
-    std::vector<FuseViewDev> fv(nv);
-    long total = 0;
-    int maxP = 0;
-    for (int i = 0; i < nv; ++i) {
-        const dpe_fuse_view &v = views[i];
-        if (v.width <= 0 || v.height <= 0 || !v.depth || !v.normal || !v.pixel_states || !v.color)
-            return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "fusion view " + std::to_string(i) + " is incomplete");
-        if (v.num_src < 0 || v.num_src > DPE_MAX_IMAGES)
-            return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "fusion view " + std::to_string(i) + ": bad source count");
-        fv[i].width = v.width;
-        fv[i].height = v.height;
-        fv[i].offset = total;
-        fv[i].cam = v.camera;
-        fv[i].num_src = v.num_src;
-        for (int j = 0; j < v.num_src; ++j) {
-            if (v.src_index[j] < 0 || v.src_index[j] >= nv || v.src_index[j] == i)
-                return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "fusion view " + std::to_string(i) + ": bad source index");
-            fv[i].src[j] = v.src_index[j];
-        }
-        total += (long)v.width * v.height;
-        maxP = std::max(maxP, v.width * v.height);
-    }
-    FuseDev f{};
-    FuseViewDev *dv = (FuseViewDev *)buffer(ctx, B_F_VIEWS, sizeof(FuseViewDev) * nv);
-    float *depth = (float *)buffer(ctx, B_F_DEPTH, sizeof(float) * total);
-    float *normal = (float *)buffer(ctx, B_F_NORMAL, sizeof(float) * 3 * total);
-    uint8_t *states = (uint8_t *)buffer(ctx, B_F_STATES, total);
-    uint8_t *color = (uint8_t *)buffer(ctx, B_F_COLOR, 3 * total);
-    uint8_t *mask = (uint8_t *)buffer(ctx, B_F_MASK, total);
-    int *claim = (int *)buffer(ctx, B_F_CLAIM, sizeof(int) * total);
+    FuseUpload u = fuse_upload(ctx, views, nv, true, dev);
+    const std::vector<FuseViewDev> &fv = u.fv;
+    const int maxP = u.maxP;
+    FuseDev &f = u.f;
+    int *claim = (int *)buffer(ctx, B_F_CLAIM, sizeof(int) * u.total);
     int *used[2] = {(int *)buffer(ctx, B_F_USED0, sizeof(int) * (size_t)maxP * DPE_MAX_IMAGES),
                     (int *)buffer(ctx, B_F_USED1, sizeof(int) * (size_t)maxP * DPE_MAX_IMAGES)};
     uint8_t *acc[2] = {(uint8_t *)buffer(ctx, B_F_ACC0, maxP), (uint8_t *)buffer(ctx, B_F_ACC1, maxP)};
@@ -877,22 +929,6 @@ dpe_status fuse_impl(dpe_ctx *ctx, const dpe_fuse_view *views, int nv, const dpe
     HIPCHK(fuse_scan_temp_bytes(maxP, &temp_bytes));
     void *temp = buffer(ctx, B_F_TEMP, temp_bytes);
     int *changed = (int *)buffer(ctx, B_F_CHANGED, sizeof(int));
-    HIPCHK(hipMemcpyAsync(dv, fv.data(), sizeof(FuseViewDev) * nv, hipMemcpyHostToDevice, s));
-    const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    for (int i = 0; i < nv; ++i) {
-        const size_t P = (size_t)fv[i].width * fv[i].height;
-        HIPCHK(hipMemcpyAsync(depth + fv[i].offset, views[i].depth, sizeof(float) * P, kind, s));
-        HIPCHK(hipMemcpyAsync(normal + 3 * fv[i].offset, views[i].normal, sizeof(float) * 3 * P, kind, s));
-        HIPCHK(hipMemcpyAsync(states + fv[i].offset, views[i].pixel_states, P, kind, s));
-        HIPCHK(hipMemcpyAsync(color + 3 * fv[i].offset, views[i].color, 3 * P, kind, s));
-    }
-    HIPCHK(hipMemsetAsync(mask, 0, total, s));
-    f.views = dv;
-    f.depth = depth;
-    f.normal = normal;
-    f.states = states;
-    f.color = color;
-    f.mask = mask;
     f.claim = claim;
     f.max_reproj = fp->max_reproj_px;
     f.max_rel = fp->max_rel_depth;
@@ -925,17 +961,77 @@ dpe_status fuse_impl(dpe_ctx *ctx, const dpe_fuse_view *views, int nv, const dpe
         }
         HIPCHK(launch_fuse_compact(f, i, P, used[cur], acc[cur], flags, index, temp, temp_bytes, stage, 0, s));
         HIPCHK(hipGetLastError());
-        int last_index = 0, last_flag = 0;
-        HIPCHK(hipMemcpyAsync(&last_index, index + P - 1, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(&last_flag, flags + P - 1, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        const int64_t n = (int64_t)last_index + last_flag;
-        if (points && count < capacity && n > 0) {
-            const int64_t m = std::min<int64_t>(n, capacity - count);
-            HIPCHK(hipMemcpyAsync(points + count, stage, sizeof(dpe_point) * m,
-                                  dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
+        const int64_t n = fuse_count(ctx, index, flags, P);
+        fuse_emit(ctx, stage, n, points, capacity, count, dev);
+        count += n;
+    }
+    *num_points = count;
+    return DPE_OK;
+}
+
+dpe_status fuse_tat_impl(dpe_ctx *ctx, const dpe_fuse_view *views, int nv, const uint8_t *const *blocks,
+                         const dpe_fuse_tat_params *tp, dpe_point *points, int64_t capacity, int64_t *num_points) {
+    if (!views || nv <= 0 || !tp || !num_points)
+        return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad fusion arguments");
+    if (tp->memory != DPE_MEM_HOST && tp->memory != DPE_MEM_DEVICE)
+        return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad memory kind");
+    if (tp->variant != DPE_TAT_INTERMEDIATE && tp->variant != DPE_TAT_ADVANCED)
+        return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "bad TAT fusion variant");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const bool dev = tp->memory == DPE_MEM_DEVICE;
+    FuseUpload u = fuse_upload(ctx, views, nv, false, dev);
+    const std::vector<FuseViewDev> &fv = u.fv;
+    const int maxP = u.maxP;
+    FuseDev &f = u.f;
+    int maxN = 0;
+    for (int i = 0; i < nv; ++i) {
+        if ((long)fv[i].num_src * fv[i].width * fv[i].height > 0x7FFFFFFFL)
+            return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "TAT fusion view " + std::to_string(i) + " is too large");
+        maxN = std::max(maxN, fv[i].num_src * fv[i].width * fv[i].height);
+    }
+    if (blocks) {
+        // block maps packed like the depth maps; a view without one lets every pixel through
+        uint8_t *block = (uint8_t *)buffer(ctx, B_F_STATES, u.total);
+        const hipMemcpyKind kind = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        for (int i = 0; i < nv; ++i) {
+            const size_t P = (size_t)fv[i].width * fv[i].height;
+            if (blocks[i]) HIPCHK(hipMemcpyAsync(block + fv[i].offset, blocks[i], P, kind, s));
+            else HIPCHK(hipMemsetAsync(block + fv[i].offset, 255, P, s));
         }
+        f.block = block;
+    }
+    f.dist_base = tp->dist_base;
+    f.depth_base = tp->depth_base;
+    f.angle_base = tp->angle_base;
+    f.angle_grad = tp->angle_grad;
+    f.tat_intermediate = tp->variant == DPE_TAT_INTERMEDIATE;
+    int *keys = (int *)buffer(ctx, B_F_USED0, sizeof(int) * (size_t)std::max(maxN, 1));
+    int *last = (int *)buffer(ctx, B_F_USED1, sizeof(int) * (size_t)std::max(maxN, 1));
+    uint8_t *acc = (uint8_t *)buffer(ctx, B_F_ACC0, maxP);
+    uint32_t *colour = (uint32_t *)buffer(ctx, B_F_CLAIM, sizeof(uint32_t) * maxP);
+    int *flags = (int *)buffer(ctx, B_F_FLAGS, sizeof(int) * maxP);
+    int *index = (int *)buffer(ctx, B_F_INDEX, sizeof(int) * maxP);
+    dpe_point *stage = (dpe_point *)buffer(ctx, B_F_POINTS, sizeof(dpe_point) * (size_t)maxP);
+    int64_t count = 0;
+    for (int i = 0; i < nv; ++i) {
+        const int P = fv[i].width * fv[i].height;
+        const int ns = fv[i].num_src;
+        if (ns < 2) continue;  // k runs over 2..num_src (DPE.cpp:1684)
+        // scan workspace sized for this view's two scans
+        size_t scan_bytes = 0, last_bytes = 0;
+        HIPCHK(fuse_scan_temp_bytes(P, &scan_bytes));
+        HIPCHK(tat_scan_temp_bytes(ns * P, &last_bytes));
+        const size_t temp_bytes = std::max(scan_bytes, last_bytes);
+        void *temp = buffer(ctx, B_F_TEMP, temp_bytes);
+        launch_tat_keys(f, i, P, keys, s);
+        HIPCHK(hipGetLastError());
+        HIPCHK(launch_tat_last(keys, last, ns * P, temp, temp_bytes, s));
+        launch_tat_decide(f, i, P, last, acc, colour, s);
+        HIPCHK(hipGetLastError());
+        HIPCHK(launch_tat_compact(f, i, P, acc, colour, flags, index, temp, temp_bytes, stage, s));
+        const int64_t n = fuse_count(ctx, index, flags, P);
+        fuse_emit(ctx, stage, n, points, capacity, count, dev);
         count += n;
     }
     *num_points = count;
@@ -1163,6 +1259,17 @@ dpe_status dpe_fuse(dpe_ctx *ctx, const dpe_fuse_view *views, int num_views, con
     if (!ctx) return DPE_ERR_INVALID_ARGUMENT;
     try {
         return fuse_impl(ctx, views, num_views, params, points, capacity, num_points);
+    } catch (const Fail &f) {
+        return f.st;
+    }
+}
+
+dpe_status dpe_fuse_tat(dpe_ctx *ctx, const dpe_fuse_view *views, int num_views, const uint8_t *const *blocks,
+                        const dpe_fuse_tat_params *params, dpe_point *points, int64_t capacity,
+                        int64_t *num_points) {
+    if (!ctx) return DPE_ERR_INVALID_ARGUMENT;
+    try {
+        return fuse_tat_impl(ctx, views, num_views, blocks, params, points, capacity, num_points);
     } catch (const Fail &f) {
         return f.st;
     }

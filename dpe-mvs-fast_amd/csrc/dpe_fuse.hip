@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <cfloat>
+
 #include "dpe_fuse.h"
 
 namespace dpe {
@@ -197,6 +199,176 @@ hipError_t launch_fuse_compact(const FuseDev &f, int vi, int P, const int *used,
     const hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, flags, index, P, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_fuse_commit, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, used, acc, index, out, base);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ TAT fusions
+// RunFusion_TAT_Intermediate (DPE.cpp:1550-1718) and RunFusion_TAT_advanced
+// (DPE.cpp:1720-1867).  The reference keeps one measurement record per source
+// across the pixels of a reference view and overwrites it only where the source
+// pixel is in bounds, unmasked and has depth (DPE.cpp:1656-1682): the record a
+// pixel p reads is the one taken at the last such pixel q <= p in raster order.
+// Within one view only the view's own mask changes and no view is its own
+// source, so every validity test reads masks fixed before the view starts.
+// The GPU writes per source the key j*P + p where source j is valid at p (-1
+// elsewhere), takes one running maximum over the concatenated sources (a key
+// below j*P in segment j means "no valid pixel yet", FLT_MAX in the reference),
+// recomputes the record at that pixel and decides every pixel in parallel.
+
+// DPE.cpp:1643-1647: the pixel takes part (block >= 128, depth > 0)
+__device__ __forceinline__ bool tat_active(const FuseDev &f, long rp) {
+    if (f.block && f.block[rp] < 128) return false;
+    return f.depth[rp] > 0.0f;
+}
+
+// One source record, DPE.cpp:1656-1682; false where the reference leaves it.
+__device__ __forceinline__ bool tat_measure(const FuseDev &f, const FuseViewDev &rv, int j, int c, int r,
+                                            float ref_depth, long rp, const float3 &X, float &dist, float &rel,
+                                            float &angle, long &sp) {
+    const FuseViewDev &sv = f.views[rv.src[j]];
+    float2 pt;
+    float proj_depth;
+    fuse_project(X, sv.cam, pt, proj_depth);
+    const int src_r = fuse_round(pt.y);
+    const int src_c = fuse_round(pt.x);
+    if (!(src_c >= 0 && src_c < sv.width && src_r >= 0 && src_r < sv.height)) return false;
+    sp = sv.offset + (long)src_r * sv.width + src_c;
+    if (f.mask[sp] == 1) return false;
+    const float src_depth = f.depth[sp];
+    if (src_depth <= 0.0f) return false;
+    const float3 tX = fuse_point_world(src_c, src_r, src_depth, sv.cam);
+    float2 tp;
+    fuse_project(tX, rv.cam, tp, proj_depth);
+    const float dx = (float)c - tp.x, dy = (float)r - tp.y;
+    dist = (float)sqrt((double)dx * (double)dx + (double)dy * (double)dy);
+    rel = fabsf(proj_depth - ref_depth) / ref_depth;
+    const float dot = f.normal[3 * rp] * f.normal[3 * sp] + f.normal[3 * rp + 1] * f.normal[3 * sp + 1] +
+                      f.normal[3 * rp + 2] * f.normal[3 * sp + 2];
+    angle = d_acosf(dot);
+    if (angle != angle) angle = 0.0f;  // GetAngle, DPE.cpp:1375-1385
+    return true;
+}
+
+__global__ void k_tat_keys(FuseDev f, int vi, int *__restrict__ keys) {
+    const FuseViewDev &rv = f.views[vi];
+    const int P = rv.width * rv.height;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int r = p / rv.width, c = p - r * rv.width;
+    const long rp = rv.offset + p;
+    const bool active = tat_active(f, rp);
+    const float ref_depth = f.depth[rp];
+    const float3 X = fuse_point_world(c, r, active ? ref_depth : 1.0f, rv.cam);
+    for (int j = 0; j < rv.num_src; ++j) {
+        float dist, rel, angle;
+        long sp;
+        const bool valid = active && tat_measure(f, rv, j, c, r, ref_depth, rp, X, dist, rel, angle, sp);
+        keys[(size_t)j * P + p] = valid ? j * P + p : -1;
+    }
+}
+
+// k-of-n decision, DPE.cpp:1684-1710 (intermediate) and 1836-1855 (advanced).
+// Source j passes at k when its record is under all thresholds of k; bit k-2 of
+// pass[j] holds that.  The first k with >= k passing sources accepts.
+__global__ void k_tat_decide(FuseDev f, int vi, const int *__restrict__ last, uint8_t *__restrict__ acc,
+                             uint32_t *__restrict__ colour) {
+    const FuseViewDev &rv = f.views[vi];
+    const int P = rv.width * rv.height;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const long rp = rv.offset + p;
+    uint8_t accepted = 0;
+    uint32_t bgr = 0;
+    if (tat_active(f, rp)) {
+        const int ns = rv.num_src;
+        uint32_t pass[kMaxImages];
+        uint32_t scol[kMaxImages];
+        for (int j = 0; j < ns; ++j) {
+            pass[j] = 0;
+            scol[j] = 0;
+            const int q = last[(size_t)j * P + p];
+            if (q < j * P) continue;
+            const int qp = q - j * P;
+            const int qr = qp / rv.width, qc = qp - qr * rv.width;
+            const long qrp = rv.offset + qp;
+            const float qd = f.depth[qrp];
+            const float3 X = fuse_point_world(qc, qr, qd, rv.cam);
+            float dist = FLT_MAX, rel = FLT_MAX, angle = FLT_MAX;
+            long sp = 0;
+            tat_measure(f, rv, j, qc, qr, qd, qrp, X, dist, rel, angle, sp);  // valid: the key says so
+            for (int k = 2; k <= ns; ++k) {
+                const bool ok = dist < (float)k * f.dist_base && rel < (float)k * f.depth_base &&
+                                (!f.tat_intermediate || angle < ((float)k * f.angle_grad + f.angle_base));
+                pass[j] |= (ok ? 1u : 0u) << (k - 2);
+            }
+            scol[j] = (uint32_t)f.color[3 * sp] | ((uint32_t)f.color[3 * sp + 1] << 8) |
+                      ((uint32_t)f.color[3 * sp + 2] << 16);
+        }
+        for (int k = 2; k <= ns; ++k) {
+            int n = 0;
+            for (int j = 0; j < ns; ++j) n += (pass[j] >> (k - 2)) & 1u;
+            if (n < k) continue;
+            float col[3] = {(float)f.color[3 * rp], (float)f.color[3 * rp + 1], (float)f.color[3 * rp + 2]};
+            if (f.tat_intermediate) {
+                for (int j = 0; j < ns; ++j) {
+                    if (!((pass[j] >> (k - 2)) & 1u)) continue;
+                    col[0] += (float)(scol[j] & 0xFF);
+                    col[1] += (float)((scol[j] >> 8) & 0xFF);
+                    col[2] += (float)((scol[j] >> 16) & 0xFF);
+                }
+                col[0] /= (n + 1.0f);
+                col[1] /= (n + 1.0f);
+                col[2] /= (n + 1.0f);
+            }
+            bgr = (uint32_t)(uint8_t)col[0] | ((uint32_t)(uint8_t)col[1] << 8) | ((uint32_t)(uint8_t)col[2] << 16);
+            accepted = 1;
+            break;
+        }
+    }
+    acc[p] = accepted;
+    colour[p] = bgr;
+    if (accepted) f.mask[rp] = 1;  // DPE.cpp:1707; read only by later views
+}
+
+__global__ void k_tat_commit(FuseDev f, int vi, const uint8_t *__restrict__ acc, const uint32_t *__restrict__ colour,
+                             const int *__restrict__ index, dpe_point *__restrict__ out) {
+    const FuseViewDev &rv = f.views[vi];
+    const int P = rv.width * rv.height;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || !acc[p]) return;
+    const int r = p / rv.width, c = p - r * rv.width;
+    const float3 X = fuse_point_world(c, r, f.depth[rv.offset + p], rv.cam);
+    dpe_point o;
+    o.x = X.x;
+    o.y = X.y;
+    o.z = X.z;
+    o.b = (uint8_t)(colour[p] & 0xFF);
+    o.g = (uint8_t)((colour[p] >> 8) & 0xFF);
+    o.r = (uint8_t)((colour[p] >> 16) & 0xFF);
+    o.pad = 0;
+    out[index[p]] = o;
+}
+
+void launch_tat_keys(const FuseDev &f, int vi, int P, int *keys, hipStream_t s) {
+    hipLaunchKernelGGL(k_tat_keys, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, keys);
+}
+hipError_t tat_scan_temp_bytes(int n, size_t *bytes) {
+    *bytes = 0;
+    return hipcub::DeviceScan::InclusiveScan(nullptr, *bytes, (const int *)nullptr, (int *)nullptr, hipcub::Max(), n);
+}
+hipError_t launch_tat_last(const int *keys, int *last, int n, void *temp, size_t temp_bytes, hipStream_t s) {
+    return hipcub::DeviceScan::InclusiveScan(temp, temp_bytes, keys, last, hipcub::Max(), n, s);
+}
+void launch_tat_decide(const FuseDev &f, int vi, int P, const int *last, uint8_t *acc, uint32_t *colour,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_tat_decide, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, last, acc, colour);
+}
+hipError_t launch_tat_compact(const FuseDev &f, int vi, int P, const uint8_t *acc, const uint32_t *colour, int *flags,
+                              int *index, void *temp, size_t temp_bytes, dpe_point *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_fuse_flags, dim3((P + 255) / 256), dim3(256), 0, s, acc, flags, P);
+    const hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, flags, index, P, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tat_commit, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, acc, colour, index, out);
     return hipGetLastError();
 }
 

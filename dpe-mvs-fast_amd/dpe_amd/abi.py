@@ -52,7 +52,7 @@ class Outputs(C.Structure):
 
 NCC_FAMILIES = 4
 FAMILY_FULL, FAMILY_REFINE, FAMILY_POST, FAMILY_WEAK = range(NCC_FAMILIES)
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 
 class Stats(C.Structure):
@@ -83,6 +83,14 @@ class FuseParams(C.Structure):
                 ("memory", C.c_int32)]
 
 
+TAT_INTERMEDIATE, TAT_ADVANCED = 0, 1
+
+
+class FuseTatParams(C.Structure):
+    _fields_ = [("variant", C.c_int32), ("dist_base", C.c_float), ("depth_base", C.c_float),
+                ("angle_base", C.c_float), ("angle_grad", C.c_float), ("memory", C.c_int32)]
+
+
 class Point(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("b", C.c_uint8), ("g", C.c_uint8),
                 ("r", C.c_uint8), ("pad", C.c_uint8)]
@@ -108,6 +116,18 @@ def fuse_defaults(**kw):
     """RunFusion thresholds (DPE.cpp:1508, 1518-1519)."""
     f = FuseParams(max_reproj_px=2.0, max_rel_depth=0.01, max_angle_rad=0.174533, strong_factor=0.3,
                    weak_factor=0.45, min_consistent=1, memory=MEM_HOST)
+    for k, v in kw.items():
+        setattr(f, k, v)
+    return f
+
+
+def fuse_tat_defaults(variant, **kw):
+    """RunFusion_TAT_Intermediate / RunFusion_TAT_advanced constants (DPE.cpp:1555-1559, 1725-1726),
+    with 1.0f / 3500.0f and 1.0f / 3000.0f evaluated in float as the reference does."""
+    import numpy as np
+    depth_base = np.float32(1.0) / np.float32(3500.0 if variant == TAT_INTERMEDIATE else 3000.0)
+    f = FuseTatParams(variant=variant, dist_base=0.25, depth_base=float(depth_base),
+                      angle_base=0.06981317007977318, angle_grad=0.05235987755982988, memory=MEM_HOST)
     for k, v in kw.items():
         setattr(f, k, v)
     return f
@@ -160,6 +180,9 @@ def declare(lib):
     lib.dpe_fuse.argtypes = [C.c_void_p, P(FuseView), C.c_int, P(FuseParams), C.c_void_p, C.c_int64,
                              P(C.c_int64)]
     lib.dpe_fuse.restype = C.c_int
+    lib.dpe_fuse_tat.argtypes = [C.c_void_p, P(FuseView), C.c_int, C.c_void_p, P(FuseTatParams), C.c_void_p,
+                                 C.c_int64, P(C.c_int64)]
+    lib.dpe_fuse_tat.restype = C.c_int
     lib.dpe_batch_create.argtypes = [C.c_int, C.c_int, P(C.c_void_p)]
     lib.dpe_batch_create.restype = C.c_int
     lib.dpe_batch_destroy.argtypes = [C.c_void_p]
@@ -175,5 +198,5 @@ def declare(lib):
 # Every symbol include/dpe_hip.h declares.
 EXPORTS = ["dpe_create", "dpe_destroy", "dpe_last_error", "dpe_abi_version", "dpe_process", "dpe_process_ex",
            "dpe_last_stats", "dpe_ncc_cost", "dpe_ncc_cost_ex", "dpe_ncc_cost_adjoint", "dpe_fuse",
-           "dpe_batch_create", "dpe_batch_destroy", "dpe_batch_last_error", "dpe_batch_process"]
+           "dpe_fuse_tat", "dpe_batch_create", "dpe_batch_destroy", "dpe_batch_last_error", "dpe_batch_process"]
 MAX_LANES = 16

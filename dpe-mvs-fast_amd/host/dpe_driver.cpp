@@ -5,7 +5,8 @@
 //
 //   dpe_driver input_folder [gpu_index] [--output=folder] [--vis=none|final|all]
 //              [--checkpoint=none|final|all] [--profile=on|off]
-//              [--schedule=file] [--support=own|reference] [--dry-run]
+//              [--schedule=file] [--support=own|reference]
+//              [--fusion=eth|tat_intermediate|tat_advanced] [--dry-run]
 //
 // Input folder: images/%08d.jpg (sequential 8-bit JPEG, decoded as libjpeg
 // does for cv::imread), .pgm (8-bit gray) or .ppm (8-bit colour), cams/%08d_cam.txt,
@@ -21,6 +22,9 @@
 // --support=reference computes the edge and label maps with the reference's
 // EdgeSegment pipeline (Canny edges, Roberts + Hough + Connect labels;
 // dpe_host_edge_segment), --support=own (default) with dpe_host_support_maps.
+// --fusion selects RunFusion (eth, default; DPE.cpp:1387-1548) or one of the
+// k-of-n Tanks-and-Temples fusions (RunFusion_TAT_Intermediate / _advanced,
+// DPE.cpp:1550-1867), which read blocks/mask_<id>.jpg when that folder exists.
 // --dry-run prints the problems and every call of the schedule, then exits
 // without touching a GPU.
 #include <sys/stat.h>
@@ -55,6 +59,7 @@ struct Options {
     bool profile = true;
     bool dry_run = false;
     bool reference_support = false;
+    int fusion = -1;  // -1 RunFusion, else DPE_TAT_INTERMEDIATE / DPE_TAT_ADVANCED
     std::string output, schedule;
 };
 
@@ -602,7 +607,8 @@ struct Driver {
                   << ", GPU " << st.gpu_ms << " ms)" << std::endl;
     }
 
-    // RunFusion (DPE.cpp:1387-1548) through dpe_fuse, then ExportPointCloud
+    // RunFusion (DPE.cpp:1387-1548) through dpe_fuse, or a TAT fusion
+    // (DPE.cpp:1550-1867) through dpe_fuse_tat, then ExportPointCloud
     int64_t fuse(int min_consistent) {
         std::unordered_map<int, int> index;
         for (const Problem &p : problems) index.emplace(p.ref, p.index);
@@ -656,17 +662,53 @@ struct Driver {
             }
             total += (size_t)vs.w * vs.h;
         }
-        dpe_fuse_params fp{};
-        fp.max_reproj_px = 2.0f;
-        fp.max_rel_depth = 0.01f;
-        fp.max_angle_rad = 0.174533f;
-        fp.strong_factor = 0.3f;
-        fp.weak_factor = 0.45f;
-        fp.min_consistent = min_consistent;
-        fp.memory = DPE_MEM_HOST;
         std::vector<dpe_point> points(total);
         int64_t count = 0;
-        check(dpe_fuse(ctx, views.data(), n, &fp, points.data(), (int64_t)points.size(), &count), "dpe_fuse");
+        if (opt.fusion < 0) {
+            dpe_fuse_params fp{};
+            fp.max_reproj_px = 2.0f;
+            fp.max_rel_depth = 0.01f;
+            fp.max_angle_rad = 0.174533f;
+            fp.strong_factor = 0.3f;
+            fp.weak_factor = 0.45f;
+            fp.min_consistent = min_consistent;
+            fp.memory = DPE_MEM_HOST;
+            check(dpe_fuse(ctx, views.data(), n, &fp, points.data(), (int64_t)points.size(), &count), "dpe_fuse");
+        } else {
+            // blocks/mask_<id>.jpg, read as cv::imread(IMREAD_GRAYSCALE) and indexed
+            // at the depth map's (r, c) (DPE.cpp:1574-1600, 1643)
+            std::vector<std::vector<uint8_t>> blocks;
+            std::vector<const uint8_t *> block_ptrs;
+            if (exists(dense + "/blocks")) {
+                for (int i = 0; i < n; ++i) {
+                    const std::string bp = dense + "/blocks/mask_" + std::to_string(problems[i].ref) + ".jpg";
+                    int bw = 0, bh = 0;
+                    std::vector<uint8_t> img;
+                    if (dpe_host_read_jpeg(bp.c_str(), 0, nullptr, 0, &bw, &bh) == 0) {
+                        img.resize((size_t)bw * bh);
+                        if (dpe_host_read_jpeg(bp.c_str(), 0, img.data(), img.size(), &bw, &bh) != 0) img.clear();
+                    }
+                    if (img.empty()) throw std::runtime_error("Cannot read block mask: " + bp);
+                    if (bw < views[i].width || bh < views[i].height)
+                        throw std::runtime_error("Block mask smaller than the depth map: " + bp);
+                    std::vector<uint8_t> b((size_t)views[i].width * views[i].height);
+                    for (int r = 0; r < views[i].height; ++r)
+                        std::memcpy(&b[(size_t)r * views[i].width], &img[(size_t)r * bw], views[i].width);
+                    blocks.push_back(std::move(b));
+                }
+                for (const auto &b : blocks) block_ptrs.push_back(b.data());
+            }
+            dpe_fuse_tat_params tp{};
+            tp.variant = opt.fusion;
+            tp.dist_base = 0.25f;
+            tp.depth_base = opt.fusion == DPE_TAT_INTERMEDIATE ? 1.0f / 3500.0f : 1.0f / 3000.0f;
+            tp.angle_base = 0.06981317007977318f;
+            tp.angle_grad = 0.05235987755982988f;
+            tp.memory = DPE_MEM_HOST;
+            check(dpe_fuse_tat(ctx, views.data(), n, block_ptrs.empty() ? nullptr : block_ptrs.data(), &tp,
+                               points.data(), (int64_t)points.size(), &count),
+                  "dpe_fuse_tat");
+        }
         const std::string ply = output + "/DPE.ply";
         if (dpe_host_write_ply(ply.c_str(), points.data(), count) != 0) throw std::runtime_error("Cannot write " + ply);
         std::cout << "[DPE Progress] fusion done; point cloud saved to " << ply << " with " << count << " points"
@@ -699,6 +741,9 @@ Options parse_options(int argc, char **argv) {
         else if (a == "--dry-run") o.dry_run = true;
         else if (a == "--support=reference") o.reference_support = true;
         else if (a == "--support=own") o.reference_support = false;
+        else if (a == "--fusion=eth") o.fusion = -1;
+        else if (a == "--fusion=tat_intermediate") o.fusion = DPE_TAT_INTERMEDIATE;
+        else if (a == "--fusion=tat_advanced") o.fusion = DPE_TAT_ADVANCED;
         else if (!gpu_set && !a.empty() && a[0] != '-') {
             o.gpu = std::atoi(a.c_str());
             gpu_set = true;
@@ -713,7 +758,7 @@ int main(int argc, char **argv) {
     if (argc < 2) {
         std::cerr << "USAGE: dpe_driver input_folder [gpu_index] [--output=output_folder] [--vis=none|final|all] "
                      "[--checkpoint=none|final|all] [--profile=on|off] [--schedule=file] "
-                     "[--support=own|reference] [--dry-run]\n";
+                     "[--support=own|reference] [--fusion=eth|tat_intermediate|tat_advanced] [--dry-run]\n";
         return EXIT_FAILURE;
     }
     Driver d;
@@ -756,7 +801,10 @@ int main(int argc, char **argv) {
                           << " iters=" << c.iters << " apd=" << c.apd << " edge=" << c.edge << " geom=" << c.geom
                           << " peak=" << c.peak << " ransac=" << c.ransac << " rotate=" << c.rotate
                           << " final=" << (c.final_call ? 1 : 0) << "\n";
-            std::cout << "fusion min_consistent=" << min_consistent << " max_scale_size=" << d.max_scale_size
+            std::cout << "fusion "
+                      << (d.opt.fusion < 0 ? "eth" : d.opt.fusion == DPE_TAT_INTERMEDIATE ? "tat_intermediate"
+                                                                                         : "tat_advanced")
+                      << " min_consistent=" << min_consistent << " max_scale_size=" << d.max_scale_size
                       << std::endl;
             return EXIT_SUCCESS;
         }

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DPE_ABI_VERSION 8
+#define DPE_ABI_VERSION 9
 /* main.h:41-44 */
 #define DPE_MAX_IMAGES 32
 #define DPE_NEIGHBOUR_NUM 9
@@ -267,6 +267,36 @@ typedef struct dpe_point {
 dpe_status dpe_fuse(dpe_ctx *ctx, const dpe_fuse_view *views, int num_views,
                     const dpe_fuse_params *params, dpe_point *points, int64_t capacity,
                     int64_t *num_points);
+
+/* Tanks-and-Temples fusions, k-of-n consistency: RunFusion_TAT_Intermediate
+ * (DPE.cpp:1550-1718) and RunFusion_TAT_advanced (DPE.cpp:1720-1867).  Views
+ * as for dpe_fuse (pixel_states unused, may be NULL).  A pixel of view i with
+ * depth > 0 (and block >= 128 when blocks are given) keeps, per source, the
+ * last measurement (reprojection error, relative depth, normal angle) taken at
+ * this or an earlier pixel of the view in raster order where the source pixel
+ * was in bounds, unmasked and had depth > 0; for k = 2..num_src it counts the
+ * sources under k*dist_base, k*depth_base (and, intermediate only,
+ * k*angle_grad + angle_base) and emits its back-projection at the first k
+ * with count >= k, then masks itself in view i.  Colour: intermediate
+ * averages the reference and counted source pixels, advanced takes the
+ * reference pixel.
+ *   blocks: NULL, or num_views pointers to width x height u8 maps (the
+ *     reference's blocks/mask_<id>.jpg read at the depth map's size; a NULL
+ *     entry lets every pixel of that view through). */
+#define DPE_TAT_INTERMEDIATE 0
+#define DPE_TAT_ADVANCED 1
+typedef struct dpe_fuse_tat_params {
+    int32_t variant;          /* DPE_TAT_* */
+    float dist_base;          /* 0.25 (DPE.cpp:1555, 1725) */
+    float depth_base;         /* 1/3500 intermediate (DPE.cpp:1556), 1/3000 advanced (DPE.cpp:1726) */
+    float angle_base;         /* 0.06981317 rad, intermediate only (DPE.cpp:1558) */
+    float angle_grad;         /* 0.05235988 rad, intermediate only (DPE.cpp:1559) */
+    int32_t memory;           /* dpe_memory of the view pointers, blocks and `points` */
+} dpe_fuse_tat_params;
+
+dpe_status dpe_fuse_tat(dpe_ctx *ctx, const dpe_fuse_view *views, int num_views, const uint8_t *const *blocks,
+                        const dpe_fuse_tat_params *params, dpe_point *points, int64_t capacity,
+                        int64_t *num_points);
 
 /* Batched form (no counterpart in the reference, whose main.cpp processes one
  * problem at a time): many independent problems of uneven image size and

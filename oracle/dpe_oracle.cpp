@@ -2328,6 +2328,106 @@ int oracle_fuse(const dpe_fuse_view *views, int num_views, const dpe_fuse_params
     return DPE_OK;
 }
 
+// RunFusion_TAT_Intermediate (DPE.cpp:1550-1718) and RunFusion_TAT_advanced
+// (DPE.cpp:1720-1867), sequential: views in pair order, pixels in raster order,
+// one CostData record per source kept across the pixels of a view (DPE.cpp:1637,
+// 1788) and overwritten only where the source pixel passes its checks.
+int oracle_fuse_tat(const dpe_fuse_view *views, int num_views, const uint8_t *const *blocks,
+                    const dpe_fuse_tat_params *tp, dpe_point *points, int64_t capacity, int64_t *num_points) {
+    if (!views || num_views <= 0 || !tp || !num_points) return DPE_ERR_INVALID_ARGUMENT;
+    if (tp->variant != DPE_TAT_INTERMEDIATE && tp->variant != DPE_TAT_ADVANCED) return DPE_ERR_INVALID_ARGUMENT;
+    const bool inter = tp->variant == DPE_TAT_INTERMEDIATE;
+    struct CostData {
+        float dist = FLT_MAX, depth = FLT_MAX, angle = FLT_MAX;
+        int src_r = 0, src_c = 0;
+        bool use = false;
+    };
+    std::vector<std::vector<uint8_t>> masks(num_views);
+    for (int i = 0; i < num_views; ++i) masks[i].assign((size_t)views[i].width * views[i].height, 0);
+    int64_t count = 0;
+    for (int i = 0; i < num_views; ++i) {
+        const dpe_fuse_view &rv = views[i];
+        const int cols = rv.width, rows = rv.height;
+        const int num_ngb = rv.num_src;
+        const uint8_t *block = blocks ? blocks[i] : nullptr;
+        std::vector<CostData> diff(num_ngb);
+        for (int r = 0; r < rows; ++r) {
+            for (int c = 0; c < cols; ++c) {
+                const size_t rp = (size_t)r * cols + c;
+                if (block && block[rp] < 128) continue;
+                const float ref_depth = rv.depth[rp];
+                if (ref_depth <= 0.0) continue;
+                const float *rn = rv.normal + 3 * rp;
+                const F3 PointX = FuseGet3DPointonWorld(c, r, ref_depth, rv.camera);
+                for (int j = 0; j < num_ngb; ++j) {
+                    const int si = rv.src_index[j];
+                    const dpe_fuse_view &sv = views[si];
+                    F2 pt;
+                    float proj_depth;
+                    ProjectonCamera_cu(PointX, sv.camera, pt, proj_depth);
+                    const int src_r = fuse_round(pt.y);
+                    const int src_c = fuse_round(pt.x);
+                    if (src_c >= 0 && src_c < sv.width && src_r >= 0 && src_r < sv.height) {
+                        const size_t sp = (size_t)src_r * sv.width + src_c;
+                        if (masks[si][sp] == 1) continue;
+                        const float src_depth = sv.depth[sp];
+                        if (src_depth <= 0.0) continue;
+                        const float *sn = sv.normal + 3 * sp;
+                        const F3 tX = FuseGet3DPointonWorld(src_c, src_r, src_depth, sv.camera);
+                        F2 tp2;
+                        ProjectonCamera_cu(tX, rv.camera, tp2, proj_depth);
+                        const float dx = (float)c - tp2.x, dy = (float)r - tp2.y;
+                        diff[j].dist = (float)std::sqrt((double)dx * (double)dx + (double)dy * (double)dy);
+                        diff[j].depth = std::fabs(proj_depth - ref_depth) / ref_depth;
+                        const float angle = o_acosf(rn[0] * sn[0] + rn[1] * sn[1] + rn[2] * sn[2]);
+                        diff[j].angle = angle != angle ? 0.0f : angle;  // GetAngle, DPE.cpp:1375-1385
+                        diff[j].src_r = src_r;
+                        diff[j].src_c = src_c;
+                    }
+                }
+                for (int k = 2; k <= num_ngb; ++k) {
+                    int n = 0;
+                    for (int j = 0; j < num_ngb; ++j) {
+                        diff[j].use = diff[j].dist < k * tp->dist_base && diff[j].depth < k * tp->depth_base &&
+                                      (!inter || diff[j].angle < (k * tp->angle_grad + tp->angle_base));
+                        n += diff[j].use;
+                    }
+                    if (n < k) continue;
+                    float col[3] = {(float)rv.color[3 * rp], (float)rv.color[3 * rp + 1], (float)rv.color[3 * rp + 2]};
+                    if (inter) {
+                        for (int j = 0; j < num_ngb; ++j) {
+                            if (!diff[j].use) continue;
+                            const dpe_fuse_view &sv = views[rv.src_index[j]];
+                            const size_t sp = (size_t)diff[j].src_r * sv.width + diff[j].src_c;
+                            col[0] += (float)sv.color[3 * sp];
+                            col[1] += (float)sv.color[3 * sp + 1];
+                            col[2] += (float)sv.color[3 * sp + 2];
+                        }
+                        col[0] /= (n + 1.0f);
+                        col[1] /= (n + 1.0f);
+                        col[2] /= (n + 1.0f);
+                    }
+                    if (points && count < capacity) {
+                        dpe_point &o = points[count];
+                        o.x = PointX.x;
+                        o.y = PointX.y;
+                        o.z = PointX.z;
+                        o.b = (uint8_t)col[0];
+                        o.g = (uint8_t)col[1];
+                        o.r = (uint8_t)col[2];
+                        o.pad = 0;
+                    }
+                    count++;
+                    masks[i][rp] = 1;
+                    break;
+                }
+            }
+        }
+    }
+    *num_points = count;
+    return DPE_OK;
+}
+
 // Numerics test access.
 float oracle_expf(float x) { return o_expf(x); }
 float oracle_acosf(float x) { return o_acosf(x); }

@@ -19,6 +19,9 @@ int oracle_ncc_cost(const dpe_params *params, const dpe_inputs *in, int num_hyps
 /* RunFusion, DPE.cpp:1387-1548. */
 int oracle_fuse(const dpe_fuse_view *views, int num_views, const dpe_fuse_params *fp, dpe_point *points,
                 int64_t capacity, int64_t *num_points);
+/* RunFusion_TAT_Intermediate / RunFusion_TAT_advanced, DPE.cpp:1550-1867 (layout of dpe_fuse_tat). */
+int oracle_fuse_tat(const dpe_fuse_view *views, int num_views, const uint8_t *const *blocks,
+                    const dpe_fuse_tat_params *tp, dpe_point *points, int64_t capacity, int64_t *num_points);
 float oracle_expf(float x);
 double oracle_exp(double x);
 float oracle_acosf(float x);

@@ -32,6 +32,43 @@ class FuseSet:
         self.n = n
 
 
+POINT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("b", "u1"), ("g", "u1"), ("r", "u1"),
+                        ("pad", "u1")])
+
+
+def _blocks_arg(fs, blocks):
+    """NULL, or an array of per-view u8 block pointers (None entries = no block)."""
+    if blocks is None:
+        return None
+    arr = (C.c_void_p * fs.n)()
+    for i, b in enumerate(blocks):
+        if b is not None:
+            b = np.ascontiguousarray(b, np.uint8)
+            fs.keep.append(b)
+            arr[i] = b.ctypes.data
+    fs.keep.append(arr)
+    return C.cast(arr, C.c_void_p)
+
+
+def fuse_tat_oracle(orc, fs, tp, blocks=None):
+    bl = _blocks_arg(fs, blocks)
+    cnt = C.c_int64()
+    assert orc.oracle_fuse_tat(fs.views, fs.n, bl, C.byref(tp), None, 0, C.byref(cnt)) == 0
+    pts = (abi.Point * max(cnt.value, 1))()
+    assert orc.oracle_fuse_tat(fs.views, fs.n, bl, C.byref(tp), pts, cnt.value, C.byref(cnt)) == 0
+    return np.frombuffer(pts, dtype=POINT_DTYPE)[:cnt.value]
+
+
+def fuse_tat_hip(ctx, fs, tp, blocks=None):
+    bl = _blocks_arg(fs, blocks)
+    cnt = C.c_int64()
+    ctx.check(ctx.lib.dpe_fuse_tat(ctx.h, fs.views, fs.n, bl, C.byref(tp), None, 0, C.byref(cnt)))
+    pts = (abi.Point * max(cnt.value, 1))()
+    ctx.check(ctx.lib.dpe_fuse_tat(ctx.h, fs.views, fs.n, bl, C.byref(tp), C.cast(pts, C.c_void_p), cnt.value,
+                                   C.byref(cnt)))
+    return np.frombuffer(pts, dtype=POINT_DTYPE)[:cnt.value]
+
+
 def fuse_oracle(orc, fs, fp):
     cnt = C.c_int64()
     assert orc.oracle_fuse(fs.views, fs.n, C.byref(fp), None, 0, C.byref(cnt)) == 0
@@ -81,4 +118,41 @@ def coincident_pair(depth1_scale_rows=None, W=40, H=30):
         v.depth, v.normal, v.pixel_states, v.color = d.ctypes.data, nrm.ctypes.data, st.ctypes.data, col.ctypes.data
         v.num_src = 1
         v.src_index[0] = 1 - i
+    return fs, K
+
+
+def coincident_views(n, W=40, H=30, depth_scale=None, tilt=None, color_b=None, depth_zero_rows=None):
+    """n views with one camera looking at the plane z = 4, each listing all the
+    others (ascending) as sources; normals (0, 0, -1), colour 100 + i.
+    depth_scale: {view: factor} on the depth map; tilt: {view: angle} rotates
+    that view's normals about y; color_b: {view: (H, W) u8} blue channel;
+    depth_zero_rows: {view: row slice} with depth 0 (holes)."""
+    K = np.array([[50.0, 0, (W - 1) / 2], [0, 50.0, (H - 1) / 2], [0, 0, 1]])
+    cam = abi.camera_from_arrays(K, np.eye(3), np.zeros(3), W, H, 2, 6)
+    fs = _Views()
+    fs.keep = []
+    fs.views = (abi.FuseView * n)()
+    fs.n = n
+    for i in range(n):
+        d = np.full((H, W), 4.0, np.float32)
+        if depth_scale and i in depth_scale:
+            d *= np.float32(depth_scale[i])
+        if depth_zero_rows and i in depth_zero_rows:
+            d[depth_zero_rows[i]] = 0.0
+        nrm = np.zeros((H, W, 3), np.float32)
+        th = (tilt or {}).get(i, 0.0)
+        nrm[..., 0] = np.float32(np.sin(th))
+        nrm[..., 2] = np.float32(-np.cos(th))
+        st = np.full((H, W), abi.STRONG, np.uint8)
+        col = np.full((H, W, 3), 100 + i, np.uint8)
+        if color_b and i in color_b:
+            col[..., 0] = color_b[i]
+        fs.keep += [d, nrm, st, col]
+        v = fs.views[i]
+        v.width, v.height, v.camera = W, H, cam
+        v.depth, v.normal, v.pixel_states, v.color = d.ctypes.data, nrm.ctypes.data, st.ctypes.data, col.ctypes.data
+        srcs = [j for j in range(n) if j != i]
+        v.num_src = len(srcs)
+        for j, s in enumerate(srcs):
+            v.src_index[j] = s
     return fs, K

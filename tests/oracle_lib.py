@@ -28,6 +28,9 @@ def load_oracle():
     lib.oracle_ncc_cost.restype = C.c_int
     lib.oracle_fuse.argtypes = [P(abi.FuseView), C.c_int, P(abi.FuseParams), C.c_void_p, C.c_int64, P(C.c_int64)]
     lib.oracle_fuse.restype = C.c_int
+    lib.oracle_fuse_tat.argtypes = [P(abi.FuseView), C.c_int, C.c_void_p, P(abi.FuseTatParams), C.c_void_p,
+                                    C.c_int64, P(C.c_int64)]
+    lib.oracle_fuse_tat.restype = C.c_int
     lib.oracle_expf.argtypes = [C.c_float]
     lib.oracle_expf.restype = C.c_float
     lib.oracle_exp.argtypes = [C.c_double]

@@ -32,7 +32,7 @@ def test_struct_layouts():
 
 STRUCTS = {"dpe_camera": abi.Camera, "dpe_params": abi.Params, "dpe_inputs": abi.Inputs, "dpe_outputs": abi.Outputs,
            "dpe_stats": abi.Stats, "dpe_fuse_view": abi.FuseView, "dpe_fuse_params": abi.FuseParams,
-           "dpe_point": abi.Point, "dpe_form": abi.Form}
+           "dpe_point": abi.Point, "dpe_form": abi.Form, "dpe_fuse_tat_params": abi.FuseTatParams}
 
 
 def test_ctypes_mirror_matches_the_c_header(tmp_path):
@@ -44,7 +44,7 @@ def test_ctypes_mirror_matches_the_c_header(tmp_path):
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
         for fname, _ in cls._fields_:
             lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
-    for const in ("DPE_ABI_VERSION", "DPE_MAX_IMAGES", "DPE_NCC_FAMILIES", "DPE_FAMILY_WEAK"):
+    for const in ("DPE_ABI_VERSION", "DPE_MAX_IMAGES", "DPE_NCC_FAMILIES", "DPE_FAMILY_WEAK", "DPE_TAT_ADVANCED"):
         lines.append(f'printf("const {const} %d\\n", (int)({const}));')
     lines.append("return 0; }")
     src = tmp_path / "layout.c"
@@ -64,6 +64,7 @@ def test_ctypes_mirror_matches_the_c_header(tmp_path):
     assert got[("const", "DPE_MAX_IMAGES")] == abi.MAX_IMAGES
     assert got[("const", "DPE_NCC_FAMILIES")] == abi.NCC_FAMILIES
     assert got[("const", "DPE_FAMILY_WEAK")] == abi.FAMILY_WEAK
+    assert got[("const", "DPE_TAT_ADVANCED")] == abi.TAT_ADVANCED
 
 
 def test_result_assembly_zeroes_out_of_range_depth():

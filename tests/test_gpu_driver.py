@@ -120,3 +120,52 @@ def test_driver_reads_jpeg_folders(tmp_path):
         assert np.array_equal(hostio.read_bin_mat(str(f / "depths.dmb")), store.get(i)["depth"]), i
     _, verts = hostio.read_ply(str(out / "DPE.ply"))
     assert len(verts) > 0
+
+
+def test_driver_tat_fusions_with_blocks(tmp_path):
+    """--fusion=tat_intermediate / tat_advanced with a blocks/ folder: the PLY
+    equals dpe_fuse_tat on the driver's own final state and on the block masks
+    decoded from blocks/mask_<id>.jpg; a mask smaller than the depth map is an
+    error."""
+    from PIL import Image
+    from fuse_util import fuse_tat_hip
+    s = scene.make_scene(160, 120, 150.0, 4, 24.0, seed=0x5EED0008, supersample=1)
+    folder = tmp_path / "scene"
+    scene.write_folder(s, str(folder))
+    (folder / "blocks").mkdir()
+    yy, xx = np.mgrid[0:120, 0:160]
+    for i in range(4):
+        m = np.where((xx // 20 + yy // 20 + i) % 3 == 0, 0, 255).astype(np.uint8)
+        Image.fromarray(m).save(folder / "blocks" / ("mask_%d.jpg" % i), quality=90)
+    sched = tmp_path / "short.sched"
+    sched.write_text("call scale=1 state=FIRST_INIT iters=3 apd=0 edge=0 geom=0 peak=6\n")
+    hl = hostio.load()
+    blocks = [hostio.read_jpeg(hl, folder / "blocks" / ("mask_%d.jpg" % i)) for i in range(4)]
+    assert 0 < np.mean([(b < 128).mean() for b in blocks]) < 0.6
+    for name, variant in (("tat_intermediate", abi.TAT_INTERMEDIATE), ("tat_advanced", abi.TAT_ADVANCED)):
+        out = tmp_path / ("out_" + name)
+        print(f"driver --fusion={name}: 4 views 160x120 with blocks", flush=True)
+        r = subprocess.run([hostio.DRIVER_PATH, str(folder), "0", "--output=" + str(out), "--checkpoint=final",
+                            "--schedule=" + str(sched), "--fusion=" + name], capture_output=True, text=True,
+                           timeout=300)
+        print(r.stdout[-1500:], r.stderr[-1500:])
+        assert r.returncode == 0
+        views = [out / "views" / ("%08d" % i) for i in range(4)]
+        fs = FuseSet(s, [hostio.read_bin_mat(str(v / "depths.dmb")) for v in views],
+                     [hostio.read_bin_mat(str(v / "normals.dmb")) for v in views],
+                     [hostio.read_bin_mat(str(v / "weak.bin")) for v in views])
+        ctx = host.Context(0)
+        try:
+            pts = fuse_tat_hip(ctx, fs, abi.fuse_tat_defaults(variant), blocks)
+        finally:
+            ctx.close()
+        _, verts = hostio.read_ply(str(out / "DPE.ply"))
+        print(f"--fusion={name}: driver {len(verts)} points, python {len(pts)}")
+        assert len(verts) == len(pts) > 0
+        for k in ("x", "y", "z", "b", "g", "r"):
+            assert np.array_equal(verts[k], pts[k]), k
+    Image.fromarray(np.zeros((100, 160), np.uint8)).save(folder / "blocks" / "mask_2.jpg")
+    r = subprocess.run([hostio.DRIVER_PATH, str(folder), "0", "--output=" + str(tmp_path / "bad"),
+                        "--schedule=" + str(sched), "--fusion=tat_advanced"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode != 0 and "Block mask smaller than the depth map" in r.stderr

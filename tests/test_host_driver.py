@@ -170,6 +170,11 @@ def test_driver_dry_run_reference_schedule(tmp_path):
             assert int(kv["rotate"]) == c["rotate"] and int(kv["final"]) == int(c["final"])
         assert f"max_scale_size={1 << (rounds - 1)}" in out
         assert "problem 0 ref 0 srcs 1 2" in out
+        assert "fusion eth min_consistent=" in out
+    for name in ("tat_intermediate", "tat_advanced"):
+        out = subprocess.run([hostio.DRIVER_PATH, str(folder), "--dry-run", "--fusion=" + name], capture_output=True,
+                             text=True, check=True).stdout
+        assert f"fusion {name} min_consistent=" in out
 
 
 def test_driver_rejects_bad_input(tmp_path):
