@@ -26,6 +26,18 @@ constexpr int kMaxImages = DPE_MAX_IMAGES;
 constexpr int kNeighbourNum = DPE_NEIGHBOUR_NUM;
 constexpr int kMaxTaps = 11;  // taps per window dimension held in registers
 
+// The dispatcher deals workgroups round-robin over the 8 XCDs (workgroup b runs
+// on XCD b % 8), and each XCD has its own L2.  xcd_block(b, nb) renumbers the
+// nb workgroups of a 1-D grid so that XCD x works through one contiguous range
+// of logical workgroups (XCDs 0..nb%8-1 take one more); the mapping is a
+// bijection on [0, nb).  Used where neighbouring workgroups read overlapping
+// data that would otherwise be fetched into eight L2s.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
 // ----------------------------------------------------------------- numerics
 __device__ __forceinline__ float d_expf(float x) {
     if (x != x) return x;

