@@ -182,7 +182,9 @@ def main():
         fv.num_src = len(sc.pairs[i])
         for j, s_ in enumerate(sc.pairs[i]):
             fv.src_index[j] = s_
-    fparams = abi.fuse_defaults(min_consistent=3, memory=abi.MEM_DEVICE)
+    # the precision form also takes fusion's reprojection geometry to FP64
+    fparams = abi.fuse_defaults(min_consistent=3, memory=abi.MEM_DEVICE,
+                                fp64_reprojection=int(args.precision != "plain"))
 
     def step():
         tot = dict(hyps=0, taps=0, gpu_ms=0.0, fam_ms=[0.0] * 4, fam_hyps=[0] * 4, fam_taps=[0] * 4,

@@ -936,6 +936,9 @@ dpe_status fuse_impl(dpe_ctx *ctx, const dpe_fuse_view *views, int nv, const dpe
     f.strong_factor = fp->strong_factor;
     f.weak_factor = fp->weak_factor;
     f.min_consistent = fp->min_consistent;
+    if (fp->fp64_reprojection != 0 && fp->fp64_reprojection != 1)
+        return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "fp64_reprojection must be 0 or 1");
+    f.fp64 = fp->fp64_reprojection;
     // points of one view go to a staging buffer, then to the caller
     dpe_point *stage = (dpe_point *)buffer(ctx, B_F_POINTS, sizeof(dpe_point) * (size_t)maxP);
     int64_t count = 0;

@@ -25,6 +25,7 @@ struct FuseDev {
     int *claim;
     float max_reproj, max_rel, max_angle, strong_factor, weak_factor;
     int min_consistent;
+    int fp64;  // RunFusion's reprojection geometry in FP64 (precision form)
     // TAT fusions (DPE.cpp:1550-1867): block map (u8 per pixel, packed like
     // depth; NULL = no blocks) and the k-of-n thresholds
     const uint8_t *block;

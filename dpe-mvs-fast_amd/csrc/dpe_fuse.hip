@@ -22,43 +22,65 @@
 
 namespace dpe {
 
+// Fusion geometry in scalar type T: float is the reference's arithmetic; double
+// is the precision form's FP64 reprojection check (camera entries widened, the
+// same operations in the same order).
+template <typename T> struct FVec3 { T x, y, z; };
+template <typename T> struct FVec2 { T x, y; };
+
 // Get3DPointonWorld (host), DPE.cpp:1337-1361: the centre comes from R and t.
-__device__ __forceinline__ float3 fuse_point_world(int x, int y, float depth, const dpe_camera &cam) {
-    float3 X, T, Cc;
-    X.x = depth * ((float)x - cam.K[2]) / cam.K[0];
-    X.y = depth * ((float)y - cam.K[5]) / cam.K[4];
+template <typename T>
+__device__ __forceinline__ FVec3<T> fuse_point_world(int x, int y, T depth, const dpe_camera &cam) {
+    const float *R = cam.R, *K = cam.K, *t = cam.t;
+    FVec3<T> X, Tr, Cc;
+    X.x = depth * ((T)x - (T)K[2]) / (T)K[0];
+    X.y = depth * ((T)y - (T)K[5]) / (T)K[4];
     X.z = depth;
-    T.x = cam.R[0] * X.x + cam.R[3] * X.y + cam.R[6] * X.z;
-    T.y = cam.R[1] * X.x + cam.R[4] * X.y + cam.R[7] * X.z;
-    T.z = cam.R[2] * X.x + cam.R[5] * X.y + cam.R[8] * X.z;
-    Cc.x = -(cam.R[0] * cam.t[0] + cam.R[3] * cam.t[1] + cam.R[6] * cam.t[2]);
-    Cc.y = -(cam.R[1] * cam.t[0] + cam.R[4] * cam.t[1] + cam.R[7] * cam.t[2]);
-    Cc.z = -(cam.R[2] * cam.t[0] + cam.R[5] * cam.t[1] + cam.R[8] * cam.t[2]);
-    X.x = T.x + Cc.x;
-    X.y = T.y + Cc.y;
-    X.z = T.z + Cc.z;
+    Tr.x = (T)R[0] * X.x + (T)R[3] * X.y + (T)R[6] * X.z;
+    Tr.y = (T)R[1] * X.x + (T)R[4] * X.y + (T)R[7] * X.z;
+    Tr.z = (T)R[2] * X.x + (T)R[5] * X.y + (T)R[8] * X.z;
+    Cc.x = -((T)R[0] * (T)t[0] + (T)R[3] * (T)t[1] + (T)R[6] * (T)t[2]);
+    Cc.y = -((T)R[1] * (T)t[0] + (T)R[4] * (T)t[1] + (T)R[7] * (T)t[2]);
+    Cc.z = -((T)R[2] * (T)t[0] + (T)R[5] * (T)t[1] + (T)R[8] * (T)t[2]);
+    X.x = Tr.x + Cc.x;
+    X.y = Tr.y + Cc.y;
+    X.z = Tr.z + Cc.z;
     return X;
+}
+__device__ __forceinline__ float3 fuse_point_world(int x, int y, float depth, const dpe_camera &cam) {
+    const FVec3<float> X = fuse_point_world<float>(x, y, depth, cam);
+    return make_float3(X.x, X.y, X.z);
 }
 
 // ProjectCamera, DPE.cpp:1363-1373
+template <typename T>
+__device__ __forceinline__ void fuse_project(const FVec3<T> &X, const dpe_camera &cam, FVec2<T> &pt, T &depth) {
+    const float *R = cam.R, *K = cam.K, *t0 = cam.t;
+    FVec3<T> t;
+    t.x = (T)R[0] * X.x + (T)R[1] * X.y + (T)R[2] * X.z + (T)t0[0];
+    t.y = (T)R[3] * X.x + (T)R[4] * X.y + (T)R[5] * X.z + (T)t0[1];
+    t.z = (T)R[6] * X.x + (T)R[7] * X.y + (T)R[8] * X.z + (T)t0[2];
+    depth = (T)K[6] * t.x + (T)K[7] * t.y + (T)K[8] * t.z;
+    pt.x = ((T)K[0] * t.x + (T)K[1] * t.y + (T)K[2] * t.z) / depth;
+    pt.y = ((T)K[3] * t.x + (T)K[4] * t.y + (T)K[5] * t.z) / depth;
+}
 __device__ __forceinline__ void fuse_project(const float3 &X, const dpe_camera &cam, float2 &pt, float &depth) {
-    float3 t;
-    t.x = cam.R[0] * X.x + cam.R[1] * X.y + cam.R[2] * X.z + cam.t[0];
-    t.y = cam.R[3] * X.x + cam.R[4] * X.y + cam.R[5] * X.z + cam.t[1];
-    t.z = cam.R[6] * X.x + cam.R[7] * X.y + cam.R[8] * X.z + cam.t[2];
-    depth = cam.K[6] * t.x + cam.K[7] * t.y + cam.K[8] * t.z;
-    pt.x = (cam.K[0] * t.x + cam.K[1] * t.y + cam.K[2] * t.z) / depth;
-    pt.y = (cam.K[3] * t.x + cam.K[4] * t.y + cam.K[5] * t.z) / depth;
+    FVec2<float> p;
+    fuse_project<float>(FVec3<float>{X.x, X.y, X.z}, cam, p, depth);
+    pt = make_float2(p.x, p.y);
 }
 
 // int(v + 0.5f) (DPE.cpp:1491-1492); NaN and huge values map to -1.
-__device__ __forceinline__ int fuse_round(float v) {
-    const float r = v + 0.5f;
-    if (!(r > -1.0e9f && r < 1.0e9f)) return -1;
+template <typename T>
+__device__ __forceinline__ int fuse_round(T v) {
+    const T r = v + (T)0.5;
+    if (!(r > (T)-1.0e9 && r < (T)1.0e9)) return -1;
     return (int)r;
 }
 
-// One pixel of one reference view (DPE.cpp:1475-1541 up to the point output).
+// One pixel of one reference view (DPE.cpp:1475-1541 up to the point output);
+// T = double for the precision form's FP64 reprojection check.
+template <typename T>
 __global__ void k_fuse_eval(FuseDev f, int vi, const int *__restrict__ used_old, const uint8_t *__restrict__ acc_old,
                             int *__restrict__ used_new, uint8_t *__restrict__ acc_new, int *__restrict__ changed) {
     const FuseViewDev &rv = f.views[vi];
@@ -73,27 +95,28 @@ __global__ void k_fuse_eval(FuseDev f, int vi, const int *__restrict__ used_old,
     const float ref_depth = f.depth[rp];
     if (f.mask[rp] != 1 && ref_depth > 0.0f) {
         const float rn0 = f.normal[3 * rp], rn1 = f.normal[3 * rp + 1], rn2 = f.normal[3 * rp + 2];
-        const float3 X = fuse_point_world(c, r, ref_depth, rv.cam);
+        const FVec3<T> X = fuse_point_world<T>(c, r, (T)ref_depth, rv.cam);
         int num_consistent = 0;
         float dyn = 0.0f;
         for (int j = 0; j < rv.num_src; ++j) {
             const FuseViewDev &sv = f.views[rv.src[j]];
-            float2 pt;
-            float proj_depth;
-            fuse_project(X, sv.cam, pt, proj_depth);
-            const int src_r = fuse_round(pt.y);
-            const int src_c = fuse_round(pt.x);
+            FVec2<T> pt;
+            T proj_depth;
+            fuse_project<T>(X, sv.cam, pt, proj_depth);
+            const int src_r = fuse_round<T>(pt.y);
+            const int src_c = fuse_round<T>(pt.x);
             if (src_c >= 0 && src_c < sv.width && src_r >= 0 && src_r < sv.height) {
                 const long sp = sv.offset + (long)src_r * sv.width + src_c;
                 if (f.mask[sp] == 1 || f.claim[sp] < p) continue;
                 const float src_depth = f.depth[sp];
                 if (src_depth <= 0.0f) continue;
-                const float3 tX = fuse_point_world(src_c, src_r, src_depth, sv.cam);
-                float2 tp;
-                fuse_project(tX, rv.cam, tp, proj_depth);
-                const float dx = (float)c - tp.x, dy = (float)r - tp.y;
+                const FVec3<T> tX = fuse_point_world<T>(src_c, src_r, (T)src_depth, sv.cam);
+                FVec2<T> tp;
+                fuse_project<T>(tX, rv.cam, tp, proj_depth);
+                // sqrt(pow(c - x, 2) + pow(r - y, 2)) evaluates in double (DPE.cpp:1506)
+                const T dx = (T)c - tp.x, dy = (T)r - tp.y;
                 const float reproj = (float)sqrt((double)dx * (double)dx + (double)dy * (double)dy);
-                const float rel = fabsf(proj_depth - ref_depth) / ref_depth;
+                const float rel = (float)(fabs(proj_depth - (T)ref_depth) / (T)ref_depth);
                 const float dot = rn0 * f.normal[3 * sp] + rn1 * f.normal[3 * sp + 1] + rn2 * f.normal[3 * sp + 2];
                 float angle = d_acosf(dot);
                 if (angle != angle) angle = 0.0f;  // GetAngle, DPE.cpp:1379-1382
@@ -138,6 +161,7 @@ __global__ void k_fuse_claim(FuseDev f, int vi, const int *__restrict__ used, co
 }
 
 // Point output and mask update of the accepted pixels (DPE.cpp:1520-1541).
+template <typename T>
 __global__ void k_fuse_commit(FuseDev f, int vi, const int *__restrict__ used, const uint8_t *__restrict__ acc,
                               const int *__restrict__ index, dpe_point *__restrict__ out, long base) {
     const FuseViewDev &rv = f.views[vi];
@@ -146,7 +170,7 @@ __global__ void k_fuse_commit(FuseDev f, int vi, const int *__restrict__ used, c
     if (p >= P || !acc[p]) return;
     const int r = p / rv.width, c = p - r * rv.width;
     const long rp = rv.offset + p;
-    const float3 X = fuse_point_world(c, r, f.depth[rp], rv.cam);
+    const FVec3<T> X = fuse_point_world<T>(c, r, (T)f.depth[rp], rv.cam);
     float col[3] = {(float)f.color[3 * rp], (float)f.color[3 * rp + 1], (float)f.color[3 * rp + 2]};
     int n = 0;
     for (int j = 0; j < rv.num_src; ++j) {
@@ -163,9 +187,9 @@ __global__ void k_fuse_commit(FuseDev f, int vi, const int *__restrict__ used, c
     col[1] /= (n + 1);
     col[2] /= (n + 1);
     dpe_point o;
-    o.x = X.x;
-    o.y = X.y;
-    o.z = X.z;
+    o.x = (float)X.x;
+    o.y = (float)X.y;
+    o.z = (float)X.z;
     o.b = (uint8_t)col[0];
     o.g = (uint8_t)col[1];
     o.r = (uint8_t)col[2];
@@ -180,8 +204,12 @@ __global__ void k_fuse_flags(const uint8_t *__restrict__ acc, int *__restrict__ 
 
 void launch_fuse_eval(const FuseDev &f, int vi, int P, const int *used_old, const uint8_t *acc_old, int *used_new,
                       uint8_t *acc_new, int *changed, hipStream_t s) {
-    hipLaunchKernelGGL(k_fuse_eval, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, used_old, acc_old, used_new, acc_new,
-                       changed);
+    if (f.fp64)
+        hipLaunchKernelGGL(k_fuse_eval<double>, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, used_old, acc_old,
+                           used_new, acc_new, changed);
+    else
+        hipLaunchKernelGGL(k_fuse_eval<float>, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, used_old, acc_old,
+                           used_new, acc_new, changed);
 }
 void launch_fuse_fill_claims(const FuseDev &f, int vi, int num_src, hipStream_t s) {
     if (num_src > 0) hipLaunchKernelGGL(k_fuse_fill_claims, dim3(1024, num_src), dim3(256), 0, s, f, vi);
@@ -198,7 +226,12 @@ hipError_t launch_fuse_compact(const FuseDev &f, int vi, int P, const int *used,
     hipLaunchKernelGGL(k_fuse_flags, dim3((P + 255) / 256), dim3(256), 0, s, acc, flags, P);
     const hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, flags, index, P, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_fuse_commit, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, used, acc, index, out, base);
+    if (f.fp64)
+        hipLaunchKernelGGL(k_fuse_commit<double>, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, used, acc, index, out,
+                           base);
+    else
+        hipLaunchKernelGGL(k_fuse_commit<float>, dim3((P + 255) / 256), dim3(256), 0, s, f, vi, used, acc, index, out,
+                           base);
     return hipGetLastError();
 }
 

@@ -52,7 +52,7 @@ class Outputs(C.Structure):
 
 NCC_FAMILIES = 4
 FAMILY_FULL, FAMILY_REFINE, FAMILY_POST, FAMILY_WEAK = range(NCC_FAMILIES)
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 
 class Stats(C.Structure):
@@ -80,7 +80,7 @@ class FuseView(C.Structure):
 class FuseParams(C.Structure):
     _fields_ = [("max_reproj_px", C.c_float), ("max_rel_depth", C.c_float), ("max_angle_rad", C.c_float),
                 ("strong_factor", C.c_float), ("weak_factor", C.c_float), ("min_consistent", C.c_int32),
-                ("memory", C.c_int32)]
+                ("memory", C.c_int32), ("fp64_reprojection", C.c_int32)]
 
 
 TAT_INTERMEDIATE, TAT_ADVANCED = 0, 1

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DPE_ABI_VERSION 9
+#define DPE_ABI_VERSION 10
 /* main.h:41-44 */
 #define DPE_MAX_IMAGES 32
 #define DPE_NEIGHBOUR_NUM 9
@@ -255,6 +255,9 @@ typedef struct dpe_fuse_params {
     float weak_factor;        /* 0.45 */
     int32_t min_consistent;   /* 1 in the reference (DPE.cpp:1519) */
     int32_t memory;           /* dpe_memory of the view pointers and of `points` */
+    int32_t fp64_reprojection; /* 0: the reference's fp32 geometry; 1 (precision form): back-projection,
+                                  projection, reprojection error and relative depth in FP64 (points are
+                                  emitted rounded to fp32; the angle and the weighting stay fp32) */
 } dpe_fuse_params;
 
 /* Points: xyz float + bgr uchar, the PLY vertex of ExportPointCloud

@@ -2168,10 +2168,46 @@ F3 FuseGet3DPointonWorld(int x, int y, float depth, const dpe_camera &cam) {
 
 // int(v + 0.5f) of RunFusion (DPE.cpp:1491-1492); NaN and huge values map to -1
 // (outside every image).
-inline int fuse_round(float v) {
-    const float r = v + 0.5f;
-    if (!(r > -1.0e9f && r < 1.0e9f)) return -1;
+template <typename T>
+inline int fuse_round(T v) {
+    const T r = v + (T)0.5;
+    if (!(r > (T)-1.0e9 && r < (T)1.0e9)) return -1;
     return (int)r;
+}
+
+// The fusion geometry in scalar type T (float: the reference; double: the
+// precision form's FP64 reprojection check, camera entries widened, the same
+// operations in the same order).
+template <typename T> struct FV3 { T x, y, z; };
+template <typename T> struct FV2 { T x, y; };
+template <typename T>
+FV3<T> fuse_world(int x, int y, T depth, const dpe_camera &cam) {
+    const float *R = cam.R, *K = cam.K, *t = cam.t;
+    FV3<T> X, Tr, Cc;
+    X.x = depth * ((T)x - (T)K[2]) / (T)K[0];
+    X.y = depth * ((T)y - (T)K[5]) / (T)K[4];
+    X.z = depth;
+    Tr.x = (T)R[0] * X.x + (T)R[3] * X.y + (T)R[6] * X.z;
+    Tr.y = (T)R[1] * X.x + (T)R[4] * X.y + (T)R[7] * X.z;
+    Tr.z = (T)R[2] * X.x + (T)R[5] * X.y + (T)R[8] * X.z;
+    Cc.x = -((T)R[0] * (T)t[0] + (T)R[3] * (T)t[1] + (T)R[6] * (T)t[2]);
+    Cc.y = -((T)R[1] * (T)t[0] + (T)R[4] * (T)t[1] + (T)R[7] * (T)t[2]);
+    Cc.z = -((T)R[2] * (T)t[0] + (T)R[5] * (T)t[1] + (T)R[8] * (T)t[2]);
+    X.x = Tr.x + Cc.x;
+    X.y = Tr.y + Cc.y;
+    X.z = Tr.z + Cc.z;
+    return X;
+}
+template <typename T>
+void fuse_proj(const FV3<T> &X, const dpe_camera &cam, FV2<T> &pt, T &depth) {
+    const float *R = cam.R, *K = cam.K, *t0 = cam.t;
+    FV3<T> t;
+    t.x = (T)R[0] * X.x + (T)R[1] * X.y + (T)R[2] * X.z + (T)t0[0];
+    t.y = (T)R[3] * X.x + (T)R[4] * X.y + (T)R[5] * X.z + (T)t0[1];
+    t.z = (T)R[6] * X.x + (T)R[7] * X.y + (T)R[8] * X.z + (T)t0[2];
+    depth = (T)K[6] * t.x + (T)K[7] * t.y + (T)K[8] * t.z;
+    pt.x = ((T)K[0] * t.x + (T)K[1] * t.y + (T)K[2] * t.z) / depth;
+    pt.y = ((T)K[3] * t.x + (T)K[4] * t.y + (T)K[5] * t.z) / depth;
 }
 }  // namespace
 }  // namespace oracle
@@ -2237,10 +2273,16 @@ int oracle_ncc_cost(const dpe_params *params, const dpe_inputs *in, int num_hyps
     return DPE_OK;
 }
 
-// Fusion, RunFusion DPE.cpp:1387-1548 (sequential, pair order, raster order).
-int oracle_fuse(const dpe_fuse_view *views, int num_views, const dpe_fuse_params *fp, dpe_point *points,
-                int64_t capacity, int64_t *num_points) {
-    if (!views || num_views <= 0 || !fp || !num_points) return DPE_ERR_INVALID_ARGUMENT;
+}  // extern "C"
+
+namespace oracle {
+namespace {
+// Fusion, RunFusion DPE.cpp:1387-1548 (sequential, pair order, raster order);
+// T = double: the precision form's FP64 reprojection geometry (the errors are
+// rounded to fp32 for the threshold tests and the weighting).
+template <typename T>
+int fuse_impl(const dpe_fuse_view *views, int num_views, const dpe_fuse_params *fp, dpe_point *points,
+              int64_t capacity, int64_t *num_points) {
     std::vector<std::vector<uint8_t>> masks(num_views);
     for (int i = 0; i < num_views; ++i) masks[i].assign((size_t)views[i].width * views[i].height, 0);
     int64_t count = 0;
@@ -2255,7 +2297,7 @@ int oracle_fuse(const dpe_fuse_view *views, int num_views, const dpe_fuse_params
                 const float ref_depth = rv.depth[rp];
                 if (ref_depth <= 0.0) continue;
                 const float *rn = rv.normal + 3 * rp;
-                const F3 PointX = FuseGet3DPointonWorld(c, r, ref_depth, rv.camera);
+                const FV3<T> PointX = fuse_world<T>(c, r, (T)ref_depth, rv.camera);
                 int num_consistent = 0;
                 float dyn = 0.0f;
                 int used[DPE_MAX_IMAGES][2];
@@ -2263,24 +2305,24 @@ int oracle_fuse(const dpe_fuse_view *views, int num_views, const dpe_fuse_params
                 for (int j = 0; j < num_ngb; ++j) {
                     const int si = rv.src_index[j];
                     const dpe_fuse_view &sv = views[si];
-                    F2 pt;
-                    float proj_depth;
-                    ProjectonCamera_cu(PointX, sv.camera, pt, proj_depth);
-                    const int src_r = fuse_round(pt.y);
-                    const int src_c = fuse_round(pt.x);
+                    FV2<T> pt;
+                    T proj_depth;
+                    fuse_proj<T>(PointX, sv.camera, pt, proj_depth);
+                    const int src_r = fuse_round<T>(pt.y);
+                    const int src_c = fuse_round<T>(pt.x);
                     if (src_c >= 0 && src_c < sv.width && src_r >= 0 && src_r < sv.height) {
                         const size_t sp = (size_t)src_r * sv.width + src_c;
                         if (masks[si][sp] == 1) continue;
                         const float src_depth = sv.depth[sp];
                         if (src_depth <= 0.0) continue;
                         const float *sn = sv.normal + 3 * sp;
-                        const F3 tX = FuseGet3DPointonWorld(src_c, src_r, src_depth, sv.camera);
-                        F2 tp;
-                        ProjectonCamera_cu(tX, rv.camera, tp, proj_depth);
+                        const FV3<T> tX = fuse_world<T>(src_c, src_r, (T)src_depth, sv.camera);
+                        FV2<T> tp;
+                        fuse_proj<T>(tX, rv.camera, tp, proj_depth);
                         // sqrt(pow(c - x, 2) + pow(r - y, 2)) evaluates in double (DPE.cpp:1506)
-                        const float dx = (float)c - tp.x, dy = (float)r - tp.y;
+                        const T dx = (T)c - tp.x, dy = (T)r - tp.y;
                         const float reproj = (float)std::sqrt((double)dx * (double)dx + (double)dy * (double)dy);
-                        const float rel = std::fabs(proj_depth - ref_depth) / ref_depth;
+                        const float rel = (float)(std::fabs(proj_depth - (T)ref_depth) / (T)ref_depth);
                         const float dot = rn[0] * sn[0] + rn[1] * sn[1] + rn[2] * sn[2];
                         const float angle = o_acosf(dot);
                         const bool angle_ok = (angle != angle) ? true : (angle < fp->max_angle_rad);
@@ -2311,9 +2353,9 @@ int oracle_fuse(const dpe_fuse_view *views, int num_views, const dpe_fuse_params
                     col[2] /= (num_consistent + 1);
                     if (points && count < capacity) {
                         dpe_point &o = points[count];
-                        o.x = PointX.x;
-                        o.y = PointX.y;
-                        o.z = PointX.z;
+                        o.x = (float)PointX.x;
+                        o.y = (float)PointX.y;
+                        o.z = (float)PointX.z;
                         o.b = (uint8_t)col[0];
                         o.g = (uint8_t)col[1];
                         o.r = (uint8_t)col[2];
@@ -2326,6 +2368,19 @@ int oracle_fuse(const dpe_fuse_view *views, int num_views, const dpe_fuse_params
     }
     *num_points = count;
     return DPE_OK;
+}
+
+}  // namespace
+}  // namespace oracle
+
+extern "C" {
+
+int oracle_fuse(const dpe_fuse_view *views, int num_views, const dpe_fuse_params *fp, dpe_point *points,
+                int64_t capacity, int64_t *num_points) {
+    if (!views || num_views <= 0 || !fp || !num_points) return DPE_ERR_INVALID_ARGUMENT;
+    if (fp->fp64_reprojection == 1) return fuse_impl<double>(views, num_views, fp, points, capacity, num_points);
+    if (fp->fp64_reprojection != 0) return DPE_ERR_INVALID_ARGUMENT;
+    return fuse_impl<float>(views, num_views, fp, points, capacity, num_points);
 }
 
 // RunFusion_TAT_Intermediate (DPE.cpp:1550-1718) and RunFusion_TAT_advanced

@@ -92,13 +92,14 @@ class _Views:
     pass
 
 
-def coincident_pair(depth1_scale_rows=None, W=40, H=30):
-    """Two views with the same camera looking at the plane z = 4 (all pixels
-    strong, normals (0, 0, -1)): every pixel of view 0 reprojects onto itself
-    in view 1.  depth1_scale_rows: (row slice, factor) applied to view 1's
-    depth.  Returns (views holder for fuse_oracle / fuse_hip, K)."""
+def coincident_pair(depth1_scale_rows=None, W=40, H=30, R=None, t=None):
+    """Two views with the same camera looking at the plane z = 4 in camera
+    coordinates (all pixels strong, normals (0, 0, -1)): every pixel of view 0
+    reprojects onto itself in view 1.  depth1_scale_rows: (row slice, factor)
+    applied to view 1's depth; R, t: the shared pose (default identity).
+    Returns (views holder for fuse_oracle / fuse_hip, K)."""
     K = np.array([[50.0, 0, (W - 1) / 2], [0, 50.0, (H - 1) / 2], [0, 0, 1]])
-    cam = abi.camera_from_arrays(K, np.eye(3), np.zeros(3), W, H, 2, 6)
+    cam = abi.camera_from_arrays(K, np.eye(3) if R is None else R, np.zeros(3) if t is None else t, W, H, 2, 6)
     fs = _Views()
     fs.keep = []
     fs.views = (abi.FuseView * 2)()
@@ -110,6 +111,8 @@ def coincident_pair(depth1_scale_rows=None, W=40, H=30):
             d[rows] *= np.float32(f)
         nrm = np.zeros((H, W, 3), np.float32)
         nrm[..., 2] = -1.0
+        if R is not None:
+            nrm = (nrm.reshape(-1, 3) @ np.asarray(R, np.float64)).astype(np.float32).reshape(H, W, 3)  # world frame
         st = np.full((H, W), abi.STRONG, np.uint8)
         col = np.full((H, W, 3), 100 + i, np.uint8)
         fs.keep += [d, nrm, st, col]
@@ -119,6 +122,29 @@ def coincident_pair(depth1_scale_rows=None, W=40, H=30):
         v.num_src = 1
         v.src_index[0] = 1 - i
     return fs, K
+
+
+def rotated_pose():
+    """A fixed non-trivial rotation (about an oblique axis) and translation."""
+    ax = np.array([0.3, -0.8, 0.52])
+    ax /= np.linalg.norm(ax)
+    a = 0.7
+    Kx = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+    R = np.eye(3) + np.sin(a) * Kx + (1 - np.cos(a)) * Kx @ Kx
+    return R, np.array([0.37, -1.21, 2.9])
+
+
+def world_points_fp64(cam, W, H, depth):
+    """Get3DPointonWorld of every pixel evaluated in float64 from the camera's
+    float32 entries (the FP64 form's arithmetic up to operation order), rounded
+    to float32 as the point cloud stores it.  (H*W, 3)."""
+    K = np.array(cam.K[:], np.float64).reshape(3, 3)
+    R = np.array(cam.R[:], np.float64).reshape(3, 3)
+    t = np.array(cam.t[:], np.float64)
+    ys, xs = np.mgrid[0:H, 0:W]
+    Xc = np.stack([depth * (xs - K[0, 2]) / K[0, 0], depth * (ys - K[1, 2]) / K[1, 1],
+                   np.full((H, W), depth)], -1).reshape(-1, 3)
+    return (Xc @ R - R.T @ t).astype(np.float32)
 
 
 def coincident_views(n, W=40, H=30, depth_scale=None, tilt=None, color_b=None, depth_zero_rows=None):

@@ -192,3 +192,31 @@ def test_fusion_closed_form_on_gpu(ctx):
     assert len(fuse_hip(ctx, fs, abi.fuse_defaults())) == W * (H - 7)
     fs, _ = coincident_pair(None, W, H)
     assert len(fuse_hip(ctx, fs, abi.fuse_defaults(min_consistent=2))) == 0
+
+
+def test_fusion_fp64_reprojection_matches_oracle(ctx, orc, c1):
+    """The precision form's FP64 fusion geometry on the GPU equals the oracle's
+    FP64 restatement bit for bit (ground-truth and perturbed depths), and on
+    coincident views under a rotated camera its points are the float64
+    back-projections rounded to float32."""
+    from fuse_util import FuseSet, coincident_pair, fuse_hip, fuse_oracle, rotated_pose, world_points_fp64
+    rng = np.random.default_rng(19)
+    states = [np.where(v.weak, abi.WEAK, abi.STRONG).astype(np.uint8) for v in c1.views]
+    normals = [v.normal for v in c1.views]
+    for noise in (0.0, 0.004):
+        depths = [v.depth * (1 + noise * rng.standard_normal(v.depth.shape)).astype(np.float32) for v in c1.views]
+        fs = FuseSet(c1, depths, normals, states)
+        fp = abi.fuse_defaults(fp64_reprojection=1)
+        po = fuse_oracle(orc, fs, fp)
+        pg = fuse_hip(ctx, fs, fp)
+        p32 = fuse_hip(ctx, fs, abi.fuse_defaults())
+        print(f"fp64 fusion noise {noise}: oracle {len(po)} points, gpu {len(pg)}, fp32 path {len(p32)}")
+        assert len(po) > 0 and len(pg) == len(po)
+        assert np.array_equal(pg.view(np.uint8), po.view(np.uint8))
+        assert abs(len(p32) - len(pg)) <= 0.005 * len(pg)
+    W, H = 40, 30
+    R, t = rotated_pose()
+    fs, _ = coincident_pair(None, W, H, R, t)
+    pg = fuse_hip(ctx, fs, abi.fuse_defaults(fp64_reprojection=1))
+    assert len(pg) == W * H
+    assert np.array_equal(np.stack([pg["x"], pg["y"], pg["z"]], -1), world_points_fp64(fs.views[0].camera, W, H, 4.0))

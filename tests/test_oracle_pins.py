@@ -157,3 +157,28 @@ def test_fusion_closed_form_two_coincident_views(orc):
     assert len(fuse_oracle(orc, fs, abi.fuse_defaults())) == W * (H - 7)
     fs, _ = coincident_pair(None, W, H)
     assert len(fuse_oracle(orc, fs, abi.fuse_defaults(min_consistent=2))) == 0
+
+
+def test_fusion_fp64_reprojection_closed_form(orc):
+    """The precision form's FP64 fusion geometry (dpe_fuse_params.fp64_reprojection):
+    on two coincident views under a rotated, translated camera every pixel of
+    view 0 is kept, and its point equals Get3DPointonWorld evaluated in float64
+    and rounded to float32, while the fp32 path's points differ from that in
+    the last bits (the test fails if either path computes in the other's type)."""
+    from fuse_util import coincident_pair, fuse_oracle, rotated_pose, world_points_fp64
+    W, H = 40, 30
+    R, t = rotated_pose()
+    fs, _ = coincident_pair(None, W, H, R, t)
+    p64 = fuse_oracle(orc, fs, abi.fuse_defaults(fp64_reprojection=1))
+    p32 = fuse_oracle(orc, fs, abi.fuse_defaults())
+    assert len(p64) == len(p32) == W * H
+    exp = world_points_fp64(fs.views[0].camera, W, H, 4.0)
+    got64 = np.stack([p64["x"], p64["y"], p64["z"]], -1)
+    got32 = np.stack([p32["x"], p32["y"], p32["z"]], -1)
+    assert np.array_equal(got64, exp)
+    assert (got32 != exp).any()
+    np.testing.assert_allclose(got32, exp, rtol=1e-5, atol=1e-5)
+    fs, _ = coincident_pair(None, W, H, R, t)
+    bad = abi.fuse_defaults(fp64_reprojection=2)
+    cnt = C.c_int64()
+    assert orc.oracle_fuse(fs.views, fs.n, C.byref(bad), None, 0, C.byref(cnt)) == abi.ERR_INVALID_ARGUMENT
