@@ -1,7 +1,8 @@
 // dpe_device.h -- device-side numerics and per-pixel geometry of the gfx950 path.
 //
 // Numerics contract (the HIP path and the CPU oracle follow it independently):
-//   * IEEE fp32, correctly rounded divide and sqrt, no contraction
+//   * IEEE fp32, correctly rounded divide and sqrt, no contraction; the tap
+//     projection multiplies by the correctly rounded 1/w (see project())
 //     (-ffp-contract=off): fused multiply-adds only where written as fmaf
 //     (NCC tap loop, homography projection, bilinear lerp, polynomials).
 //   * exp: Cody-Waite reduction by ln2 (16-bit high part) and a degree-7 Taylor
@@ -295,13 +296,18 @@ __device__ __forceinline__ void homography64(const ViewConst64 &v, const float4 
         for (int k = 0; k < 3; ++k) H[3 * r + k] = (float)fma(-v.b[r], c[k], v.A[3 * r + k]);
 }
 
-// ComputeCorrespondingPoint, DPE.cu:519-526 (projection written with fmaf)
+// ComputeCorrespondingPoint, DPE.cu:519-526, projection written with fmaf.  The
+// divisions by w are multiplications by the correctly rounded 1/w, shared by
+// both coordinates: (u * r, v * r), r = RN(1 / w).  The reference's build
+// (--use_fast_math, CMakeLists.txt:26) does not divide correctly rounded
+// either; this form costs one reciprocal per tap instead of two quotients.
 __device__ __forceinline__ float2 project(const float H[9], int px, int py) {
     const float x = (float)px, y = (float)py;
     const float u = fmaf(H[0], x, fmaf(H[1], y, H[2]));
     const float v = fmaf(H[3], x, fmaf(H[4], y, H[5]));
     const float w = fmaf(H[6], x, fmaf(H[7], y, H[8]));
-    return make_float2(u / w, v / w);
+    const float r = 1.0f / w;
+    return make_float2(u * r, v * r);
 }
 
 // ---- exact division without the scaling steps
@@ -323,26 +329,27 @@ __device__ __forceinline__ float quotient_refined(float a, float w, float y1) {
     return fmaf(fmaf(-w, q1, a), y1, q1);
 }
 
-// project() through the refined division; equal to project() wherever
-// refined_division_ok() holds for a window containing (px, py), except that a
-// zero coordinate may carry the other sign (floor and fraction are the same).
+// project() with 1 / w through the refined division (numerator 1); equal to
+// project() wherever refined_division_ok() holds for a window containing
+// (px, py).
 __device__ __forceinline__ float2 project_refined(const float H[9], int px, int py) {
     const float x = (float)px, y = (float)py;
     const float u = fmaf(H[0], x, fmaf(H[1], y, H[2]));
     const float v = fmaf(H[3], x, fmaf(H[4], y, H[5]));
     const float w = fmaf(H[6], x, fmaf(H[7], y, H[8]));
-    const float r = recip_refined(w);
-    return make_float2(quotient_refined(u, w, r), quotient_refined(v, w, r));
+    const float r = quotient_refined(1.0f, w, recip_refined(w));
+    return make_float2(u * r, v * r);
 }
 
-// True when the refined division is exact at every tap (X, Y) of the window
-// [x0, x1] x [y0, y1] (|X|, |Y| <= 2^17):
-//  - each of H[0..5] is 0 or has magnitude in [2^-60, 2^20]: then u and v are
-//    multiples of 2^-83 (so 0 or >= 2^-83 in magnitude) and below 2^38;
+// True when the refined division gives RN(1 / w) at every tap (X, Y) of the
+// window [x0, x1] x [y0, y1] (|X|, |Y| <= 2^17):
+//  - each of H[0..5] is 0 or has magnitude in [2^-60, 2^20] (u and v are then
+//    multiples of 2^-83 below 2^38; kept so the window test also bounds the
+//    projected coordinates);
 //  - w keeps one sign with |w| in [2^-40, 2^40]: w is affine in (X, Y), so its
 //    extremes sit at the window corners, and computed values differ from exact
 //    ones by less than 2^-22 M, M = |H6| max|X| + |H7| max|Y| + |H8|.
-// Hence w, 1/w and u/w, v/w are normal and the exponent gaps stay below 80.
+// Hence w and 1/w are normal and the exponent gap of 1 and w stays below 41.
 __device__ __forceinline__ bool refined_division_ok(const float H[9], int x0, int x1, int y0, int y1) {
     bool ok = true;
 #pragma unroll

@@ -339,13 +339,18 @@ void ComputeHomography(const dpe_camera &ref, const dpe_camera &src, const F4 &p
     H[8] = src.K[8] * tmp[8];
 }
 
-// ComputeCorrespondingPoint, DPE.cu:519-526 (projection written with fmaf)
+// ComputeCorrespondingPoint, DPE.cu:519-526 (projection written with fmaf).
+// Numerics contract: the divisions by w are multiplications by the correctly
+// rounded reciprocal, (u * r, v * r) with r = 1.0f / w (the reference's
+// --use_fast_math build, CMakeLists.txt:26, does not divide correctly rounded
+// either).
 inline F2 ComputeCorrespondingPoint(const float *H, int px, int py) {
     const float x = (float)px, y = (float)py;
     const float u = std::fma(H[0], x, std::fma(H[1], y, H[2]));
     const float v = std::fma(H[3], x, std::fma(H[4], y, H[5]));
     const float w = std::fma(H[6], x, std::fma(H[7], y, H[8]));
-    return F2{u / w, v / w};
+    const float r = 1.0f / w;
+    return F2{u * r, v * r};
 }
 
 // TransformNormal, DPE.cu:528-536 (camera -> world)
