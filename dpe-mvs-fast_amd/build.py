@@ -29,6 +29,10 @@ def hipcc():
             return c
 
 
+# exhaustive check of the projection's reciprocal (a GPU test runs it)
+RCP_CHECK = os.path.join(HERE, "dpe_rcp_check")
+RCP_CHECK_SOURCE = os.path.join(HERE, "csrc", "dpe_rcp_check.hip")
+
 HOST_LIB = os.path.join(HERE, "libdpe_host.so")
 DRIVER = os.path.join(HERE, "dpe_driver")
 HOST_SOURCES = [os.path.join(HERE, "host", "dpe_hostio.cpp"), os.path.join(HERE, "host", "dpe_edges.cpp"),
@@ -77,6 +81,12 @@ def build(verbose=True, jobs=2):
     os.replace(LIB + ".tmp", LIB)
     for o in objs:
         os.remove(o)
+    cmd = [hipcc()] + [f for f in HIPCC_FLAGS if f not in ("-shared", "-fPIC")] + [
+        RCP_CHECK_SOURCE, "-o", RCP_CHECK + ".tmp", "-I", os.path.join(ROOT, "include")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(RCP_CHECK + ".tmp", RCP_CHECK)
     build_host(verbose)
     return LIB
 

@@ -310,46 +310,38 @@ __device__ __forceinline__ float2 project(const float H[9], int px, int py) {
     return make_float2(u * r, v * r);
 }
 
-// ---- exact division without the scaling steps
-// For IEEE a / w the compiler emits w' = div_scale(w), a' = div_scale(a),
-// y = rcp(w'), y1 = fma(fma(-w', y, 1), y, y), q0 = a' * y1,
-// q1 = fma(fma(-w', q0, a'), y1, q0), q2 = fma(fma(-w', q1, a'), y1, q1)
-// (div_fmas), then div_fixup.  Scaling is the identity and the fixup passes q2
-// through when w and 1/w are normal, a is 0 or |a| >= 2^-103, the exponents of
-// a and w differ by less than 96 and a / w is normal.  Under those conditions
-// the two functions below give the bits of a / w (for a = 0 possibly with the
-// other sign of zero), and y1 is shared by every numerator over the same w.
+// ---- exact reciprocal in three instructions
+// On gfx950 one Newton step from the hardware reciprocal estimate,
+// y1 = fma(fma(-w, y, 1), y, y) with y = v_rcp_f32(w), equals the correctly
+// rounded 1 / w for every float w with |w| in [2^-40, 2^40), both signs: checked
+// exhaustively (all 1.34e9 such values) by dpe_rcp_check, which the GPU tests
+// run.  IEEE 1.0f / w as the compiler emits it takes about 12 instructions
+// (div_scale x2, rcp, 7 FMA/MUL, div_fmas, div_fixup).
 __device__ __forceinline__ float recip_refined(float w) {
     const float y = __builtin_amdgcn_rcpf(w);
     return fmaf(fmaf(-w, y, 1.0f), y, y);
 }
-__device__ __forceinline__ float quotient_refined(float a, float w, float y1) {
-    const float q0 = a * y1;
-    const float q1 = fmaf(fmaf(-w, q0, a), y1, q0);
-    return fmaf(fmaf(-w, q1, a), y1, q1);
-}
 
-// project() with 1 / w through the refined division (numerator 1); equal to
-// project() wherever refined_division_ok() holds for a window containing
-// (px, py).
+// project() with the three-instruction reciprocal; equal to project() wherever
+// refined_division_ok() holds for a window containing (px, py).
 __device__ __forceinline__ float2 project_refined(const float H[9], int px, int py) {
     const float x = (float)px, y = (float)py;
     const float u = fmaf(H[0], x, fmaf(H[1], y, H[2]));
     const float v = fmaf(H[3], x, fmaf(H[4], y, H[5]));
     const float w = fmaf(H[6], x, fmaf(H[7], y, H[8]));
-    const float r = quotient_refined(1.0f, w, recip_refined(w));
+    const float r = recip_refined(w);
     return make_float2(u * r, v * r);
 }
 
-// True when the refined division gives RN(1 / w) at every tap (X, Y) of the
-// window [x0, x1] x [y0, y1] (|X|, |Y| <= 2^17):
+// True when recip_refined gives RN(1 / w) at every tap (X, Y) of the window
+// [x0, x1] x [y0, y1] (|X|, |Y| <= 2^17):
 //  - each of H[0..5] is 0 or has magnitude in [2^-60, 2^20] (u and v are then
 //    multiples of 2^-83 below 2^38; kept so the window test also bounds the
 //    projected coordinates);
 //  - w keeps one sign with |w| in [2^-40, 2^40]: w is affine in (X, Y), so its
 //    extremes sit at the window corners, and computed values differ from exact
 //    ones by less than 2^-22 M, M = |H6| max|X| + |H7| max|Y| + |H8|.
-// Hence w and 1/w are normal and the exponent gap of 1 and w stays below 41.
+// Hence every computed w lies in the checked range [2^-40, 2^40).
 __device__ __forceinline__ bool refined_division_ok(const float H[9], int x0, int x1, int y0, int y1) {
     bool ok = true;
 #pragma unroll
