@@ -571,39 +571,58 @@ def fusion_bench(args):
     print(json.dumps(result), flush=True)
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sc, threads):
     """The CPU oracle on a bounded sample of the same workload: one FIRST_INIT
     call with 2 iterations on a 960x720 centre crop of reference view 10 and its
-    10 sources (principal points shifted), timed with `threads` host threads."""
+    10 sources (principal points shifted), timed with `threads` host threads,
+    and the same call on a 240x180 crop with one thread."""
     import numpy as np
     import oracle_lib
     from dpe_amd import abi, host
     nthreads = threads or min(16, os.cpu_count() or 1)
     ref = min(10, len(sc.views) - 1)
     ids = [ref] + sc.pairs[ref]
-    cw, ch = 960, 720
-    x0, y0 = (sc.width - cw) // 2, (sc.height - ch) // 2
-    imgs, cams = [], []
-    for i in ids:
-        v = sc.views[i]
-        imgs.append(v.image[y0:y0 + ch, x0:x0 + cw].astype(np.float32))
-        K = v.K.astype(np.float32).copy()
-        K[0, 2] -= x0
-        K[1, 2] -= y0
-        cams.append(abi.camera_from_arrays(K, v.R, v.t, cw, ch, sc.depth_min, sc.depth_max))
-    prm = host.call_params(sc.depth_min, sc.depth_max, max_iterations=2, state=abi.FIRST_INIT, use_APD=0,
-                           use_edge=0, use_limit=0, use_label=0, geom_consistency=0)
-    pin = host.ProblemInputs(imgs, cams, prm)
     lib = oracle_lib.load_oracle()
-    fam = []
-    t = time.perf_counter()
-    oracle_lib.run_oracle(lib, pin, nthreads, fam)
-    dt = time.perf_counter() - t
-    # the GPU's unit: every family but the evaluations the reference never reads
-    hyps = sum(h for h, _ in fam[:abi.NCC_FAMILIES])
+
+    def run(cw, ch, nt):
+        x0, y0 = (sc.width - cw) // 2, (sc.height - ch) // 2
+        imgs, cams = [], []
+        for i in ids:
+            v = sc.views[i]
+            imgs.append(v.image[y0:y0 + ch, x0:x0 + cw].astype(np.float32))
+            K = v.K.astype(np.float32).copy()
+            K[0, 2] -= x0
+            K[1, 2] -= y0
+            cams.append(abi.camera_from_arrays(K, v.R, v.t, cw, ch, sc.depth_min, sc.depth_max))
+        prm = host.call_params(sc.depth_min, sc.depth_max, max_iterations=2, state=abi.FIRST_INIT, use_APD=0,
+                               use_edge=0, use_limit=0, use_label=0, geom_consistency=0)
+        pin = host.ProblemInputs(imgs, cams, prm)
+        fam = []
+        t = time.perf_counter()
+        oracle_lib.run_oracle(lib, pin, nt, fam)
+        dt = time.perf_counter() - t
+        # the GPU's unit: every family but the evaluations the reference never reads
+        return sum(h for h, _ in fam[:abi.NCC_FAMILIES]), dt
+
+    hyps, dt = run(960, 720, nthreads)
+    hyps1, dt1 = run(240, 180, 1)
     return {"value": hyps / dt, "unit": "NCC plane-hyp/s", "cores": nthreads, "kind": "port",
-            "sample": f"oracle FIRST_INIT call, 2 iterations, {cw}x{ch} centre crop of view {ref} + 10 sources, "
-                      f"{hyps} NCC-hyps in {dt:.1f}s"}
+            "sample": f"oracle FIRST_INIT call, 2 iterations, 960x720 centre crop of view {ref} + 10 sources, "
+                      f"{hyps} NCC-hyps in {dt:.1f}s on {nthreads} threads",
+            "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
+            "single_thread": {"value": hyps1 / dt1, "unit": "NCC plane-hyp/s",
+                              "sample": f"same call on a 240x180 centre crop, {hyps1} NCC-hyps in {dt1:.1f}s"}}
 
 
 if __name__ == "__main__":
