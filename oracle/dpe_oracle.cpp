@@ -2488,6 +2488,73 @@ int oracle_fuse_tat(const dpe_fuse_view *views, int num_views, const uint8_t *co
     return DPE_OK;
 }
 
+// RANSACToGetFitPlane (DPE.cu:2895-3128) of one WEAK pixel (px, py) on a given
+// state (test access): camera-frame planes of every pixel (float4 n, d), the
+// pixel's 9 neighbour slots (x, y; slot 0 the pixel itself, (-1, -1) none), all
+// other pixels STRONG, use_limit / use_label / geom_consistency / use_radius
+// off.  Writes the fitted plane (0 0 0 0: none) to out.
+void oracle_fit_plane(const dpe_camera *cam, int W, int H, const float *planes, const int16_t *neighbours, int px,
+                      int py, uint64_t seed, float *out) {
+    Ctx C;
+    C.W = W;
+    C.H = H;
+    C.P = W * H;
+    C.N = 2;
+    std::memset(&C.prm, 0, sizeof(C.prm));
+    C.prm.seed = seed;
+    C.cams[0] = *cam;
+    C.planes.resize((size_t)C.P);
+    for (int i = 0; i < C.P; ++i)
+        C.planes[i] = F4{planes[4 * i], planes[4 * i + 1], planes[4 * i + 2], planes[4 * i + 3]};
+    C.fit_planes.assign((size_t)C.P, F4{0, 0, 0, 0});
+    const int center = px + py * W;
+    C.weak.assign((size_t)C.P, DPE_STRONG);
+    C.weak[center] = DPE_WEAK;
+    C.nmap.assign((size_t)C.P, 0);
+    C.neighbours.resize(NNUM);
+    for (int k = 0; k < NNUM; ++k) C.neighbours[k] = S2{neighbours[2 * k], neighbours[2 * k + 1]};
+    C.complex_map.assign((size_t)C.P, 0.0f);
+    C.radius.assign((size_t)C.P, 5);
+    RANSACToGetFitPlane(C, center, 0);
+    const F4 f = C.fit_planes[center];
+    out[0] = f.x;
+    out[1] = f.y;
+    out[2] = f.z;
+    out[3] = f.w;
+}
+
+// GenEdgeInform (DPE.cu:2487-2595) over a whole image (test access): edge map
+// (u8), label map (i32, or NULL = use_label off), pixel states; edge_neigh: 8
+// (x, y) pairs per pixel, complex_map: per pixel, label_bound: 8 (x, y) pairs
+// per pixel (written for WEAK pixels with a label > 0, else (-1, -1)).
+void oracle_gen_edge_inform(int W, int H, const uint8_t *edge, const int32_t *label, const uint8_t *weak,
+                            int strong_radius, int16_t *edge_neigh, float *complex_map, int16_t *label_bound) {
+    Ctx C;
+    C.W = W;
+    C.H = H;
+    C.P = W * H;
+    std::memset(&C.prm, 0, sizeof(C.prm));
+    C.prm.use_edge = 1;
+    C.prm.use_label = label != nullptr;
+    C.prm.strong_radius = strong_radius;
+    C.edge = edge;
+    C.label = label;
+    C.weak.assign(weak, weak + C.P);
+    C.edge_neigh.assign((size_t)C.P * 8, S2{-1, -1});
+    C.complex_map.assign((size_t)C.P, 0.0f);
+    C.nmap.assign((size_t)C.P, 0);
+    for (int i = 0; i < C.P; ++i) C.nmap[i] = i;  // one label_bound row per pixel
+    C.label_bound.assign((size_t)C.P * 8, S2{-1, -1});
+    for (int i = 0; i < C.P; ++i) GenEdgeInform(C, i);
+    for (size_t k = 0; k < (size_t)C.P * 8; ++k) {
+        edge_neigh[2 * k] = C.edge_neigh[k].x;
+        edge_neigh[2 * k + 1] = C.edge_neigh[k].y;
+        label_bound[2 * k] = C.label_bound[k].x;
+        label_bound[2 * k + 1] = C.label_bound[k].y;
+    }
+    for (int i = 0; i < C.P; ++i) complex_map[i] = C.complex_map[i];
+}
+
 // Numerics test access.
 float oracle_expf(float x) { return o_expf(x); }
 float oracle_acosf(float x) { return o_acosf(x); }

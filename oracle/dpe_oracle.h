@@ -40,6 +40,13 @@ void oracle_nearest_strong(const uint8_t *weak, int W, int H, int16_t *out);
 /* CheckerboardFilterStrong (DPE.cu:1961-2071) of one pixel; returns its filtered depth */
 float oracle_filter_strong(const float *depth, const float *costs, const uint8_t *weak, int W, int H, int px,
                            int py);
+/* RANSACToGetFitPlane (DPE.cu:2895-3128) of one WEAK pixel on a given state (camera-frame planes, 9 neighbour
+ * slots of (x, y)); out: the fitted plane, 0 0 0 0 when none */
+void oracle_fit_plane(const dpe_camera *cam, int W, int H, const float *planes, const int16_t *neighbours, int px,
+                      int py, uint64_t seed, float *out);
+/* GenEdgeInform (DPE.cu:2487-2595) over a whole image; label NULL = use_label off */
+void oracle_gen_edge_inform(int W, int H, const uint8_t *edge, const int32_t *label, const uint8_t *weak,
+                            int strong_radius, int16_t *edge_neigh, float *complex_map, int16_t *label_bound);
 #ifdef __cplusplus
 }
 #endif

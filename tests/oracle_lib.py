@@ -59,6 +59,12 @@ def load_oracle():
     lib.oracle_nearest_strong.restype = None
     lib.oracle_filter_strong.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
     lib.oracle_filter_strong.restype = C.c_float
+    lib.oracle_fit_plane.argtypes = [P(abi.Camera), C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                     C.c_uint64, C.c_void_p]
+    lib.oracle_fit_plane.restype = None
+    lib.oracle_gen_edge_inform.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                           C.c_void_p, C.c_void_p]
+    lib.oracle_gen_edge_inform.restype = None
     return lib
 
 

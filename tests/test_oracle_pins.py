@@ -182,3 +182,118 @@ def test_fusion_fp64_reprojection_closed_form(orc):
     bad = abi.fuse_defaults(fp64_reprojection=2)
     cnt = C.c_int64()
     assert orc.oracle_fuse(fs.views, fs.n, C.byref(bad), None, 0, C.byref(cnt)) == abi.ERR_INVALID_ARGUMENT
+
+
+def _fit(orc, cam, W, H, planes, nbrs, px, py, seed=7):
+    nb = np.full((9, 2), -1, np.int16)
+    nb[0] = (px, py)
+    for k, (x, y) in enumerate(nbrs):
+        nb[1 + k] = (x, y)
+    out = np.zeros(4, np.float32)
+    orc.oracle_fit_plane(C.byref(cam), W, H, np.ascontiguousarray(planes, np.float32).ctypes.data, nb.ctypes.data,
+                         px, py, seed, out.ctypes.data)
+    return out
+
+
+def test_ransac_fit_plane_closed_form(orc):
+    """RANSACToGetFitPlane (DPE.cu:2895-3128): eight neighbours around the pixel,
+    all on one camera-frame plane -> the fit is that plane (normal and offset),
+    oriented towards the camera; one neighbour off the plane by 10 % in depth
+    does not move it (the triple with the lowest residual sum excludes it);
+    neighbours all on one side of the pixel (no triangle contains it) -> no
+    fit; fewer than three neighbours -> the pixel's own plane."""
+    W, H = 96, 72
+    K = np.array([[80.0, 0, 47.5], [0, 80.0, 35.5], [0, 0, 1]])
+    cam = abi.camera_from_arrays(K, np.eye(3), np.zeros(3), W, H, 1, 10)
+    n = np.array([0.2, -0.1, -1.0])
+    n /= np.linalg.norm(n)
+    d = -float(n @ np.array([0.0, 0.0, 4.0]))  # the plane n . X + d = 0 through (0, 0, 4)
+    planes = np.tile(np.array([n[0], n[1], n[2], d], np.float32), (H, W, 1))
+    px, py = 48, 36
+    ring = [(px + dx * 12, py + dy * 12) for dx, dy in ((-1, -1), (0, -1), (1, -1), (-1, 0), (1, 0), (-1, 1), (0, 1),
+                                                        (1, 1))]
+    f = _fit(orc, cam, W, H, planes, ring, px, py)
+    assert np.allclose(f[:3], n, atol=1e-4) and abs(f[3] - d) < 1e-4 * abs(d)
+    # one neighbour off the plane: 10 % deeper along its ray
+    bad = planes.copy()
+    bx, by = ring[2]
+    bad[by, bx, 3] = d * 1.1  # scales that pixel's depth by 1.1 (depth = -d / (n . K^-1 x))
+    f2 = _fit(orc, cam, W, H, bad, ring, px, py)
+    assert np.allclose(f2[:3], n, atol=1e-4) and abs(f2[3] - d) < 1e-4 * abs(d)
+    # all neighbours to the right of the pixel (not collinear): no triangle contains it
+    right = [(px + 8 + (k % 3) * 3, py - 9 + 3 * k) for k in range(8)]
+    assert not _fit(orc, cam, W, H, planes, right, px, py).any()
+    # two neighbours: the pixel's own plane
+    own = planes.copy()
+    own[py, px] = (0.0, 0.0, -1.0, 4.0)
+    assert np.array_equal(_fit(orc, cam, W, H, own, ring[:2], px, py), own[py, px])
+
+
+DIRS = [(0, -1), (0, 1), (-1, 0), (1, 0), (-1, -1), (1, 1), (-1, 1), (1, -1)]
+
+
+def _ray(W, H, dx, dy, k):
+    ys, xs = np.mgrid[0:H, 0:W]
+    nx, ny = xs + k * dx, ys + k * dy
+    inside = (nx >= 0) & (nx < W) & (ny >= 0) & (ny < H)
+    return nx, ny, inside
+
+
+def test_gen_edge_inform_against_vectorised_definitions(orc):
+    """GenEdgeInform (DPE.cu:2487-2595) against array formulations of its
+    definitions: the first edge pixel along each of the 8 directions (found by
+    overwriting from the far end), the edge density of the (2r+1)^2 window
+    clipped to the image (a box-filter count) through the sigmoid
+    1 / (1 + exp(-25 (density - 0.35))), with use_label the boundary density
+    bound_pix / tot_pix in integer division (1 only when the whole window is
+    boundary), and for WEAK pixels with a label the last pixel of the same label
+    along each direction before a -1 label or the border."""
+    import scipy.ndimage as ndi
+    rng = np.random.default_rng(41)
+    W, H, r = 40, 30, 2
+    edge = (rng.uniform(size=(H, W)) < 0.15).astype(np.uint8) * 255
+    regions, _ = ndi.label(ndi.uniform_filter(rng.uniform(size=(H, W)), 5) > 0.5)
+    label = regions.astype(np.int32)
+    label[rng.uniform(size=(H, W)) < 0.05] = -1
+    label[(regions == 0) & (rng.uniform(size=(H, W)) < 0.5)] = 0
+    label[5:10, 5:10] = 0  # a window that is boundary everywhere
+    weak = np.where(rng.uniform(size=(H, W)) < 0.5, abi.WEAK, abi.STRONG).astype(np.uint8)
+    for lab in (None, label):
+        en = np.zeros((H, W, 8, 2), np.int16)
+        cm = np.zeros((H, W), np.float32)
+        lb = np.zeros((H, W, 8, 2), np.int16)
+        orc.oracle_gen_edge_inform(W, H, edge.ctypes.data, None if lab is None else lab.ctypes.data, weak.ctypes.data, r,
+                                   en.ctypes.data, cm.ctypes.data, lb.ctypes.data)
+        L = max(W, H)
+        for i, (dx, dy) in enumerate(DIRS):
+            exp = np.full((H, W, 2), -1, np.int64)
+            for k in range(L, 0, -1):
+                nx, ny, inside = _ray(W, H, dx, dy, k)
+                hit = inside & (edge[np.clip(ny, 0, H - 1), np.clip(nx, 0, W - 1)] != 0)
+                exp[hit] = np.stack([nx, ny], -1)[hit]
+            assert np.array_equal(en[:, :, i], exp), i
+        box = np.ones((2 * r + 1, 2 * r + 1))
+        tot = ndi.convolve(np.ones((H, W)), box, mode="constant").astype(np.int64)
+        cnt = ndi.convolve((edge != 0).astype(float), box, mode="constant").astype(np.int64)
+        density = np.float32(1.0) * cnt.astype(np.float32) / tot.astype(np.float32)
+        if lab is not None:
+            bnd = ndi.convolve((lab == 0).astype(float), box, mode="constant").astype(np.int64)
+            density = np.maximum(density, (bnd // tot).astype(np.float32))
+            assert (bnd // tot).max() == 1
+        expc = 1.0 / (1.0 + np.exp(-25.0 * (density.astype(np.float64) - 0.35)))
+        np.testing.assert_allclose(cm, expc.astype(np.float32), rtol=2e-7, atol=0)
+        if lab is None:
+            assert (lb == -1).all()
+            continue
+        sel = (weak == abi.WEAK) & (lab > 0)
+        for i, (dx, dy) in enumerate(DIRS):
+            exp = np.full((H, W, 2), -1, np.int64)
+            alive = np.ones((H, W), bool)
+            for k in range(1, L + 1):
+                nx, ny, inside = _ray(W, H, dx, dy, k)
+                nl = np.where(inside, lab[np.clip(ny, 0, H - 1), np.clip(nx, 0, W - 1)], -1)
+                alive &= inside & (nl != -1)
+                same = alive & (nl == lab)
+                exp[same] = np.stack([nx, ny], -1)[same]
+            exp[~sel] = -1
+            assert np.array_equal(lb[:, :, i], exp), i
