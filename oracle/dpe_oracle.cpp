@@ -1851,6 +1851,45 @@ inline float CameraDistance(const Ctx &C, int src_index) {
     return std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
 }
 
+// The peak analysis of DepthToWeak (DPE.cu:2699-2750) over the 61 sampled
+// costs (disparity offsets -30..30): WEAK unless the lowest strict local
+// minimum sits within weak_peak_radius of the centre with cost <= 0.5, and then
+// STRONG if it is the only one at <= 0.15, or if the other minima spread enough.
+uint8_t DepthToWeakState(const float *pc, int weak_peak_radius) {
+    const int radius = 30;
+    const int n = 2 * radius + 1;
+    bool is_peak[n];
+    for (int i = 0; i < n; ++i) is_peak[i] = false;
+    int peak_count = 0, min_peak = 0;
+    float min_cost = 2.0f;
+    for (int i = 2; i < n - 2; ++i) {
+        if (pc[i - 1] > pc[i] && pc[i + 1] > pc[i]) {
+            is_peak[i] = true;
+            peak_count++;
+            if (pc[i] < min_cost) {
+                min_peak = i;
+                min_cost = pc[i];
+            }
+        }
+    }
+    if (std::abs(min_peak - radius) > weak_peak_radius || pc[min_peak] > 0.5f) {
+        return DPE_WEAK;
+    }
+    if (peak_count == 1) {
+        return pc[min_peak] <= 0.15f ? DPE_STRONG : DPE_WEAK;
+    }
+    float var = 0.0f;
+    for (int i = 2; i < n - 2; ++i) {
+        if (is_peak[i] && i != min_peak) {
+            const float dist = pc[i] - min_cost;
+            var += dist * dist;
+        }
+    }
+    var = std::sqrt(var);
+    var /= (peak_count - 1);
+    return var > 0.2f ? DPE_STRONG : DPE_WEAK;
+}
+
 // DepthToWeak, DPE.cu:2597-2751
 void DepthToWeak(Ctx &C, int center) {
     // The cost of the current plane below is never read afterwards in the
@@ -1921,38 +1960,7 @@ void DepthToWeak(Ctx &C, int center) {
         p_cost /= wn;
         pc[d + radius] = std::min(2.0f, p_cost);
     }
-    bool is_peak[n];
-    for (int i = 0; i < n; ++i) is_peak[i] = false;
-    int peak_count = 0, min_peak = 0;
-    float min_cost = 2.0f;
-    for (int i = 2; i < n - 2; ++i) {
-        if (pc[i - 1] > pc[i] && pc[i + 1] > pc[i]) {
-            is_peak[i] = true;
-            peak_count++;
-            if (pc[i] < min_cost) {
-                min_peak = i;
-                min_cost = pc[i];
-            }
-        }
-    }
-    if (std::abs(min_peak - radius) > C.prm.weak_peak_radius || pc[min_peak] > 0.5f) {
-        C.weak[center] = DPE_WEAK;
-        return;
-    }
-    if (peak_count == 1) {
-        C.weak[center] = pc[min_peak] <= 0.15f ? DPE_STRONG : DPE_WEAK;
-        return;
-    }
-    float var = 0.0f;
-    for (int i = 2; i < n - 2; ++i) {
-        if (is_peak[i] && i != min_peak) {
-            const float dist = pc[i] - min_cost;
-            var += dist * dist;
-        }
-    }
-    var = std::sqrt(var);
-    var /= (peak_count - 1);
-    C.weak[center] = var > 0.2f ? DPE_STRONG : DPE_WEAK;
+    C.weak[center] = DepthToWeakState(pc, C.prm.weak_peak_radius);
 }
 
 // LocalRefine, DPE.cu:2753-2839
@@ -2553,6 +2561,10 @@ void oracle_gen_edge_inform(int W, int H, const uint8_t *edge, const int32_t *la
         label_bound[2 * k + 1] = C.label_bound[k].y;
     }
     for (int i = 0; i < C.P; ++i) complex_map[i] = C.complex_map[i];
+}
+
+uint8_t oracle_depth_to_weak_state(const float *costs61, int weak_peak_radius) {
+    return DepthToWeakState(costs61, weak_peak_radius);
 }
 
 // Numerics test access.

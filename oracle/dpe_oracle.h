@@ -47,6 +47,8 @@ void oracle_fit_plane(const dpe_camera *cam, int W, int H, const float *planes, 
 /* GenEdgeInform (DPE.cu:2487-2595) over a whole image; label NULL = use_label off */
 void oracle_gen_edge_inform(int W, int H, const uint8_t *edge, const int32_t *label, const uint8_t *weak,
                             int strong_radius, int16_t *edge_neigh, float *complex_map, int16_t *label_bound);
+/* DepthToWeak's peak analysis (DPE.cu:2699-2750) over its 61 sampled costs; returns the pixel state */
+uint8_t oracle_depth_to_weak_state(const float *costs61, int weak_peak_radius);
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,8 @@ def load_oracle():
     lib.oracle_gen_edge_inform.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                            C.c_void_p, C.c_void_p]
     lib.oracle_gen_edge_inform.restype = None
+    lib.oracle_depth_to_weak_state.argtypes = [C.c_void_p, C.c_int]
+    lib.oracle_depth_to_weak_state.restype = C.c_uint8
     return lib
 
 

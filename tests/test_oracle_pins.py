@@ -297,3 +297,50 @@ def test_gen_edge_inform_against_vectorised_definitions(orc):
                 exp[same] = np.stack([nx, ny], -1)[same]
             exp[~sel] = -1
             assert np.array_equal(lb[:, :, i], exp), i
+
+
+def test_depth_to_weak_peak_analysis_known_answers(orc):
+    """DepthToWeak's classification of a 61-sample cost curve (DPE.cu:2699-2750)
+    on curves whose answer follows from the stated rules: a single strict
+    minimum at the centre is STRONG at cost <= 0.15 (0.15 itself included) and
+    WEAK above; a minimum more than weak_peak_radius samples off the centre, or
+    with cost > 0.5, is WEAK; with several minima the spread of the others
+    above the lowest (sqrt of the sum of squares / (count - 1)) decides at 0.2;
+    flat curves, plateaus and minima at the first / last two samples are no
+    minima (WEAK)."""
+    def state(pc, radius=2):
+        a = np.ascontiguousarray(pc, np.float32)
+        assert a.shape == (61,)
+        return orc.oracle_depth_to_weak_state(a.ctypes.data, radius)
+
+    def valley(at, cost, base=1.0):
+        pc = np.full(61, base, np.float32)
+        pc[at] = cost
+        return pc
+
+    S, Wk = abi.STRONG, abi.WEAK
+    assert state(valley(30, 0.1)) == S
+    assert state(valley(30, np.float32(0.15))) == S
+    assert state(valley(30, np.float32(0.1500001))) == Wk
+    assert state(valley(32, 0.1)) == S and state(valley(33, 0.1)) == Wk      # weak_peak_radius 2
+    assert state(valley(33, 0.1), radius=3) == S
+    assert state(valley(30, 0.6)) == Wk                                      # above 0.5
+    assert state(np.full(61, 2.0)) == Wk                                     # no minimum
+    pl = np.full(61, 1.0, np.float32)
+    pl[30] = pl[31] = 0.1                                                    # plateau: not strict
+    assert state(pl) == Wk
+    assert state(valley(1, 0.0)) == Wk and state(valley(59, 0.0)) == Wk      # outside samples 2..58
+    two = valley(30, 0.1)
+    two[45] = 0.9                                                            # spread 0.8 > 0.2
+    assert state(two) == S
+    two[45] = 0.25                                                           # spread 0.15
+    assert state(two) == Wk
+    three = valley(30, 0.1)
+    three[10], three[50] = 0.5, 0.6                                          # sqrt(0.4^2 + 0.5^2) / 2 = 0.32
+    assert state(three) == S
+    three[10], three[50] = 0.3, 0.3                                          # sqrt(2 * 0.2^2) / 2 = 0.14
+    assert state(three) == Wk
+    # the lowest minimum decides the position: a lower one off-centre makes it WEAK
+    off = valley(30, 0.1)
+    off[45] = 0.05
+    assert state(off) == Wk
