@@ -19,6 +19,11 @@
 //     (the reference reads it uninitialised for weak pixels with < 3 neighbours);
 //   * GenNeighbours' two unsequenced curand() calls (DPE.cu:2186-2187) draw the
 //     sign first, then the magnitude.
+//
+// Parity unpinned: GenNeighbours' direction search (DPE.cu:2107-2467).  Only its
+// agreement with the HIP path and the weak-pixel recovery on a known plane (which
+// a mirrored or permuted search would still pass) check it; PointinTriangle and
+// the RANSAC plane fit it feeds have closed-form checks of their own.
 #include "dpe_oracle.h"
 #include "oracle_numerics.h"
 

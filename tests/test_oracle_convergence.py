@@ -64,3 +64,33 @@ def test_refine_call_keeps_the_plane(orc, plane_run):
     rel = np.abs(res["depth"] - gt) / gt
     assert (rel[interior] < 0.01).mean() >= 0.999
     assert (res["pixel_states"][interior] == abi.STRONG).mean() >= 0.999
+
+
+def test_weak_path_recovers_forced_weak_pixels(orc, plane_run):
+    """The weak-texture path (GenNeighbours DPE.cu:2107-2467, RANSACToGetFitPlane
+    DPE.cu:2895-3128, CheckerboardPropagationWeak / RefinementWeak
+    DPE.cu:1672-1902, 1124-1216): interior pixels of the converged plane marked
+    WEAK with their depth 30 % off (the strong propagation skips WEAK pixels,
+    and LocalRefine's +-5 disparity steps cannot bridge that) get the plane back
+    from their strong neighbours in one REFINE iteration."""
+    s, store, interior = plane_run
+    ref = 2
+    rng = np.random.default_rng(5)
+    inner = np.zeros_like(interior)
+    inner[12:-12, 12:-12] = True
+    forced = inner & (rng.uniform(size=inner.shape) < 0.2)
+    st = {k: v.copy() for k, v in store.get(ref).items()}
+    st["pixel_states"][forced] = abi.WEAK
+    st["depth"][forced] *= np.float32(1.3)
+    store2 = host.StateStore()
+    for i in range(len(s.views)):
+        store2.put(i, st if i == ref else store.get(i))
+    pin = host.refine_call_inputs(s, ref, s.pairs[ref], store2, max_iterations=1, call_index=1)
+    assert (pin.weak_info[forced] == abi.WEAK).all()
+    out, _ = oracle_lib.run_oracle(orc, pin, 8)
+    res = host.assemble(out, pin.params.depth_min, pin.params.depth_max)
+    gt = s.views[ref].depth
+    rel = np.abs(res["depth"] - gt) / gt
+    assert forced.sum() > 1000
+    assert (np.abs(st["depth"] - gt)[forced] / gt[forced] > 0.25).all()
+    assert (rel[forced] < 0.01).mean() >= 0.99
