@@ -157,6 +157,38 @@ def test_config1b_weak_path_matches_oracle(ctx, orc, c1):
     assert na >= 0.995
 
 
+@pytest.mark.parametrize("W,H,n_views,n_src", [(61, 33, 2, 1), (47, 35, 32, 31)])
+def test_ragged_sizes_and_source_counts_match_oracle(ctx, orc, W, H, n_views, n_src):
+    """Odd widths and heights off every tile size, one source (the fewest a call
+    takes) and 31 sources (the most: 32 images, DPE.cpp:944-947).  H = 33 also
+    takes the reference's checkerboard launch quirk: its half-height grid
+    (H / 2 rounded up to 16 rows, doubled) stops at row 32, so the last row keeps
+    its random initialisation on both sides.  FIRST_INIT calls of every view, then
+    a REFINE call with the weak path fed by the oracle's states."""
+    s = scene.make_scene(W, H, 60.0, n_views, 30.0, seed=0x5EED00F1, n_src=n_src, supersample=1)
+    store = host.StateStore()
+    ref = n_views // 2
+    for i in range(n_views):
+        pin = host.first_call_inputs(s, i, s.pairs[i], max_iterations=2, call_index=0)
+        assert pin.struct.num_images == n_src + 1
+        dmin, dmax = pin.params.depth_min, pin.params.depth_max
+        res_o = host.assemble(oracle_lib.run_oracle(orc, pin, 8)[0], dmin, dmax)
+        store.put(i, res_o)
+        if i == ref:
+            res_g = host.assemble(host.run_hip(ctx, pin), dmin, dmax)
+            da, na, sa, va = _compare(res_g, res_o, f"{W}x{H} first")
+            assert da >= 0.995 and na >= 0.995 and sa >= 0.99
+    pin = host.refine_call_inputs(s, ref, s.pairs[ref], store, max_iterations=1, call_index=1)
+    assert (pin.weak_info == abi.WEAK).sum() > 0
+    out_g = host.run_hip(ctx, pin)
+    fam_o = []
+    out_o, _ = oracle_lib.run_oracle(orc, pin, 8, fam_o)
+    _check_counts(ctx.stats(), fam_o, f"{W}x{H} refine")
+    dmin, dmax = pin.params.depth_min, pin.params.depth_max
+    da, na, sa, va = _compare(host.assemble(out_g, dmin, dmax), host.assemble(out_o, dmin, dmax), f"{W}x{H} refine")
+    assert da >= 0.995 and na >= 0.995
+
+
 def test_fusion_matches_oracle(ctx, orc, c1):
     """RunFusion on ground-truth and on perturbed depth maps: the GPU fixpoint
     reproduces the sequential mask semantics, so point counts and points match."""

@@ -136,3 +136,29 @@ def test_tat_device_memory_and_capacity(ctx):
     bad = abi.fuse_tat_defaults(7)
     assert ctx.lib.dpe_fuse_tat(ctx.h, fs.views, fs.n, None, C.byref(bad), None, 0, C.byref(cnt)) == \
         abi.ERR_INVALID_ARGUMENT
+
+
+def test_fusions_with_31_sources_at_a_ragged_size(ctx, orc):
+    """32 views of 47x35 (odd sizes off every tile), each fused against 31
+    sources, the most a view can list: all three fusions give the oracle's
+    points, with and without holes in the depth maps."""
+    from fuse_util import fuse_hip, fuse_oracle
+    s = scene.make_scene(47, 35, 60.0, 32, 30.0, seed=0x5EED00F1, n_src=31, supersample=1)
+    assert all(len(p) == 31 for p in s.pairs)
+    rng = np.random.default_rng(23)
+    states = [np.where(v.weak, abi.WEAK, abi.STRONG).astype(np.uint8) for v in s.views]
+    normals = [v.normal for v in s.views]
+    for noise, holes in ((0.0, 0.0), (0.002, 0.05)):
+        depths = [v.depth * (1 + noise * rng.standard_normal(v.depth.shape)).astype(np.float32) for v in s.views]
+        for d in depths:
+            d[rng.uniform(size=d.shape) < holes] = 0.0
+        fs = FuseSet(s, depths, normals, states)
+        for fp in (abi.fuse_defaults(), abi.fuse_defaults(min_consistent=3)):
+            pg, po = fuse_hip(ctx, fs, fp), fuse_oracle(orc, fs, fp)
+            print(f"eth noise {noise}: oracle {len(po)} points, gpu {len(pg)}")
+            assert len(po) > 0 and _same(pg, po)
+        for variant in (INTER, ADV):
+            tp = abi.fuse_tat_defaults(variant)
+            pg, po = fuse_tat_hip(ctx, fs, tp), fuse_tat_oracle(orc, fs, tp)
+            print(f"tat {variant} noise {noise}: oracle {len(po)} points, gpu {len(pg)}")
+            assert len(po) > 0 and _same(pg, po)
