@@ -47,8 +47,10 @@ __global__ void k_make_quads_h(const float *__restrict__ img, uint2 *__restrict_
     if (idx >= n) return;
     const int iy = idx / (W + 1) - 1;
     const int ix = idx % (W + 1) - 1;
-    quads[idx] = make_uint2(to16(texel(img, W, H, ix, iy), fmt) | (to16(texel(img, W, H, ix + 1, iy), fmt) << 16),
-                            to16(texel(img, W, H, ix, iy + 1), fmt) | (to16(texel(img, W, H, ix + 1, iy + 1), fmt) << 16));
+    // column pairs: word 0 = (t00, t01), word 1 = (t10, t11), so one packed
+    // subtraction gives both rows' x differences (sample_fast)
+    quads[idx] = make_uint2(to16(texel(img, W, H, ix, iy), fmt) | (to16(texel(img, W, H, ix, iy + 1), fmt) << 16),
+                            to16(texel(img, W, H, ix + 1, iy), fmt) | (to16(texel(img, W, H, ix + 1, iy + 1), fmt) << 16));
 }
 // Bit 0 of *flag: some texel is not fp16-exact.  Bit 1: some texel is off the
 // quarter grid {k/4 : |k/4| < 256} (8-bit images and their factor-2 pyramid

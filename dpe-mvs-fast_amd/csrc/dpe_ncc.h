@@ -65,7 +65,8 @@ __device__ __forceinline__ float4 quad_at(const Dev &d, int v, int ix, int iy) {
     const uint32_t cell = (uint32_t)(iy + 1) * (uint32_t)(d.W + 1) + (uint32_t)(ix + 1);
     if constexpr (FMT != QUAD_FP32) {
         const uint2 h = *(const uint2 *)((const char *)(d.quads_h + (size_t)v * d.qs) + cell * 8u);
-        return make_float4(texel16<FMT>(h.x & 0xFFFFu), texel16<FMT>(h.x >> 16), texel16<FMT>(h.y & 0xFFFFu),
+        // word 0 = (t00, t01), word 1 = (t10, t11)
+        return make_float4(texel16<FMT>(h.x & 0xFFFFu), texel16<FMT>(h.y & 0xFFFFu), texel16<FMT>(h.x >> 16),
                            texel16<FMT>(h.y >> 16));
     } else {
         return *(const float4 *)((const char *)(d.quads + (size_t)v * d.qs) + cell * 16u);
@@ -96,12 +97,11 @@ __device__ __forceinline__ float sample_fast(const char *__restrict__ cell0, flo
     const float a = u - fx, b = w - fy;
     const uint32_t cell = (uint32_t)fmaf(fy, row_cells, fx);
     const uint2 h = *(const uint2 *)(cell0 + cell * 8u);
-    const half2_t p = __builtin_bit_cast(half2_t, h.x);  // (t.x, t.y)
-    const half2_t q = __builtin_bit_cast(half2_t, h.y);  // (t.z, t.w)
-    const half2_t lo = {p.x, q.x}, hi = {p.y, q.y};
-    const half2_t dd = hi - lo;
+    const half2_t p = __builtin_bit_cast(half2_t, h.x);  // (t.x, t.z)
+    const half2_t q = __builtin_bit_cast(half2_t, h.y);  // (t.y, t.w)
+    const half2_t dd = q - p;
     const float top = fmaf(a, (float)dd.x, (float)p.x);
-    const float bot = fmaf(a, (float)dd.y, (float)q.x);
+    const float bot = fmaf(a, (float)dd.y, (float)p.y);
     return fmaf(b, bot - top, top);
 }
 // Every tap of the window [x0, x1] x [y0, y1] projects to [1/64, W-1-1/64] x
