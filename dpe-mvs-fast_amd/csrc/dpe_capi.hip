@@ -61,7 +61,7 @@ enum BufId {
     B_POSTBL, B_POSTDISP, B_POSTWN, B_POSTFLAGS, B_PCOST, B_COUNTERS, B_GIVEN_PLANES, B_GIVEN_COSTS,
     B_F_VIEWS, B_F_DEPTH, B_F_NORMAL, B_F_STATES, B_F_COLOR, B_F_MASK, B_F_CLAIM, B_F_USED0, B_F_USED1, B_F_ACC0,
     B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_POSTPLANES, B_WEAKLIST, B_ADJ_WEIGHTS,
-    B_ADJ_GRADS, B_QUADS_H, B_FLAG, B_WPRE, B_WSUM, B_VC64, B_COUNT
+    B_ADJ_GRADS, B_QUADS_H, B_FLAG, B_WPRE, B_WSUM, B_VC64, B_WEAKCOUNTS, B_WEAKTEMP, B_COUNT
 };
 
 struct Fail {
@@ -492,42 +492,39 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
         for (int i = 0; i < N; ++i) up.to(g + P * i, in->geom_depths[i], sizeof(float) * P);
         d.gdepth = g;
     }
-    // pixel states and the weak index map (DPE.cpp:989-1030)
-    std::vector<uint8_t> weak_h(P, DPE_STRONG);
-    if (prm->use_APD) {
-        if (dev_in) {
-            HIPCHK(hipMemcpy(weak_h.data(), in->weak_info, P, hipMemcpyDeviceToHost));
-        } else {
-            std::memcpy(weak_h.data(), in->weak_info, P);
-        }
-    }
-    std::vector<int> nmap_h(P, 0);
+    // pixel states and the weak index map (DPE.cpp:989-1030), and the weak
+    // pixel lists: all (GenNeighbours, RANSAC) and per colour as the HALF launch
+    // covers them (weak propagation, DPE.cu:1868-1902) -- built on the device;
+    // only the three list lengths come back to the host (grids and sizes)
+    d.weak = (uint8_t *)buffer(ctx, B_WEAK, P);
+    int *nmap = (int *)buffer(ctx, B_NMAP, sizeof(int) * P);
+    const size_t half_cap = (P + 1) / 2;
+    int *dwl = (int *)buffer(ctx, B_WEAKLIST, sizeof(int) * (P + 2 * half_cap + 1));
     int weak_count = 0;
-    for (size_t i = 0; i < P; ++i)
-        if (weak_h[i] == DPE_WEAK) nmap_h[i] = weak_count++;
-    // weak pixel lists: all (GenNeighbours, RANSAC) and per colour as the HALF
-    // launch covers them (weak propagation, DPE.cu:1868-1902)
-    std::vector<int> wl;
-    wl.reserve((size_t)weak_count * 2 + 2);
-    for (size_t i = 0; i < P; ++i)
-        if (weak_h[i] == DPE_WEAK) wl.push_back((int)i);
     int cc[2] = {0, 0};
-    for (int c = 0; c < 2; ++c) {
-        for (size_t i = 0; i < P; ++i) {
-            const int x = (int)(i % d.W), y = (int)(i / d.W);
-            if (weak_h[i] == DPE_WEAK && y < d.half_rows && ((x + y) & 1) == c) {
-                wl.push_back((int)i);
-                cc[c]++;
-            }
-        }
+    if (prm->use_APD) {
+        up.to(d.weak, in->weak_info, P);
+        int *dcnt = (int *)buffer(ctx, B_WEAKCOUNTS, sizeof(int) * 3);
+        size_t temp_bytes = 0;
+        HIPCHK(weak_lists_temp_bytes((int)P, &temp_bytes));
+        void *temp = buffer(ctx, B_WEAKTEMP, std::max(temp_bytes, (size_t)1));
+        HIPCHK(launch_weak_lists(d.weak, d.W, (int)P, d.half_rows, nmap, dwl, dwl + P, dwl + P + half_cap, dcnt, temp,
+                                 temp_bytes, ctx->stream));
+        int cnt_h[3] = {0, 0, 0};
+        HIPCHK(hipMemcpyAsync(cnt_h, dcnt, sizeof(cnt_h), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        weak_count = cnt_h[0];
+        cc[0] = cnt_h[1];
+        cc[1] = cnt_h[2];
+    } else {
+        HIPCHK(hipMemsetAsync(d.weak, DPE_STRONG, P, ctx->stream));
+        HIPCHK(hipMemsetAsync(nmap, 0, sizeof(int) * P, ctx->stream));
     }
-    int *dwl = (int *)buffer(ctx, B_WEAKLIST, sizeof(int) * (wl.size() + 1));
-    if (!wl.empty()) HIPCHK(hipMemcpyAsync(dwl, wl.data(), sizeof(int) * wl.size(), hipMemcpyHostToDevice, ctx->stream));
     d.weak_list = dwl;
     d.weak_count = weak_count;
-    d.weak_colour_list[0] = dwl + weak_count;
+    d.weak_colour_list[0] = dwl + P;
     d.weak_colour_count[0] = cc[0];
-    d.weak_colour_list[1] = dwl + weak_count + cc[0];
+    d.weak_colour_list[1] = dwl + P + half_cap;
     d.weak_colour_count[1] = cc[1];
     {
         const size_t n = (size_t)cc[0] + cc[1] + 1;
@@ -538,10 +535,6 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
         d.weak_wsum[0] = wsum;
         d.weak_wsum[1] = wsum + (size_t)3 * kNeighbourNum * cc[0];
     }
-    d.weak = (uint8_t *)buffer(ctx, B_WEAK, P);
-    int *nmap = (int *)buffer(ctx, B_NMAP, sizeof(int) * P);
-    HIPCHK(hipMemcpyAsync(d.weak, weak_h.data(), P, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(nmap, nmap_h.data(), sizeof(int) * P, hipMemcpyHostToDevice, ctx->stream));
     d.nmap = nmap;
     // state buffers
     d.planes = (float4 *)buffer(ctx, B_PLANES, sizeof(float4) * P);

@@ -16,6 +16,8 @@
 // Every kernel cites the reference function it restates.
 #include "dpe_ncc.h"
 
+#include <hipcub/hipcub.hpp>
+
 namespace dpe {
 
 enum { MODE_REFINE = 0, MODE_POST = 1 };
@@ -64,6 +66,29 @@ __global__ void k_check_fp16_exact(const float *__restrict__ img, size_t n, int 
     }
     const int bits = (bad ? 1 : 0) | (off_grid ? 2 : 0);
     if (bits) atomicOr(flag, bits);
+}
+
+// ---- weak-pixel index map and lists (the input side of DPE.cpp:989-1030 and
+// of the weak launches, DPE.cu:1868-1902), by stable selection on the device
+struct WeakSelect {  // a WEAK pixel, of colour `colour` in rows < half_rows (colour -1: any)
+    const uint8_t *weak;
+    int W, half_rows, colour;
+    __device__ __forceinline__ bool operator()(int i) const {
+        if (weak[i] != DPE_WEAK) return false;
+        if (colour < 0) return true;
+        const int x = i % W, y = i / W;
+        return y < half_rows && ((x + y) & 1) == colour;
+    }
+};
+struct WeakFlag {
+    const uint8_t *weak;
+    __device__ __forceinline__ int operator()(int i) const { return weak[i] == DPE_WEAK ? 1 : 0; }
+};
+using PixelIter = hipcub::CountingInputIterator<int>;
+using WeakFlagIter = hipcub::TransformInputIterator<int, WeakFlag, PixelIter>;
+__global__ void k_nmap_clear_others(const uint8_t *__restrict__ weak, int *__restrict__ nmap, int P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P && weak[i] != DPE_WEAK) nmap[i] = 0;
 }
 
 // Sums of one bilateral window computed directly (any radius/increment).
@@ -2172,6 +2197,29 @@ void launch_make_quads_h(const Dev &d, const float *img, uint2 *quads, int fmt, 
 }
 void launch_check_fp16_exact(const float *img, size_t n, int *flag, hipStream_t s) {
     hipLaunchKernelGGL(k_check_fp16_exact, dim3(1024), dim3(256), 0, s, img, n, flag);
+}
+hipError_t weak_lists_temp_bytes(int P, size_t *bytes) {
+    size_t a = 0, b = 0;
+    hipError_t e = hipcub::DeviceSelect::If(nullptr, a, PixelIter(0), (int *)nullptr, (int *)nullptr, P,
+                                            WeakSelect{nullptr, 1, 0, -1});
+    if (e != hipSuccess) return e;
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, b, WeakFlagIter(PixelIter(0), WeakFlag{nullptr}), (int *)nullptr, P);
+    *bytes = std::max(a, b);
+    return e;
+}
+hipError_t launch_weak_lists(const uint8_t *weak, int W, int P, int half_rows, int *nmap, int *all, int *col0,
+                             int *col1, int *counts, void *temp, size_t temp_bytes, hipStream_t s) {
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, WeakFlagIter(PixelIter(0), WeakFlag{weak}),
+                                                    nmap, P, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_nmap_clear_others, grid1(P), dim3(256), 0, s, weak, nmap, P);
+    int *out[3] = {all, col0, col1};
+    for (int k = 0; k < 3; ++k) {
+        e = hipcub::DeviceSelect::If(temp, temp_bytes, PixelIter(0), out[k], counts + k, P,
+                                     WeakSelect{weak, W, half_rows, k - 1}, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipGetLastError();
 }
 void launch_gen_edge_inform(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_gen_edge_inform, grid1(d.P), dim3(256), 0, s, d);
