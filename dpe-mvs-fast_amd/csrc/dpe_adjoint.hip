@@ -296,4 +296,9 @@ void launch_ncc_adjoint(const Dev &d, const float4 *tpd, const float *view_w, fl
                        view_w, cost, grad);
 }
 
+// An empty launch: HIP loads this file's code object at its first kernel launch
+// (tens of ms), so a context does it once when it is created.
+__global__ void k_preload_adjoint() {}
+void preload_adjoint(hipStream_t s) { hipLaunchKernelGGL(k_preload_adjoint, dim3(1), dim3(1), 0, s); }
+
 }  // namespace dpe

@@ -1070,6 +1070,16 @@ dpe_status dpe_create(int device, dpe_ctx **out) {
         delete ctx;
         return DPE_ERR_HIP;
     }
+    // load every code object of the library now rather than inside the first call
+    preload_kernels(ctx->stream);
+    preload_adjoint(ctx->stream);
+    preload_fuse(ctx->stream);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        g_create_error = "cannot load the library's kernels";
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return DPE_ERR_HIP;
+    }
     *out = ctx;
     return DPE_OK;
 }

@@ -405,4 +405,9 @@ hipError_t launch_tat_compact(const FuseDev &f, int vi, int P, const uint8_t *ac
     return hipGetLastError();
 }
 
+// An empty launch: HIP loads this file's code object at its first kernel launch
+// (tens of ms), so a context does it once when it is created.
+__global__ void k_preload_fuse() {}
+void preload_fuse(hipStream_t s) { hipLaunchKernelGGL(k_preload_fuse, dim3(1), dim3(1), 0, s); }
+
 }  // namespace dpe

@@ -2360,4 +2360,9 @@ void launch_ncc_given(const Dev &d, int num_hyps, const float4 *planes, float *c
                            (const uint32_t *)nullptr, costs);
 }
 
+// An empty launch: HIP loads this file's code object at its first kernel launch
+// (tens of ms), so a context does it once when it is created.
+__global__ void k_preload_kernels() {}
+void preload_kernels(hipStream_t s) { hipLaunchKernelGGL(k_preload_kernels, dim3(1), dim3(1), 0, s); }
+
 }  // namespace dpe

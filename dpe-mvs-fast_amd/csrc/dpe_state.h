@@ -118,6 +118,10 @@ enum NccFamily {
 
 // Launch interface (dpe_kernels.hip); every function enqueues on `stream`.
 // Kernels named ncc_* evaluate bilateral NCC windows (the timed hot path).
+// one empty launch per source file: loads its code object (dpe_create)
+void preload_kernels(hipStream_t s);
+void preload_adjoint(hipStream_t s);
+void preload_fuse(hipStream_t s);
 void launch_make_quads(const Dev &d, const float *img, float4 *quads, hipStream_t s);
 void launch_make_quads_h(const Dev &d, const float *img, uint2 *quads, int fmt, hipStream_t s);
 // sets *flag (int, zeroed by the caller) when a value of img[0..n) is not fp16-exact
