@@ -531,16 +531,34 @@ __device__ __forceinline__ bool weighted_slot(const Dev &d, int Qn, int q, int p
     return true;
 }
 
+// Post-pass samples whose costs the peak analysis never reads: DepthToWeak's
+// d = +30 (its scan stops at d = +29, DPE.cu:2700-2712) and d = -30 unless
+// weak_peak_radius >= 30 (read only as the minimum when there is no peak,
+// which the radius test already calls WEAK otherwise).
+__device__ __forceinline__ bool post_sample_unread(const Dev &d, int slot) {
+    return slot == 60 || (slot == 0 && d.prm.weak_peak_radius < 30);
+}
+// LocalRefine's samples d = -5..5 (slots 62..72) use the planes of DepthToWeak's
+// d = -5..5 (slots 25..35): same normal, same depth expression.  Where a pixel
+// runs both steps the DepthToWeak wave evaluates each view once and folds it
+// in both steps' orders (kPostTwin = 37 slots apart).
+constexpr int kPostTwin = 37;
+__device__ __forceinline__ bool post_has_twin(int slot, uint8_t fl) { return slot >= 25 && slot <= 35 && (fl & 2); }
+__device__ __forceinline__ bool post_is_twin(int slot, uint8_t fl) { return slot >= 62 && (fl & 1); }
+
 // Weighted sums over views.
 //   MODE_REFINE: slot s of the refinement hypotheses (PlaneHypothesisRefinementStrong,
 //     DPE.cu:1101-1113): sum over views with weight > 0 of w*cost, / weight_norm.
 //   MODE_POST: DepthToWeak samples (DPE.cu:2667-2690) and LocalRefine
 //     (DPE.cu:2776-2835), all over the pixel's selected views.
+//   With `twin` (MODE_POST, DepthToWeak slots 25..35) the LocalRefine sum of
+//   the same plane goes to *twin as well.
 template <int NT, int MODE, bool PREC>
 __device__ __forceinline__ float weighted_sum(const Dev &d, const void *Sv, int lane, int x, int y,
                                               int slot, const uint8_t *vw, uint32_t sel, const float4 &pl, float wn,
-                                              unsigned long long &hyps, unsigned long long &taps) {
-    float acc = 0.0f;
+                                              unsigned long long &hyps, unsigned long long &taps,
+                                              float *twin = nullptr) {
+    float acc = 0.0f, acc2 = 0.0f;
     const bool geom = d.prm.geom_consistency != 0;
     const float gf = d.prm.geom_factor;
     for (int v = 1; v < d.N; ++v) {
@@ -557,15 +575,21 @@ __device__ __forceinline__ float weighted_sum(const Dev &d, const void *Sv, int 
         if (MODE == MODE_REFINE) {
             acc += wv * c;
         } else if (slot <= 61) {
+            const float g = geom ? geom_cost(d, x, y, v, pl) : 0.0f;
             float tc = c;
-            if (geom) tc += gf * geom_cost(d, x, y, v, pl);
+            if (geom) tc += gf * g;
             acc += tc * wv;
+            if (twin) {  // LocalRefine's order (DPE.cu:2822-2825)
+                acc2 += c * wv;
+                if (geom) acc2 += gf * g * wv;
+            }
         } else {
             acc += c * wv;
             if (geom) acc += gf * geom_cost(d, x, y, v, pl) * wv;
         }
     }
     acc /= wn;
+    if (twin) *twin = acc2 / wn;
     if (MODE == MODE_POST && slot < 61) acc = fminf(2.0f, acc);
     return acc;
 }
@@ -591,7 +615,15 @@ __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_weighted(Dev d, int colo
     uint32_t sel = 0;
     if (any && slot < nslot && weighted_slot<MODE>(d, Qn, q, p, slot, pl, wn, sel)) {
         const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
-        out[(size_t)slot * Qn + q] = weighted_sum<NT, MODE, PREC>(d, &S, lane, x, y, slot, vw, sel, pl, wn, hyps, taps);
+        const uint8_t fl = MODE == MODE_POST ? d.post_flags[p] : 0;
+        if (MODE == MODE_POST && post_has_twin(slot, fl)) {
+            float tw;
+            out[(size_t)slot * Qn + q] =
+                weighted_sum<NT, MODE, PREC>(d, &S, lane, x, y, slot, vw, sel, pl, wn, hyps, taps, &tw);
+            out[(size_t)(slot + kPostTwin) * Qn + q] = tw;
+        } else if (!(MODE == MODE_POST && post_is_twin(slot, fl))) {
+            out[(size_t)slot * Qn + q] = weighted_sum<NT, MODE, PREC>(d, &S, lane, x, y, slot, vw, sel, pl, wn, hyps, taps);
+        }
     }
     count_add(d.counters, (MODE == MODE_REFINE ? kFamRefine : kFamPost), hyps, taps);
 }
@@ -610,7 +642,15 @@ __global__ __launch_bounds__(256) void k_ncc_weighted_direct(Dev d, int colour, 
     if (active) active = weighted_slot<MODE>(d, Qn, q, p, slot, pl, wn, sel);
     if (active) {
         const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
-        out[(size_t)slot * Qn + q] = weighted_sum<0, MODE, false>(d, nullptr, 0, x, y, slot, vw, sel, pl, wn, hyps, taps);
+        const uint8_t fl = MODE == MODE_POST ? d.post_flags[p] : 0;
+        if (MODE == MODE_POST && post_has_twin(slot, fl)) {
+            float tw;
+            out[(size_t)slot * Qn + q] =
+                weighted_sum<0, MODE, false>(d, nullptr, 0, x, y, slot, vw, sel, pl, wn, hyps, taps, &tw);
+            out[(size_t)(slot + kPostTwin) * Qn + q] = tw;
+        } else if (!(MODE == MODE_POST && post_is_twin(slot, fl))) {
+            out[(size_t)slot * Qn + q] = weighted_sum<0, MODE, false>(d, nullptr, 0, x, y, slot, vw, sel, pl, wn, hyps, taps);
+        }
     }
     count_add(d.counters, (MODE == MODE_REFINE ? kFamRefine : kFamPost), hyps, taps);
 }
@@ -2107,6 +2147,11 @@ __global__ void k_post_planes(Dev d) {
     for (int slot = 0; slot < kPostSlots; ++slot) {
         const bool dtw = slot < 61;
         if (!((dtw && (fl & 1)) || (!dtw && (fl & 2)))) continue;
+        if (post_sample_unread(d, slot)) {
+            d.pcost[(size_t)slot * P + p] = 2.0f;
+            d.post_planes[(size_t)slot * P + p] = make_float4(0.0f, 0.0f, 0.0f, __builtin_nanf(""));
+            continue;
+        }
         float pd = op.w;
         if (slot != 61) {
             const int k = dtw ? slot - 30 : slot - 67;

@@ -1945,6 +1945,10 @@ void DepthToWeak(Ctx &C, int center) {
     const int n = 2 * radius + 1;
     float pc[n];
     for (int d = -radius; d <= radius; d += 1) {
+        // samples the peak analysis never reads (d = +30 always, d = -30 unless
+        // weak_peak_radius >= 30) are computed as the reference does but counted
+        // apart as dead evaluations
+        tl_family = (d == radius || (d == -radius && C.prm.weak_peak_radius < 30)) ? kFamDead : kFamPost;
         const float pd = cam.K[0] * base_line / (disp + d);
         if (pd < C.prm.depth_min || pd > C.prm.depth_max) {
             pc[d + radius] = 2.0f;
@@ -2004,12 +2008,17 @@ void LocalRefine(Ctx &C, int center) {
     }
     if (wn == 0 || valid == 0) return;
     tl_family = kFamPost;
+    // where DepthToWeak ran its samples, its d = -5..5 planes are these: the
+    // per-view costs are the same values, so they count once (as DepthToWeak's)
+    const int min_margin = 6;
+    const bool dtw_ran = !(px < min_margin || py < min_margin || px >= W - min_margin || py >= C.H - min_margin);
     cost_now /= wn;
     base_line /= valid;
     const float disp = cam.K[0] * base_line / od;
     const int radius = 5;
     float min_cost = 2.0f;
     float best_depth = od;
+    tl_family = dtw_ran ? kFamDead : kFamPost;
     for (int d = -radius; d <= radius; ++d) {
         const float pd = cam.K[0] * base_line / (disp + d);
         if (pd < C.prm.depth_min || pd > C.prm.depth_max) continue;
