@@ -1869,6 +1869,32 @@ __global__ void k_ransac(Dev d, int iter) {
     }
 }
 
+// The candidate planes of CheckerboardPropagationWeak (DPE.cu:1690-1703): the
+// STRONG ones among the 8 GenNeighbours points of weak pixel p.  src[i] = the
+// first candidate at the same pixel as candidate i (i itself when distinct):
+// different search directions often reach the same nearest strong pixel, and
+// a candidate at the same pixel has the same plane and so the same costs.
+// Returns the number of distinct candidates.
+__device__ __forceinline__ int weak_candidates(const Dev &d, int p, int positions[8], bool flag[8], int src[8]) {
+    int u = 0;
+    for (int i = 0; i < 8; ++i) {
+        flag[i] = false;
+        positions[i] = 0;
+        src[i] = i;
+        const short2 np = neighbour_point(d, p, i + 1);
+        if (np.x == -1 || np.y == -1 || d.weak[np.x + np.y * d.W] != DPE_STRONG) continue;
+        positions[i] = np.x + np.y * d.W;
+        flag[i] = true;
+        for (int k = 0; k < i; ++k)
+            if (flag[k] && positions[k] == positions[i]) {
+                src[i] = k;
+                break;
+            }
+        if (src[i] == i) u++;
+    }
+    return u;
+}
+
 // CheckerboardPropagationWeak + PlaneHypothesisRefinementWeak, DPE.cu:1672-1866, 1124-1216
 __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int iter, unsigned long long &hyps,
                                                   unsigned long long &taps) {
@@ -1889,20 +1915,24 @@ __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int 
         for (int j = 0; j < 32; ++j) cost_array[i][j] = 0.0f;
     cost_array[0][0] = 2.0f;
     bool flag[8];
-    int positions[8];
+    int positions[8], src[8];
     float4 new_plane[8];
+    weak_candidates(d, p, positions, flag, src);
+    int uniq[8], nu = 0;
     for (int i = 0; i < 8; ++i) {
-        flag[i] = false;
-        positions[i] = 0;
-        new_plane[i] = make_float4(0, 0, 0, 0);
-        const short2 np = neighbour_point(d, p, i + 1);
-        if (np.x == -1 || np.y == -1 || d.weak[np.x + np.y * W] != DPE_STRONG) continue;
-        positions[i] = np.x + np.y * W;
-        flag[i] = true;
-        const float4 pl = d.planes[positions[i]];
-        for (int v = 1; v < d.N; ++v) cost_array[i][v - 1] = ncc_new_pre(d, x, y, v, pl, colour, li, hyps, taps);
-        new_plane[i] = pl;
+        new_plane[i] = flag[i] ? d.planes[positions[i]] : make_float4(0, 0, 0, 0);
+        if (flag[i] && src[i] == i) uniq[nu++] = i;
     }
+    // the distinct candidates back to back (a wave loops to its lanes' largest
+    // count rather than over all 8 slots), then the repeated ones copied
+    for (int j = 0; j < nu; ++j) {
+        const int i = uniq[j];
+        const float4 pl = new_plane[i];
+        for (int v = 1; v < d.N; ++v) cost_array[i][v - 1] = ncc_new_pre(d, x, y, v, pl, colour, li, hyps, taps);
+    }
+    for (int i = 0; i < 8; ++i)
+        if (flag[i] && src[i] != i)
+            for (int v = 1; v < d.N; ++v) cost_array[i][v - 1] = cost_array[src[i]][v - 1];
     float priors[32];
     for (int j = 0; j < NV; ++j) priors[j] = 0.0f;
     for (int i = 0; i < 8; ++i) {

@@ -1122,7 +1122,13 @@ void CheckerboardPropagationWeak(Ctx &C, int px, int py, int iter) {
         }
         positions[i] = np.x + np.y * W;
         flag[i] = true;
+        // a candidate at the same pixel as an earlier one has the same plane and
+        // costs: computed as the reference does, counted apart as dead
+        bool repeated = false;
+        for (int k = 0; k < i; ++k) repeated = repeated || (flag[k] && positions[k] == positions[i]);
+        tl_family = repeated ? kFamDead : kFamWeak;
         CostVectorNew(C, px, py, C.planes[positions[i]], cost_array[i], C.selected);
+        tl_family = kFamWeak;
         new_plane[i] = C.planes[positions[i]];
     }
     uint8_t *vw = &C.view_weight[(size_t)center * MAXIMG];
