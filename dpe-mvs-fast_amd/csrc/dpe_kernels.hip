@@ -1970,8 +1970,12 @@ __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int 
             if (final_costs[i] <= mc) { mc = final_costs[i]; mi = i; }
     }
     const float4 cur = d.planes[p];
+    // Views of weight 0 add exactly 0 to this sum and to the final one below:
+    // the costs are finite (clamped to [0, 2] and [0, 3]) and the sums start
+    // at +0, so those views are not evaluated.
     float cost_now = 0.0f;
     for (int v = 1; v < d.N; ++v) {
+        if (vwl[v - 1] == 0) continue;
         const float c = ncc_new_pre(d, x, y, v, cur, colour, li, hyps, taps);
         if (geom) cost_now += vwl[v - 1] * (c + gf * geom_cost(d, x, y, v, cur));
         else cost_now += vwl[v - 1] * c;
@@ -2054,7 +2058,8 @@ __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int 
     }
     // final cost with the Old NCC (DPE.cu:1849-1865)
     float c2 = 0.0f;
-    for (int v = 1; v < d.N; ++v) c2 += vwl[v - 1] * ncc_old_direct(d, x, y, v, final_plane, hyps, taps);
+    for (int v = 1; v < d.N; ++v)
+        if (vwl[v - 1] > 0) c2 += vwl[v - 1] * ncc_old_direct(d, x, y, v, final_plane, hyps, taps);
     d.costs[p] = c2 / wn;
 }
 

@@ -1165,7 +1165,13 @@ void CheckerboardPropagationWeak(Ctx &C, int px, int py, int iter) {
     const int mi = FindMinCostIndex(final_costs, 8);
     float cvn[32] = {2.0f};
     const F4 cur = C.planes[center];
-    CostVectorNew(C, px, py, cur, cvn, C.selected);
+    // views of weight 0 add exactly 0 to the two weighted sums below (finite
+    // costs): computed as the reference does, counted apart as dead
+    for (int i = 1; i < C.N; ++i) {
+        tl_family = vw[i - 1] > 0 ? kFamWeak : kFamDead;
+        cvn[i - 1] = ComputeBilateralNCCNew(C, px, py, i, cur, C.selected);
+    }
+    tl_family = kFamWeak;
     float cost_now = 0.0f;
     for (int i = 0; i < NV; ++i) {
         if (C.prm.geom_consistency)
@@ -1198,7 +1204,11 @@ void CheckerboardPropagationWeak(Ctx &C, int px, int py, int iter) {
     }
     // final cost with the Old NCC (DPE.cu:1849-1865); the radius swap there has no effect
     float cvo[32] = {2.0f};
-    CostVectorOld(C, px, py, C.planes[center], cvo);
+    for (int i = 1; i < C.N; ++i) {
+        tl_family = vw[i - 1] > 0 ? kFamWeak : kFamDead;
+        cvo[i - 1] = ComputeBilateralNCCOld(C, px, py, i, C.planes[center]);
+    }
+    tl_family = kFamWeak;
     float c2 = 0.0f;
     for (int i = 0; i < NV; ++i) c2 += vw[i] * cvo[i];
     c2 /= wnorm;
