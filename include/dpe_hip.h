@@ -152,7 +152,8 @@ typedef struct dpe_ctx dpe_ctx;
 
 /* Context = one GPU, one HIP stream and a device memory pool (GpuWorkspace,
  * DPE.h:67-84).  One context per host thread.  Replaces cudaSetDevice
- * (main.cpp:343) and the GpuWorkspace constructor. */
+ * (main.cpp:343) and the GpuWorkspace constructor.  Loads the library's GPU
+ * code on the device (once per process; the first call pays no load). */
 dpe_status dpe_create(int device, dpe_ctx **out);
 void dpe_destroy(dpe_ctx *ctx);
 /* Last error message of this context (or of dpe_create when ctx is NULL). */
