@@ -1,6 +1,6 @@
 """Parity of the HIP path (through the C-ABI) with the CPU oracle.
 
-Bar (north_star): >= 99.5 % of pixels with |dd|/d < 1e-3 and normals within
+Bar: >= 99.5 % of pixels with |dd|/d < 1e-3 and normals within
 0.5 degrees.  Both sides follow the same numerics contract, so most outputs are
 expected to be bit-identical; the thresholds are the stated tolerance.
 """
