@@ -114,7 +114,7 @@ def _compare(res_g, res_o, label):
 
 
 def test_config1_first_init_matches_oracle(ctx, orc, c1):
-    """Config 1: one FIRST_INIT call, 3 checkerboard iterations, FP32."""
+    """320x240 scene, 4 sources: one FIRST_INIT call, 3 checkerboard iterations, FP32."""
     ref = c1.meta["ref_mid"]
     pin = host.first_call_inputs(c1, ref, c1.pairs[ref], max_iterations=3)
     out_g = host.run_hip(ctx, pin)
@@ -131,7 +131,7 @@ def test_config1_first_init_matches_oracle(ctx, orc, c1):
 
 
 def test_config1b_weak_path_matches_oracle(ctx, orc, c1):
-    """Config 1b: a REFINE_ITER call with APD, edge sampling, the weak path and
+    """320x240 scene, 4 sources: a REFINE_ITER call with APD, edge sampling, the weak path and
     geometric consistency on, fed by a FIRST_INIT call of every view."""
     store_g, store_o = host.StateStore(), host.StateStore()
     ref = c1.meta["ref_mid"]
@@ -207,7 +207,7 @@ def test_fusion_matches_oracle(ctx, orc, c1):
         assert abs(len(pg) - len(po)) <= 0.005 * len(po)
         if len(pg) == len(po):
             assert np.array_equal(pg["x"], po["x"]) and np.array_equal(pg["b"], po["b"])
-    # config 2's rule: at least 3 consistent views
+    # the 1600x1200 workload's rule: at least 3 consistent views
     fp3 = abi.fuse_defaults(min_consistent=3)
     assert abs(len(fuse_hip(ctx, fs, fp3)) - len(fuse_oracle(orc, fs, fp3))) <= 0.005 * len(po)
 
