@@ -69,11 +69,17 @@ def fuse_tat_hip(ctx, fs, tp, blocks=None):
     return np.frombuffer(pts, dtype=POINT_DTYPE)[:cnt.value]
 
 
-def fuse_oracle(orc, fs, fp):
+def fuse_oracle(orc, fs, fp, capacity=None):
+    """The oracle's RunFusion points; with `capacity`, one pass that writes up to
+    that many points (a second pass only if there are more)."""
     cnt = C.c_int64()
-    assert orc.oracle_fuse(fs.views, fs.n, C.byref(fp), None, 0, C.byref(cnt)) == 0
-    pts = (abi.Point * max(cnt.value, 1))()
-    assert orc.oracle_fuse(fs.views, fs.n, C.byref(fp), pts, cnt.value, C.byref(cnt)) == 0
+    if capacity is None:
+        assert orc.oracle_fuse(fs.views, fs.n, C.byref(fp), None, 0, C.byref(cnt)) == 0
+        capacity = cnt.value
+    pts = (abi.Point * max(capacity, 1))()
+    assert orc.oracle_fuse(fs.views, fs.n, C.byref(fp), pts, capacity, C.byref(cnt)) == 0
+    if cnt.value > capacity:
+        return fuse_oracle(orc, fs, fp, cnt.value)
     return np.frombuffer(pts, dtype=np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("b", "u1"), ("g", "u1"),
                                               ("r", "u1"), ("pad", "u1")]))[:cnt.value]
 

@@ -20,3 +20,21 @@ def normal_agreement(n_gpu, n_ref, max_deg=0.5):
     ang = np.degrees(np.arccos(np.clip(dot, -1, 1)))
     same = np.all(a == b, -1)
     return float((same | (ang <= max_deg)).mean())
+
+
+def bitwise_fractions(out_g, out_o):
+    """Fractions of pixels whose raw outputs are bit-identical between two
+    ProblemOutputs (planes: all four floats; costs; pixel states; selected
+    views), compared as integers so that NaNs compare by their bits."""
+    def frac(a, b, axis=None):
+        a = np.ascontiguousarray(a)
+        b = np.ascontiguousarray(b)
+        if a.dtype == np.float32:
+            a, b = a.view(np.uint32), b.view(np.uint32)
+        eq = a == b
+        if axis is not None:
+            eq = eq.all(axis)
+        return float(eq.mean())
+    return dict(planes=frac(out_g.planes, out_o.planes, -1), costs=frac(out_g.costs, out_o.costs),
+                pixel_states=frac(out_g.pixel_states, out_o.pixel_states),
+                selected_views=frac(out_g.selected_views, out_o.selected_views))
