@@ -10,9 +10,11 @@
 
 #include <algorithm>
 #include <chrono>
+#include <memory>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -46,9 +48,27 @@ struct dpe_ctx {
     std::vector<dpe_tune> tunes;
     hipGraphExec_t graph = nullptr;  // the last captured schedule (graph form), updated in place when it fits
     int graph_rebuilds = 0, graph_updates = 0;
+    // allocation-free calls (dpe_ncc_cost_noalloc): while `arena` is set, buffer()
+    // carves every buffer of the call from this device's scratch array instead of
+    // the pool, and never allocates
+    char *arena = nullptr;
+    size_t arena_bytes = 0, arena_used = 0;
 };
 
 static thread_local std::string g_create_error;
+
+// Scratch of the allocation-free entry point (dpe_ncc_cost_noalloc): every GPU
+// the library runs on holds its own copy of this array from the library's first
+// context on that device until the process ends.  Host code never uses the name
+// `scratch` as an address (on the host it is only a stand-in for the symbol):
+// dpe_create looks up each device's address once, with that device current, and
+// the calls use that address.  Calls that overlap on one GPU share the array and
+// race; each call rewrites what it reads.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static char *g_scratch_ptr[kMaxScratchDevices];
 
 namespace {
 
@@ -78,9 +98,19 @@ struct Fail {
     } while (0)
 
 void *buffer(dpe_ctx *ctx, int id, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (ctx->arena) {
+        const size_t off = (ctx->arena_used + 255) & ~(size_t)255;
+        if (off > ctx->arena_bytes || bytes > ctx->arena_bytes - off) {
+            ctx->err = "the call needs more than the " + std::to_string(ctx->arena_bytes) +
+                       " bytes of the allocation-free scratch";
+            throw Fail{DPE_ERR_UNSUPPORTED};
+        }
+        ctx->arena_used = off + bytes;
+        return ctx->arena + off;
+    }
     if (ctx->bufs.size() < (size_t)B_COUNT) ctx->bufs.resize(B_COUNT);
     auto &b = ctx->bufs[id];
-    if (bytes == 0) bytes = 16;
     if (b.bytes < bytes) {
         if (b.ptr) {
             HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -781,8 +811,21 @@ dpe_status ncc_adjoint_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_input
     return DPE_OK;
 }
 
+// Upper bound of the scratch bytes one dpe_ncc_cost call with device-resident
+// inputs carves (every buffer rounded up to 256 B, quads counted at fp32 size):
+// images, quads, cameras, view constants, spatial weights, flag, counters.
+size_t ncc_cost_scratch_bytes(const dpe_params *p, const dpe_inputs *in) {
+    auto up = [](size_t b) { return (std::max(b, (size_t)16) + 255) & ~(size_t)255; };
+    const size_t P = (size_t)in->width * in->height, N = (size_t)in->num_images;
+    const size_t qs = (size_t)(in->width + 1) * (in->height + 1);
+    const size_t ntap = (size_t)(2 * p->strong_radius / p->strong_increment + 1);
+    return up(sizeof(float) * P * N) + up(sizeof(float4) * qs * N) + up(sizeof(dpe_camera) * N) +
+           up(sizeof(ViewConst) * N) + up(sizeof(float) * ntap * ntap) + up(sizeof(int)) +
+           up(sizeof(unsigned long long) * kNumCounters);
+}
+
 dpe_status ncc_cost_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *form, const dpe_inputs *in, int num_hyps,
-                         const float *planes, float *costs) {
+                         const float *planes, float *costs, bool no_alloc = false) {
     dpe_params p = *prm;
     p.use_APD = p.use_edge = p.use_limit = p.use_label = p.geom_consistency = 0;
     p.state = DPE_FIRST_INIT;
@@ -794,8 +837,40 @@ dpe_status ncc_cost_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *fo
         return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "unknown precision");
     if (precision != DPE_PRECISION_FP32 && 2 * p.strong_radius / p.strong_increment + 1 != 6)
         return fail(ctx, DPE_ERR_UNSUPPORTED, "the precision form needs the 11x11 step-2 window");
+    // allocation-free form: device-resident inputs and outputs only, every
+    // buffer of the call carved from this device's scratch array, no timing
+    // events (their pool could grow)
+    struct ArenaGuard {
+        dpe_ctx *c;
+        ~ArenaGuard() {
+            c->arena = nullptr;
+            c->arena_bytes = c->arena_used = 0;
+        }
+    };
+    std::unique_ptr<ArenaGuard> arena_guard;
+    if (no_alloc) {
+        if (in->memory != DPE_MEM_DEVICE)
+            return fail(ctx, DPE_ERR_INVALID_ARGUMENT, "the allocation-free call needs device-resident inputs (DPE_MEM_DEVICE)");
+        char *base = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(g_scratch_mu);
+            base = ctx->device < kMaxScratchDevices ? g_scratch_ptr[ctx->device] : nullptr;
+        }
+        if (!base) return fail(ctx, DPE_ERR_HIP, "the device's scratch address was not resolved at dpe_create");
+        const size_t need = ncc_cost_scratch_bytes(&p, in);
+        if (need > kNoAllocScratchBytes)
+            return fail(ctx, DPE_ERR_UNSUPPORTED,
+                        "the allocation-free call holds " + std::to_string(kNoAllocScratchBytes) +
+                            " bytes of scratch; this problem needs " + std::to_string(need) +
+                            " (4 B x pixels x images + 16 B x (W+1)(H+1) x images + 64 KiB)");
+        ctx->arena = base;
+        ctx->arena_bytes = kNoAllocScratchBytes;
+        ctx->arena_used = 0;
+        arena_guard.reset(new ArenaGuard{ctx});
+    }
     HIPCHK(hipSetDevice(ctx->device));
     EventTimer tm{ctx};
+    tm.record = !no_alloc;
     Dev d = setup_views(ctx, &p, in, precision);
     const size_t P = d.P;
     const bool dev = in->memory == DPE_MEM_DEVICE;
@@ -1080,6 +1155,22 @@ dpe_status dpe_create(int device, dpe_ctx **out) {
         delete ctx;
         return DPE_ERR_HIP;
     }
+    // this device's address of the allocation-free scratch array, once per device
+    // (with the device current: each GPU holds its own copy at its own address)
+    if (device < kMaxScratchDevices) {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[device]) {
+            void *p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                g_create_error = std::string("cannot resolve the scratch array: ") + hipGetErrorString(e);
+                (void)hipStreamDestroy(ctx->stream);
+                delete ctx;
+                return DPE_ERR_HIP;
+            }
+            g_scratch_ptr[device] = (char *)p;
+        }
+    }
     *out = ctx;
     return DPE_OK;
 }
@@ -1245,6 +1336,16 @@ dpe_status dpe_ncc_cost_ex(dpe_ctx *ctx, const dpe_params *params, const dpe_for
     if (!ctx) return DPE_ERR_INVALID_ARGUMENT;
     try {
         return ncc_cost_impl(ctx, params, form, in, num_hyps, planes, costs);
+    } catch (const Fail &f) {
+        return f.st;
+    }
+}
+
+dpe_status dpe_ncc_cost_noalloc(dpe_ctx *ctx, const dpe_params *params, const dpe_inputs *in, int num_hyps,
+                                const float *planes, float *costs) {
+    if (!ctx) return DPE_ERR_INVALID_ARGUMENT;
+    try {
+        return ncc_cost_impl(ctx, params, nullptr, in, num_hyps, planes, costs, true);
     } catch (const Fail &f) {
         return f.st;
     }

@@ -52,7 +52,7 @@ class Outputs(C.Structure):
 
 NCC_FAMILIES = 4
 FAMILY_FULL, FAMILY_REFINE, FAMILY_POST, FAMILY_WEAK = range(NCC_FAMILIES)
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 
 class Stats(C.Structure):
@@ -174,6 +174,8 @@ def declare(lib):
     lib.dpe_ncc_cost.restype = C.c_int
     lib.dpe_ncc_cost_ex.argtypes = [C.c_void_p, P(Params), C.c_void_p, P(Inputs), C.c_int, C.c_void_p, C.c_void_p]
     lib.dpe_ncc_cost_ex.restype = C.c_int
+    lib.dpe_ncc_cost_noalloc.argtypes = [C.c_void_p, P(Params), P(Inputs), C.c_int, C.c_void_p, C.c_void_p]
+    lib.dpe_ncc_cost_noalloc.restype = C.c_int
     lib.dpe_ncc_cost_adjoint.argtypes = [C.c_void_p, P(Params), P(Inputs), C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p]
     lib.dpe_ncc_cost_adjoint.restype = C.c_int
@@ -197,6 +199,6 @@ def declare(lib):
 
 # Every symbol include/dpe_hip.h declares.
 EXPORTS = ["dpe_create", "dpe_destroy", "dpe_last_error", "dpe_abi_version", "dpe_process", "dpe_process_ex",
-           "dpe_last_stats", "dpe_ncc_cost", "dpe_ncc_cost_ex", "dpe_ncc_cost_adjoint", "dpe_fuse",
+           "dpe_last_stats", "dpe_ncc_cost", "dpe_ncc_cost_ex", "dpe_ncc_cost_noalloc", "dpe_ncc_cost_adjoint", "dpe_fuse",
            "dpe_fuse_tat", "dpe_batch_create", "dpe_batch_destroy", "dpe_batch_last_error", "dpe_batch_process"]
 MAX_LANES = 16

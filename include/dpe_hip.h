@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DPE_ABI_VERSION 10
+#define DPE_ABI_VERSION 11
 /* main.h:41-44 */
 #define DPE_MAX_IMAGES 32
 #define DPE_NEIGHBOUR_NUM 9
@@ -213,6 +213,21 @@ dpe_status dpe_process_ex(dpe_ctx *ctx, const dpe_params *params, const dpe_form
  * cost against source view v+1.  num_hyps >= 1. */
 dpe_status dpe_ncc_cost(dpe_ctx *ctx, const dpe_params *params, const dpe_inputs *in,
                         int num_hyps, const float *planes, float *costs);
+
+/* dpe_ncc_cost that allocates no device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Inputs,
+ * planes and costs must be device-resident (in->memory == DPE_MEM_DEVICE).
+ * Every buffer of the call (a copy of the images, the bilinear quads, camera
+ * constants, counters) comes from a 64 MiB scratch array that each GPU holds
+ * from its first dpe_create until the process ends; a problem needs at most
+ * 4 B x W*H x num_images + 16 B x (W+1)(H+1) x num_images + 64 KiB of it, and a
+ * larger one returns DPE_ERR_UNSUPPORTED before anything runs.  Calls that
+ * overlap on one GPU (several contexts or threads) share the array and race.
+ * Results equal dpe_ncc_cost's bit for bit; dpe_last_stats reports the counts
+ * and no launch times. */
+dpe_status dpe_ncc_cost_noalloc(dpe_ctx *ctx, const dpe_params *params, const dpe_inputs *in,
+                                int num_hyps, const float *planes, float *costs);
 
 /* dpe_ncc_cost with a form (its precision field; NULL = plain).  No reference counterpart. */
 dpe_status dpe_ncc_cost_ex(dpe_ctx *ctx, const dpe_params *params, const dpe_form *form, const dpe_inputs *in,
