@@ -280,36 +280,134 @@ def reference_schedule(round_num):
     return calls
 
 
-def run_schedule_scale1(ctx, scene, calls, views=None):
-    """Python mirror of the C++ driver for schedules at scale 1 over a synthetic
-    scene: every call group over every view in order, state hand-off through a
-    StateStore, call_index = call group index.  Returns the StateStore."""
-    from . import support as sup
+def scaled_size(width, height, scale):
+    """Image size at a pyramid scale, as SceneCache::GetGrayFloatImage rounds it
+    (round(W / s) x round(H / s), DPE.cpp:720-737; the driver's lround of the
+    float product)."""
+    inv = np.float32(1.0) / np.float32(scale)
+    return int(np.floor(np.float32(width) * inv + 0.5)), int(np.floor(np.float32(height) * inv + 0.5))
+
+
+class ScaledScene:
+    """Float images, cameras and support maps of a synthetic scene at every
+    pyramid scale, built as the C++ driver builds them: INTER_LINEAR reduction
+    of the 8-bit image (dpe_host_resize_linear; factor 2 = 2x2 box mean),
+    cameras with K scaled to the level (SceneCache::GetCamera, DPE.cpp:739-760),
+    and support maps of the level's image rounded back to 8 bits
+    (GetProblemEdges, main.cpp:166-221)."""
+
+    def __init__(self, scene, hostlib=None):
+        self.scene = scene
+        self.hostlib = hostlib
+        self._img = {}
+        self._maps = {}
+
+    def size(self, scale):
+        return scaled_size(self.scene.width, self.scene.height, scale)
+
+    def image(self, i, scale):
+        key = (i, scale)
+        if key not in self._img:
+            full = self.scene.views[i].image.astype(np.float32)
+            if scale == 1:
+                self._img[key] = full
+            else:
+                w, h = self.size(scale)
+                out = np.zeros((h, w), np.float32)
+                self.hostlib.dpe_host_resize_linear(full.ctypes.data, self.scene.width, self.scene.height,
+                                                    out.ctypes.data, w, h)
+                self._img[key] = out
+        return self._img[key]
+
+    def cameras(self, ids, scale):
+        w, h = self.size(scale)
+        return scene_cameras(self.scene, ids, w, h)
+
+    def support(self, i, scale):
+        key = (i, scale)
+        if key not in self._maps:
+            from . import support as sup
+            u8 = np.clip(np.rint(self.image(i, scale)), 0, 255).astype(np.uint8)
+            self._maps[key] = sup.support_maps(u8)
+        return self._maps[key]
+
+
+def rescale_nearest(a, width, height):
+    """RescaleMatToTargetSize (DPE.cpp:1313-1335): nearest, o = int(r / s) with
+    s = target / source in float; identity at the same size."""
+    h0, w0 = a.shape[:2]
+    if (w0, h0) == (width, height):
+        return a
+    sx = np.float32(width) / np.float32(w0)
+    sy = np.float32(height) / np.float32(h0)
+    rows = (np.arange(height, dtype=np.float32) / sy).astype(np.int64)
+    cols = (np.arange(width, dtype=np.float32) / sx).astype(np.int64)
+    out = np.zeros((height, width) + a.shape[2:], a.dtype)
+    ok_r = rows < h0
+    ok_c = cols < w0
+    out[np.ix_(ok_r, ok_c)] = a[np.ix_(rows[ok_r], cols[ok_c])]
+    return out
+
+
+def schedule_call_inputs(ss, c, call_index, ref, srcs, store, max_scale_size):
+    """ProblemInputs of one call of a schedule at the call's scale: what the C++
+    driver's ProcessProblem hands to dpe_process (main.cpp:241-319 with
+    InuputInitialization and SupportInitialization, DPE.cpp:934-1069,
+    1187-1214).  Stored states of another size are brought to the call's size
+    with RescaleMatToTargetSize; the low-resolution edge map is the one of the
+    coarsest scale (high_res_img)."""
+    sc = ss.scene
+    scale = c["scale"]
+    W, H = ss.size(scale)
+    ids = [ref] + list(srcs)
+    imgs = [ss.image(i, scale) for i in ids]
+    cams = ss.cameras(ids, scale)
+    prm = call_params(sc.depth_min, sc.depth_max, max_iterations=c["iters"], state=c["state"], use_APD=c["apd"],
+                      use_edge=c["edge"], geom_consistency=c["geom"], weak_peak_radius=c["peak"],
+                      ransac_threshold=np.float32(c["ransac"]), rotate_time=c["rotate"], call_index=call_index,
+                      max_scale_size=max_scale_size, scale_size=scale)
+    kw = {}
+    if c["geom"]:
+        zero = np.zeros((H, W), np.float32)
+        kw["geom_depths"] = [rescale_nearest(store.get(i)["depth"], W, H) if store.get(i) is not None else zero
+                             for i in ids]
+    st = store.get(ref)
+    if c["apd"]:
+        kw["weak_info"] = rescale_nearest(st["pixel_states"], W, H)
+    if c["state"] != abi.FIRST_INIT:
+        depth = rescale_nearest(st["depth"], W, H)
+        normal = rescale_nearest(st["normal"], W, H)
+        kw["prior_planes"] = np.concatenate([normal, depth[..., None]], -1).astype(np.float32)
+        kw["prior_selected"] = rescale_nearest(st["selected_views"], W, H)
+    edges, labels = ss.support(ref, scale)
+    edges_low, _ = ss.support(ref, max_scale_size)
+    return ProblemInputs(imgs, cams, prm, edges=edges, edges_low=edges_low, labels=labels, **kw)
+
+
+def run_schedule(runner, scene, calls, views=None, hostlib=None, on_call=None):
+    """Python mirror of the C++ driver for schedules over a synthetic scene (any
+    scales): every call group over every view in order, state hand-off through
+    a StateStore, call_index = call group index, max_scale_size = the largest
+    scale of the schedule.  `runner(pin)` returns the ProblemOutputs of one call
+    (the GPU: `lambda pin: run_hip(ctx, pin)`); `on_call(group, ref, pin, out)`
+    sees every call.  Scales above 1 need libdpe_host (`hostlib`) for the
+    pyramid.  Returns the StateStore."""
     views = list(range(len(scene.views))) if views is None else list(views)
+    ss = ScaledScene(scene, hostlib)
+    max_scale_size = max(c["scale"] for c in calls)
     store = StateStore()
-    maps = {i: sup.support_maps(scene.views[i].image) for i in views}
     for it, c in enumerate(calls):
-        assert c["scale"] == 1
         for ref in views:
             srcs = [j for j in scene.pairs[ref] if j in views]
-            ids = [ref] + srcs
-            imgs = [scene.views[i].image.astype(np.float32) for i in ids]
-            cams = scene_cameras(scene, ids)
-            prm = call_params(scene.depth_min, scene.depth_max, max_iterations=c["iters"], state=c["state"],
-                              use_APD=c["apd"], use_edge=c["edge"], geom_consistency=c["geom"],
-                              weak_peak_radius=c["peak"], ransac_threshold=np.float32(c["ransac"]),
-                              rotate_time=c["rotate"], call_index=it)
-            kw = {}
-            if c["geom"]:
-                zero = np.zeros((scene.height, scene.width), np.float32)
-                kw["geom_depths"] = [store.get(i)["depth"] if store.get(i) is not None else zero for i in ids]
-            st = store.get(ref)
-            if c["apd"]:
-                kw["weak_info"] = st["pixel_states"]
-            if c["state"] != abi.FIRST_INIT:
-                kw["prior_planes"] = np.concatenate([st["normal"], st["depth"][..., None]], -1).astype(np.float32)
-                kw["prior_selected"] = st["selected_views"]
-            edges, labels = maps[ref]
-            pin = ProblemInputs(imgs, cams, prm, edges=edges, edges_low=edges, labels=labels, **kw)
-            store.put(ref, assemble(run_hip(ctx, pin), prm.depth_min, prm.depth_max))
+            pin = schedule_call_inputs(ss, c, it, ref, srcs, store, max_scale_size)
+            out = runner(pin)
+            if on_call is not None:
+                on_call(it, ref, pin, out)
+            store.put(ref, assemble(out, pin.params.depth_min, pin.params.depth_max))
     return store
+
+
+def run_schedule_scale1(ctx, scene, calls, views=None):
+    """run_schedule on the GPU for schedules whose calls are all at scale 1."""
+    assert all(c["scale"] == 1 for c in calls)
+    return run_schedule(lambda pin: run_hip(ctx, pin), scene, calls, views)
