@@ -20,10 +20,11 @@
 //   * GenNeighbours' two unsequenced curand() calls (DPE.cu:2186-2187) draw the
 //     sign first, then the magnitude.
 //
-// Parity unpinned: GenNeighbours' direction search (DPE.cu:2107-2467).  Only its
-// agreement with the HIP path and the weak-pixel recovery on a known plane (which
-// a mirrored or permuted search would still pass) check it; PointinTriangle and
-// the RANSAC plane fit it feeds have closed-form checks of their own.
+// Every function is pinned by an independent check (tests/test_oracle*.py):
+// closed forms, float64 restatements of definitions, brute force, library
+// routines, invariants, known-answer vectors.  Unpinnable here: the reference's
+// exact outputs (clock64-seeded curand), OpenCV's preprocessing, curand's
+// XORWOW streams and NVIDIA's texture rounding, none of which this file uses.
 #include "dpe_oracle.h"
 #include "oracle_numerics.h"
 
@@ -620,25 +621,34 @@ float InitialCostAndSelectedViews(Ctx &C, int px, int py) {
     return cost_max;
 }
 
-// ComputeMultiViewInitialCost, DPE.cu:832-861
-float InitialCost(Ctx &C, int px, int py) {
-    const int center = px + py * C.W;
-    const F4 plane = C.planes[center];
+// The view loop of ComputeMultiViewInitialCost, DPE.cu:832-861: `view_cost(i)`
+// is the NCC cost of source view i (evaluated only for prior-selected views);
+// an invalid view clears bits 0..i-1 of the mask (unSetBit, Q1).
+template <class CostFn>
+float InitialCostViews(int N, uint32_t *selected, CostFn view_cost) {
     const float cost_max = 2.0f;
     int cost_count = 0;
     float cost = 0.0f;
-    for (int i = 1; i < C.N; ++i) {
-        if (isSet(C.selected[center], i - 1)) {
-            const float c = ComputeBilateralNCCOld(C, px, py, i, plane);
+    for (int i = 1; i < N; ++i) {
+        if (isSet(*selected, i - 1)) {
+            const float c = view_cost(i);
             if (c < cost_max) {
                 cost_count++;
                 cost += c;
             } else {
-                unSetBit(&C.selected[center], i - 1);
+                unSetBit(selected, i - 1);
             }
         }
     }
     return cost_count == 0 ? cost_max : cost / cost_count;
+}
+
+// ComputeMultiViewInitialCost, DPE.cu:832-861
+float InitialCost(Ctx &C, int px, int py) {
+    const int center = px + py * C.W;
+    const F4 plane = C.planes[center];
+    return InitialCostViews(C.N, &C.selected[center],
+                            [&](int i) { return ComputeBilateralNCCOld(C, px, py, i, plane); });
 }
 
 // RandomInitialization, DPE.cu:1039-1067
@@ -1284,6 +1294,11 @@ bool PointinTriangle(S2 A, S2 B, S2 Cc, int px, int py) {
     return t1 * t2 >= 0 && t1 * t3 >= 0;
 }
 
+// test access only: when set, BresenhamLine appends the low-resolution (x, y) of
+// every pixel its walks test, and the step count of each walk
+thread_local std::vector<long> *tl_bres_trace = nullptr;
+thread_local int tl_bres_steps[2];
+
 inline uint8_t edge_low_at(const Ctx &C, long idx) {
     if (idx < 0 || idx >= (long)C.low_w * C.low_h) return 0;
     return C.edge_low[idx];
@@ -1297,6 +1312,7 @@ bool BresenhamLine(const Ctx &C, int Ax, int Ay, int Bx, int By) {
     const int height = C.low_h, width = C.low_w;
     const int max_step = C.prm.high_res_img ? (int)std::round(std::max(height, width) / 60.0) : std::max(height, width);
     for (int pass = 0; pass < 2; ++pass) {
+        if (tl_bres_trace) tl_bres_steps[pass] = 0;
         const int fromx = pass == 0 ? Bx : Ax, fromy = pass == 0 ? By : Ay;
         const int tox = pass == 0 ? Ax : Bx, toy = pass == 0 ? Ay : By;
         int x0 = std::min((int)std::round(fromx * scale_x), width - 1);
@@ -1319,6 +1335,11 @@ bool BresenhamLine(const Ctx &C, int Ax, int Ay, int Bx, int By) {
             if (e2 < dy) {
                 erro += dx;
                 y0 += sy;
+            }
+            if (tl_bres_trace) {
+                tl_bres_trace->push_back(x0);
+                tl_bres_trace->push_back(y0);
+                tl_bres_steps[pass]++;
             }
             if (edge_low_at(C, (long)x0 + (long)y0 * width)) return true;
             step += 1;
@@ -1424,41 +1445,23 @@ void FindNearestStrongPoint(Ctx &C, int center) {
 }
 
 // GenNeighbours, DPE.cu:2107-2467
-void GenNeighbours(Ctx &C, int center) {
+// The ray search of GenNeighbours, DPE.cu:2170-2222: 8 base directions x
+// rotate_time rotations; per direction, radii 2, 4, 8, ... (min(2r, r + 25)) up
+// to MAX_SEARCH_RADIUS while the ray point is inside the image, 4 jittered
+// probes per radius; a probe's pixel, or its nearest strong pixel when it is
+// not STRONG, is taken if it lies within half the rotation angle of the
+// direction and (edge_limit) no edge separates it from the centre.  Fills
+// strong_points / dir_valid at index 4 * base direction + rotation; returns how
+// many directions found a point.
+int DirectionSearch(const Ctx &C, int px, int py, Rng &rng, bool edge_limit, S2 *strong_points, bool *dir_valid) {
     const int W = C.W, H = C.H;
-    const int px = center % W, py = center / W;
-    if (C.weak[center] != DPE_WEAK) return;
-    const int max_pt_num = 64;
     const int min_margin = 6;
-    const float depth_diff = C.prm.depth_max - C.prm.depth_min;
-    const dpe_camera &cam = C.cams[0];
-    S2 *neighbours = &C.neighbours[(size_t)C.nmap[center] * NNUM];
-    Rng rng(C.prm.seed, center, PH_GEN_NEIGHBOURS, C.prm.call_index);
-    for (int i = 0; i < NNUM; ++i) neighbours[i] = S2{-1, -1};
-    neighbours[0] = S2{(int16_t)px, (int16_t)py};
-    S2 strong_points[max_pt_num];
-    bool dir_valid[max_pt_num];
-    for (int i = 0; i < max_pt_num; ++i) {
-        strong_points[i] = S2{-1, -1};
-        dir_valid[i] = false;
-    }
     int odi = -1;
     int strong_point_size = 0;
     const int rotate_time = C.prm.rotate_time;
     const float cos_angle = C.gn_cos_angle, sin_angle = C.gn_sin_angle;
     const float threshhold = C.gn_threshold;
     const int shift_range = C.gn_shift_range;
-    const float ransac_threshold = C.prm.ransac_threshold * depth_diff;
-    bool edge_limit = false;
-    if (C.prm.use_limit) {
-        edge_limit = true;
-        if (C.prm.use_edge) {
-            const float cv = C.complex_map[center];
-            const float rp = rng.uniform() - FLT_EPSILON;
-            if (rp < cv) edge_limit = false;
-            else C.complex_map[center] = std::max(0.99f, cv);
-        }
-    }
     for (int odx = -1; odx <= 1; ++odx) {
         for (int ody = -1; ody <= 1; ++ody) {
             if (odx == 0 && ody == 0) continue;
@@ -1510,6 +1513,40 @@ void GenNeighbours(Ctx &C, int center) {
             }
         }
     }
+    return strong_point_size;
+}
+
+void GenNeighbours(Ctx &C, int center) {
+    const int W = C.W, H = C.H;
+    const int px = center % W, py = center / W;
+    if (C.weak[center] != DPE_WEAK) return;
+    const int max_pt_num = 64;
+    const int min_margin = 6;
+    const float depth_diff = C.prm.depth_max - C.prm.depth_min;
+    const dpe_camera &cam = C.cams[0];
+    S2 *neighbours = &C.neighbours[(size_t)C.nmap[center] * NNUM];
+    Rng rng(C.prm.seed, center, PH_GEN_NEIGHBOURS, C.prm.call_index);
+    for (int i = 0; i < NNUM; ++i) neighbours[i] = S2{-1, -1};
+    neighbours[0] = S2{(int16_t)px, (int16_t)py};
+    S2 strong_points[max_pt_num];
+    bool dir_valid[max_pt_num];
+    for (int i = 0; i < max_pt_num; ++i) {
+        strong_points[i] = S2{-1, -1};
+        dir_valid[i] = false;
+    }
+    const int rotate_time = C.prm.rotate_time;
+    const float ransac_threshold = C.prm.ransac_threshold * depth_diff;
+    bool edge_limit = false;
+    if (C.prm.use_limit) {
+        edge_limit = true;
+        if (C.prm.use_edge) {
+            const float cv = C.complex_map[center];
+            const float rp = rng.uniform() - FLT_EPSILON;
+            if (rp < cv) edge_limit = false;
+            else C.complex_map[center] = std::max(0.99f, cv);
+        }
+    }
+    int strong_point_size = DirectionSearch(C, px, py, rng, edge_limit, strong_points, dir_valid);
     int extend_index = 31;
     if (C.prm.use_label && C.label[center] > 0) {
         static const int dir[8][2] = {{0, -1}, {0, 1}, {-1, 0}, {1, 0}, {-1, -1}, {1, 1}, {-1, 1}, {1, -1}};
@@ -2067,6 +2104,15 @@ void colour_pixels(const Ctx &C, int colour, std::vector<int> &out) {
             if (((x + y) & 1) == colour) out.push_back(x + y * C.W);
 }
 
+// DPE.cu:2151-2157, evaluated on the host in double as the device code does
+void set_gen_neighbours_constants(Ctx &C) {
+    const float angle = 45.0f / C.prm.rotate_time;
+    C.gn_cos_angle = (float)std::cos(angle * M_PI / 180.f);
+    C.gn_sin_angle = (float)std::sin(angle * M_PI / 180.f);
+    C.gn_threshold = (float)std::cos((angle / 2.0f) * M_PI / 180.0f);
+    C.gn_shift_range = std::max((int)(std::tan((angle / 2.0f) * M_PI / 180.0f) * 20), 1);
+}
+
 int setup(Ctx &C, const dpe_params *params, const dpe_inputs *in, int nthreads) {
     if (!params || !in) return DPE_ERR_INVALID_ARGUMENT;
     if (in->num_images > MAXIMG) return DPE_ERR_TOO_MANY_IMAGES;
@@ -2131,12 +2177,7 @@ int setup(Ctx &C, const dpe_params *params, const dpe_inputs *in, int nthreads) 
         C.label = in->labels;
         C.label_bound.assign((size_t)C.weak_count * 8 + 1, S2{-1, -1});
     }
-    // DPE.cu:2151-2157, evaluated on the host in double as the device code does
-    const float angle = 45.0f / C.prm.rotate_time;
-    C.gn_cos_angle = (float)std::cos(angle * M_PI / 180.f);
-    C.gn_sin_angle = (float)std::sin(angle * M_PI / 180.f);
-    C.gn_threshold = (float)std::cos((angle / 2.0f) * M_PI / 180.0f);
-    C.gn_shift_range = std::max((int)(std::tan((angle / 2.0f) * M_PI / 180.0f) * 20), 1);
+    set_gen_neighbours_constants(C);
     return DPE_OK;
 }
 
@@ -2591,6 +2632,134 @@ void oracle_gen_edge_inform(int W, int H, const uint8_t *edge, const int32_t *la
         label_bound[2 * k + 1] = C.label_bound[k].y;
     }
     for (int i = 0; i < C.P; ++i) complex_map[i] = C.complex_map[i];
+}
+
+// ---- test access to single functions of the restatement (each checked by
+// tests/test_oracle_pins.py against closed forms, brute force or invariants)
+
+float oracle_geom_cost(const dpe_camera *ref, const dpe_camera *src, int W, int H, const float *src_depth, int px,
+                       int py, const float *plane) {
+    Ctx C;
+    C.W = W;
+    C.H = H;
+    C.P = W * H;
+    C.N = 2;
+    C.cams[0] = *ref;
+    C.cams[1] = *src;
+    C.gdepth[0] = nullptr;
+    C.gdepth[1] = src_depth;
+    return ComputeGeomConsistencyCost(C, px, py, 1, F4{plane[0], plane[1], plane[2], plane[3]});
+}
+
+float oracle_initial_cost(const float *view_costs, int num_images, uint32_t *selected) {
+    return InitialCostViews(num_images, selected, [&](int i) { return view_costs[i - 1]; });
+}
+
+void oracle_random_normal(const dpe_camera *cam, int px, int py, float depth, uint64_t seed, uint32_t pixel,
+                          uint32_t phase, uint32_t call, float *out) {
+    Rng rng(seed, pixel, phase, call);
+    const F4 n = GenerateRandomNormal(*cam, px, py, rng, depth);
+    out[0] = n.x;
+    out[1] = n.y;
+    out[2] = n.z;
+    out[3] = n.w;
+}
+
+void oracle_perturbed_normal(const dpe_camera *cam, int px, int py, const float *normal, float perturbation,
+                             uint64_t seed, uint32_t pixel, uint32_t phase, uint32_t call, float *out) {
+    Rng rng(seed, pixel, phase, call);
+    const F4 n = GeneratePerturbedNormal(*cam, px, py, F4{normal[0], normal[1], normal[2], normal[3]}, rng,
+                                         perturbation);
+    out[0] = n.x;
+    out[1] = n.y;
+    out[2] = n.z;
+    out[3] = n.w;
+}
+
+void oracle_transform_normal(const dpe_camera *cam, const float *n, int to_world, float *out) {
+    const F4 p{n[0], n[1], n[2], n[3]};
+    const F4 r = to_world ? TransformNormal(*cam, p) : TransformNormal2RefCam(*cam, p);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    out[3] = r.w;
+}
+
+void oracle_depth_and_normal(const dpe_camera *cam, int px, int py, const float *plane, float *out) {
+    Ctx C;
+    C.W = px + 1;
+    C.H = py + 1;
+    C.P = C.W * C.H;
+    C.cams[0] = *cam;
+    C.planes.assign((size_t)C.P, F4{0, 0, 0, 0});
+    const int center = px + py * C.W;
+    C.planes[center] = F4{plane[0], plane[1], plane[2], plane[3]};
+    GetDepthandNormal(C, center);
+    const F4 r = C.planes[center];
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    out[3] = r.w;
+}
+
+int oracle_bresenham(int W, int H, const uint8_t *edge, int low_w, int low_h, const uint8_t *edge_low, int high_res,
+                     int ax, int ay, int bx, int by, int64_t *trace, int cap, int *steps) {
+    Ctx C;
+    C.W = W;
+    C.H = H;
+    C.P = W * H;
+    std::memset(&C.prm, 0, sizeof(C.prm));
+    C.prm.high_res_img = high_res;
+    C.edge = edge;
+    C.edge_low = edge_low;
+    C.low_w = low_w;
+    C.low_h = low_h;
+    std::vector<long> tr;
+    tl_bres_trace = &tr;
+    tl_bres_steps[0] = tl_bres_steps[1] = 0;
+    const bool r = BresenhamLine(C, ax, ay, bx, by);
+    tl_bres_trace = nullptr;
+    for (size_t i = 0; i < tr.size() && (int)i < cap; ++i) trace[i] = tr[i];
+    steps[0] = tl_bres_steps[0];
+    steps[1] = tl_bres_steps[1];
+    return r ? 1 : 0;
+}
+
+int oracle_direction_search(int W, int H, const uint8_t *weak, const int16_t *nearest, const uint8_t *edge,
+                            const uint8_t *edge_low, int low_w, int low_h, int high_res, int rotate_time,
+                            int edge_limit, int px, int py, const uint32_t *draws, int n_draws, int16_t *points,
+                            uint8_t *valid) {
+    Ctx C;
+    C.W = W;
+    C.H = H;
+    C.P = W * H;
+    std::memset(&C.prm, 0, sizeof(C.prm));
+    C.prm.high_res_img = high_res;
+    C.prm.rotate_time = rotate_time;
+    set_gen_neighbours_constants(C);
+    C.weak.assign(weak, weak + C.P);
+    C.nearest.resize((size_t)C.P);
+    for (int i = 0; i < C.P; ++i) C.nearest[i] = S2{nearest[2 * i], nearest[2 * i + 1]};
+    C.edge = edge;
+    C.edge_low = edge_low;
+    C.low_w = low_w;
+    C.low_h = low_h;
+    Rng rng(0, 0, 0, 0);
+    rng.replay = draws;
+    rng.replay_n = (uint32_t)n_draws;
+    S2 pts[64];
+    bool ok[64];
+    for (int i = 0; i < 64; ++i) {
+        pts[i] = S2{-1, -1};
+        ok[i] = false;
+    }
+    const int n = DirectionSearch(C, px, py, rng, edge_limit != 0, pts, ok);
+    for (int i = 0; i < 64; ++i) {
+        points[2 * i] = pts[i].x;
+        points[2 * i + 1] = pts[i].y;
+        valid[i] = ok[i] ? 1 : 0;
+    }
+    return n;
 }
 
 uint8_t oracle_depth_to_weak_state(const float *costs61, int weak_peak_radius) {

@@ -49,6 +49,35 @@ void oracle_gen_edge_inform(int W, int H, const uint8_t *edge, const int32_t *la
                             int strong_radius, int16_t *edge_neigh, float *complex_map, int16_t *label_bound);
 /* DepthToWeak's peak analysis (DPE.cu:2699-2750) over its 61 sampled costs; returns the pixel state */
 uint8_t oracle_depth_to_weak_state(const float *costs61, int weak_peak_radius);
+/* ComputeGeomConsistencyCost (DPE.cu:919-957) of reference pixel (px, py) for a camera-frame plane against one
+ * source camera and its W x H depth map */
+float oracle_geom_cost(const dpe_camera *ref, const dpe_camera *src, int W, int H, const float *src_depth, int px,
+                       int py, const float *plane);
+/* the view loop of ComputeMultiViewInitialCost (DPE.cu:832-861) over given costs of source views 1..num_images-1;
+ * *selected is the prior mask in, the updated mask out; returns the cost */
+float oracle_initial_cost(const float *view_costs, int num_images, uint32_t *selected);
+/* GenerateRandomNormal (DPE.cu:365-391) and GeneratePerturbedNormal (DPE.cu:393-428) on the random stream
+ * (seed, pixel, phase, call); out: float4 */
+void oracle_random_normal(const dpe_camera *cam, int px, int py, float depth, uint64_t seed, uint32_t pixel,
+                          uint32_t phase, uint32_t call, float *out);
+void oracle_perturbed_normal(const dpe_camera *cam, int px, int py, const float *normal, float perturbation,
+                             uint64_t seed, uint32_t pixel, uint32_t phase, uint32_t call, float *out);
+/* TransformNormal (to_world = 1) / TransformNormal2RefCam (to_world = 0), DPE.cu:528-546 */
+void oracle_transform_normal(const dpe_camera *cam, const float *n, int to_world, float *out);
+/* GetDepthandNormal (DPE.cu:1944-1959) of one pixel: camera-frame (n, d) in, (world n, depth) out */
+void oracle_depth_and_normal(const dpe_camera *cam, int px, int py, const float *plane, float *out);
+/* BresenhamLine (DPE.cu:162-248) between full-resolution pixels A and B over a W x H edge map and a
+ * low_w x low_h one; trace receives (up to cap entries) the low-resolution (x, y) of every pixel the two walks
+ * test, steps[0] / steps[1] how many each walk tested; returns the result (1 = an edge separates A and B) */
+int oracle_bresenham(int W, int H, const uint8_t *edge, int low_w, int low_h, const uint8_t *edge_low, int high_res,
+                     int ax, int ay, int bx, int by, int64_t *trace, int cap, int *steps);
+/* The ray search of GenNeighbours (DPE.cu:2170-2222) for pixel (px, py) over pixel states and nearest-strong
+ * coordinates (x, y per pixel), with the random draws taken cyclically from draws[0..n_draws); points / valid:
+ * 64 slots (index 4 * base direction + rotation); returns the number of directions that found a point */
+int oracle_direction_search(int W, int H, const uint8_t *weak, const int16_t *nearest, const uint8_t *edge,
+                            const uint8_t *edge_low, int low_w, int low_h, int high_res, int rotate_time,
+                            int edge_limit, int px, int py, const uint32_t *draws, int n_draws, int16_t *points,
+                            uint8_t *valid);
 #ifdef __cplusplus
 }
 #endif

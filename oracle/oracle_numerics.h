@@ -150,7 +150,12 @@ struct Rng {
         key[0] = (uint32_t)seed;
         key[1] = (uint32_t)(seed >> 32);
     }
+    // test access only: when set, draws come from this array (cyclically)
+    // instead of the Philox stream
+    const uint32_t *replay = nullptr;
+    uint32_t replay_n = 0;
     uint32_t next() {
+        if (replay) return replay[(n++) % replay_n];
         const uint32_t b = n >> 2;
         if (b != cur) {
             const uint32_t ctr[4] = {b, pixel, phase, call};

@@ -67,6 +67,28 @@ def load_oracle():
     lib.oracle_gen_edge_inform.restype = None
     lib.oracle_depth_to_weak_state.argtypes = [C.c_void_p, C.c_int]
     lib.oracle_depth_to_weak_state.restype = C.c_uint8
+    lib.oracle_geom_cost.argtypes = [P(abi.Camera), P(abi.Camera), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                     C.c_void_p]
+    lib.oracle_geom_cost.restype = C.c_float
+    lib.oracle_initial_cost.argtypes = [C.c_void_p, C.c_int, P(C.c_uint32)]
+    lib.oracle_initial_cost.restype = C.c_float
+    lib.oracle_random_normal.argtypes = [P(abi.Camera), C.c_int, C.c_int, C.c_float, C.c_uint64, C.c_uint32,
+                                         C.c_uint32, C.c_uint32, C.c_void_p]
+    lib.oracle_random_normal.restype = None
+    lib.oracle_perturbed_normal.argtypes = [P(abi.Camera), C.c_int, C.c_int, C.c_void_p, C.c_float, C.c_uint64,
+                                            C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
+    lib.oracle_perturbed_normal.restype = None
+    lib.oracle_transform_normal.argtypes = [P(abi.Camera), C.c_void_p, C.c_int, C.c_void_p]
+    lib.oracle_transform_normal.restype = None
+    lib.oracle_depth_and_normal.argtypes = [P(abi.Camera), C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    lib.oracle_depth_and_normal.restype = None
+    lib.oracle_bresenham.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                     C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+    lib.oracle_bresenham.restype = C.c_int
+    lib.oracle_direction_search.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                            C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    lib.oracle_direction_search.restype = C.c_int
     return lib
 
 
