@@ -25,8 +25,12 @@ sys.path.insert(0, os.path.join(ROOT, "dpe-mvs-fast_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-REF_BYTES_PER_TAP = 4  # one fp32 reference texel per tap; plus one bilinear source quad (dpe_stats.quad_bytes:
-                       # 16 B fp32, 8 B when the quads are stored in 16 bits)
+# VALU issue peak of MI355X without packed fp32: the published 157.3 TFLOP/s fp32
+# vector peak counts packed FMAs (2 fp32 FMAs per lane per instruction, 2 flops
+# each); one unpacked wave64 instruction is 64 lane operations, so the issue
+# peak is 157.3e12 / 2 / 2 / 64 = 614.5e9 wave instructions per second (256 CUs x
+# 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction).
+VALU_ISSUE_PEAK = 157.3e12 / 2 / 2 / 64
 FAMILY_NAMES = ["k_ncc_full<6>", "k_ncc_weighted<6,REFINE>", "k_ncc_weighted<6,POST>", "k_weak_update"]
 # HBM traffic of one k_ncc_full<6> launch on this workload, from two rocprofv3
 # --pmc passes (FETCH_SIZE, WRITE_SIZE) over `python bench.py --no-cpu-baseline`,
@@ -35,6 +39,10 @@ FAMILY_NAMES = ["k_ncc_full<6>", "k_ncc_weighted<6,REFINE>", "k_ncc_weighted<6,P
 # the runs and the arithmetic are in profiles/ncc_kernels_1600x1200_20views_hbm_traffic.md.
 # Counters cannot be read inside a timed run.
 NCC_FULL_TRAFFIC_BYTES = 9.1017e9
+# VALU wave instructions of one k_ncc_full<6> launch on this workload (SQ_INSTS_VALU
+# summed over the step's launches / launches, one rocprofv3 --pmc pass over
+# `python bench.py --no-cpu-baseline`; profiles/ncc_kernels_1600x1200_20views_valu_issue.md).
+NCC_FULL_VALU_INSTS = 2.2713e9
 
 
 def parse():
@@ -232,21 +240,30 @@ def main():
     taps = sum(t["taps"] for t in totals)
     fam = {k: [sum(t[k][f] for t in totals) for f in range(4)]
            for k in ("fam_ms", "fam_hyps", "fam_taps", "fam_launches")}
-    # accuracy of the final depth maps against ground truth
+    # accuracy of the final depth maps against ground truth, overall and by
+    # what the ground truth says each pixel can be resolved from
     acc = []
     for v in range(V):
         gt = torch.from_numpy(sc.views[v].depth).to(dev)
         rel = (depth[v] - gt).abs() / gt
         acc.append(float((rel < 0.01).float().mean()))
+    breakdown = accuracy_breakdown(sc, depth, dev)
     # roofline of the dominant kernel, k_ncc_full<6> (candidate cost vectors):
-    # algorithmic bytes per launch (reference texel + source quad per tap) / its
-    # average launch time, both from the hipEvent pairs and exact tap counters
-    # of the timed steps
-    bytes_per_tap = REF_BYTES_PER_TAP + totals[-1]["quad_bytes"]
+    # algorithmic bytes per launch / its average launch time, both from the
+    # hipEvent pairs and exact tap counters of the timed steps.  Algorithmic
+    # bytes per tap: one reference sample and one bilinear source quad at the
+    # gather precision, 20 B with fp32 quads (4 + 16), 10 B with 16-bit quads
+    # (2 + 8), which the plain path uses on 8-bit images (same samples).
+    bytes_per_tap = 20 if totals[-1]["quad_bytes"] == 16 else 10
     full_launches = max(fam["fam_launches"][0], 1)
     full_avg_ms = fam["fam_ms"][0] / full_launches
     full_bytes = fam["fam_taps"][0] * bytes_per_tap / full_launches
     achieved = full_bytes / (full_avg_ms * 1e-3) / 1e9
+    # what binds it: the counted DRAM traffic is a quarter of the HBM peak, the
+    # VALU issue rate is near its peak
+    dram_frac = NCC_FULL_TRAFFIC_BYTES / (full_avg_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)
+    valu_frac = (None if NCC_FULL_VALU_INSTS is None
+                 else NCC_FULL_VALU_INSTS / (full_avg_ms * 1e-3) / VALU_ISSUE_PEAK)
     families = {FAMILY_NAMES[f]: {"ms": fam["fam_ms"][f], "launches": fam["fam_launches"][f],
                                   "hyps": fam["fam_hyps"][f], "taps": fam["fam_taps"][f],
                                   "hyp_per_s": fam["fam_hyps"][f] / max(fam["fam_ms"][f] * 1e-3, 1e-12)}
@@ -270,20 +287,84 @@ def main():
                    "views": V, "width": W, "height": H, "sources": 10},
         "s_per_depth_map": elapsed / args.steps / V,
         "accuracy_px_lt_1pct_vs_gt": float(np.mean(acc)),
+        "accuracy_breakdown": breakdown,
         "fused_points": totals[-1]["points"],
         "ncc_hyps_per_step": hyps // args.steps,
         "ncc_taps_per_step": taps // args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": NCC_FULL_TRAFFIC_BYTES,
                      "kernel": "k_ncc_full<6>", "launch_avg_ms": full_avg_ms, "bytes_per_launch": full_bytes,
-                     "bytes_per_tap": bytes_per_tap, "launches": fam["fam_launches"][0]},
+                     "bytes_per_tap": bytes_per_tap, "launches": fam["fam_launches"][0],
+                     "dram_frac": dram_frac, "binding": "valu_issue", "valu_frac": valu_frac,
+                     "valu_insts_per_launch": NCC_FULL_VALU_INSTS,
+                     "valu_issue_peak_per_s": VALU_ISSUE_PEAK},
         "form": {"precision": args.precision, "gauss_newton_steps": args.gauss_newton},
         "ncc_families": families,
     }
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(sc, args.cpu_threads)
+        def state(i):
+            return (depth[i].cpu().numpy(), normal[i].cpu().numpy(), states[i].cpu().numpy(),
+                    selected[i].cpu().numpy().view(np.uint32))
+        result["cpu_baseline"] = cpu_baseline(sc, args.cpu_threads, state)
     print(json.dumps(result), flush=True)
     ctx.close()
+
+
+def accuracy_breakdown(sc, depth, dev):
+    """Where the pixels that miss the 1 % ground-truth bar are.  Every pixel of
+    every reference view falls in one class, in this order: the 6-px border
+    that DepthToWeak marks UNKNOWN (DPE.cu:2605-2611); seen by fewer than 2 of
+    its sources (out of view or occluded: its ground-truth point projects
+    outside the source or lands over 1 % in front of the source's ground-truth
+    depth at the nearest pixel); on a weak-texture patch of the scene; textured
+    interior.  Per class: share of all pixels and share within 1 %.
+    "resolvable" = the textured interior seen by >= 2 sources."""
+    import numpy as np
+    import torch
+    classes = ["border", "seen_by_lt2_sources", "weak_texture", "textured_interior"]
+    tot = {c: [0, 0] for c in classes}
+    for v, view in enumerate(sc.views):
+        H, W = view.depth.shape
+        gt = torch.from_numpy(view.depth.astype(np.float64)).to(dev)
+        K = torch.from_numpy(view.K.astype(np.float64)).to(dev)
+        R = torch.from_numpy(view.R.astype(np.float64)).to(dev)
+        t = torch.from_numpy(view.t.astype(np.float64)).to(dev)
+        ys, xs = torch.meshgrid(torch.arange(H, device=dev, dtype=torch.float64),
+                                torch.arange(W, device=dev, dtype=torch.float64), indexing="ij")
+        Xc = torch.stack([(xs - K[0, 2]) / K[0, 0] * gt, (ys - K[1, 2]) / K[1, 1] * gt, gt], -1)
+        Xw = (Xc - t) @ R  # R^T (Xc - t)
+        seen = torch.zeros((H, W), dtype=torch.int32, device=dev)
+        for j in sc.pairs[v]:
+            sv = sc.views[j]
+            Ks = torch.from_numpy(sv.K.astype(np.float64)).to(dev)
+            Rs = torch.from_numpy(sv.R.astype(np.float64)).to(dev)
+            ts = torch.from_numpy(sv.t.astype(np.float64)).to(dev)
+            Xs = Xw @ Rs.T + ts
+            z = Xs[..., 2]
+            u = Ks[0, 0] * Xs[..., 0] / z + Ks[0, 2]
+            w = Ks[1, 1] * Xs[..., 1] / z + Ks[1, 2]
+            inb = (z > 0) & (u >= 0) & (u <= W - 1) & (w >= 0) & (w <= H - 1)
+            ui = u.round().clamp(0, W - 1).long()
+            wi = w.round().clamp(0, H - 1).long()
+            gs = torch.from_numpy(sv.depth.astype(np.float64)).to(dev)[wi, ui]
+            seen += (inb & ((z - gs).abs() / z < 0.01)).int()
+        good = ((depth[v].double() - gt).abs() / gt < 0.01)
+        border = torch.zeros((H, W), dtype=torch.bool, device=dev)
+        border[:6, :] = border[-6:, :] = True
+        border[:, :6] = border[:, -6:] = True
+        weak = torch.from_numpy(view.weak).to(dev)
+        rest = ~border
+        masks = {"border": border, "seen_by_lt2_sources": rest & (seen < 2)}
+        rest = rest & (seen >= 2)
+        masks["weak_texture"] = rest & weak
+        masks["textured_interior"] = rest & ~weak
+        for c in classes:
+            tot[c][0] += int(masks[c].sum())
+            tot[c][1] += int((masks[c] & good).sum())
+    n = sum(a for a, _ in tot.values())
+    out = {c: {"share_of_pixels": a / n, "within_1pct": b / max(a, 1)} for c, (a, b) in tot.items()}
+    out["resolvable_within_1pct"] = out["textured_interior"]["within_1pct"]
+    return out
 
 
 def host_io_bench(args):
@@ -594,20 +675,26 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(sc, threads):
-    """The CPU oracle on a bounded sample of the same workload: one FIRST_INIT
-    call with 2 iterations on a 960x720 centre crop of reference view 10 and its
-    10 sources (principal points shifted), timed with `threads` host threads,
-    and the same call on a 240x180 crop with one thread."""
+def cpu_baseline(sc, threads, state):
+    """The CPU oracle on a bounded sample of the same workload: both process
+    calls of one reference view (FIRST_INIT with 8 iterations, then REFINE_ITER
+    with geometric consistency, the weak path and edge sampling, 2 iterations)
+    on the 800x600 centre crop of reference view 10 and its 10 sources
+    (principal points shifted), timed with `threads` host threads; the REFINE
+    call reads the GPU's final states of the same crop (prior plane, pixel
+    states, selected views, the 11 views' depths).  s_per_depth_map scales the
+    two calls' time by the full view's pixel count (x4).  Beside it, one
+    FIRST_INIT call with 2 iterations on a 240x180 crop with one thread.
+    `state(i)` returns view i's (depth, normal, pixel states, selected views)."""
     import numpy as np
     import oracle_lib
-    from dpe_amd import abi, host
+    from dpe_amd import abi, host, support
     nthreads = threads or min(16, os.cpu_count() or 1)
     ref = min(10, len(sc.views) - 1)
     ids = [ref] + sc.pairs[ref]
     lib = oracle_lib.load_oracle()
 
-    def run(cw, ch, nt):
+    def crop_inputs(cw, ch):
         x0, y0 = (sc.width - cw) // 2, (sc.height - ch) // 2
         imgs, cams = [], []
         for i in ids:
@@ -617,9 +704,9 @@ def cpu_baseline(sc, threads):
             K[0, 2] -= x0
             K[1, 2] -= y0
             cams.append(abi.camera_from_arrays(K, v.R, v.t, cw, ch, sc.depth_min, sc.depth_max))
-        prm = host.call_params(sc.depth_min, sc.depth_max, max_iterations=2, state=abi.FIRST_INIT, use_APD=0,
-                               use_edge=0, use_limit=0, use_label=0, geom_consistency=0)
-        pin = host.ProblemInputs(imgs, cams, prm)
+        return x0, y0, imgs, cams
+
+    def timed(pin, nt):
         fam = []
         t = time.perf_counter()
         oracle_lib.run_oracle(lib, pin, nt, fam)
@@ -627,14 +714,38 @@ def cpu_baseline(sc, threads):
         # the GPU's unit: every family but the evaluations the reference never reads
         return sum(h for h, _ in fam[:abi.NCC_FAMILIES]), dt
 
-    hyps, dt = run(960, 720, nthreads)
-    hyps1, dt1 = run(240, 180, 1)
-    return {"value": hyps / dt, "unit": "NCC plane-hyp/s", "cores": nthreads, "kind": "port",
-            "sample": f"oracle FIRST_INIT call, 2 iterations, 960x720 centre crop of view {ref} + 10 sources, "
-                      f"{hyps} NCC-hyps in {dt:.1f}s on {nthreads} threads",
+    cw, ch = 800, 600
+    x0, y0, imgs, cams = crop_inputs(cw, ch)
+    edges, labels = support.support_maps(sc.views[ref].image[y0:y0 + ch, x0:x0 + cw])
+    prm = host.call_params(sc.depth_min, sc.depth_max, max_iterations=8, state=abi.FIRST_INIT, use_APD=0,
+                           use_edge=0, geom_consistency=0, weak_peak_radius=6, call_index=0)
+    h1, t1 = timed(host.ProblemInputs(imgs, cams, prm, edges=edges, edges_low=edges, labels=labels), nthreads)
+    crop = (slice(y0, y0 + ch), slice(x0, x0 + cw))
+    st = {i: state(i) for i in ids}
+    depth, normal, states, selected = st[ref]
+    prm = host.call_params(sc.depth_min, sc.depth_max, max_iterations=2, state=abi.REFINE_ITER, use_APD=1,
+                           use_edge=1, geom_consistency=1, weak_peak_radius=4, call_index=1,
+                           ransac_threshold=0.01 - 0.00125, rotate_time=2)
+    prior = np.concatenate([normal[crop], depth[crop][..., None]], -1)
+    pin = host.ProblemInputs(imgs, cams, prm, geom_depths=[st[i][0][crop] for i in ids], weak_info=states[crop],
+                             prior_planes=prior, prior_selected=selected[crop], edges=edges, edges_low=edges,
+                             labels=labels)
+    h2, t2 = timed(pin, nthreads)
+    _, _, imgs1, cams1 = crop_inputs(240, 180)
+    prm1 = host.call_params(sc.depth_min, sc.depth_max, max_iterations=2, state=abi.FIRST_INIT, use_APD=0,
+                            use_edge=0, use_limit=0, use_label=0, geom_consistency=0)
+    hyps1, dt1 = timed(host.ProblemInputs(imgs1, cams1, prm1), 1)
+    full = sc.width * sc.height / (cw * ch)
+    return {"value": (h1 + h2) / (t1 + t2), "unit": "NCC plane-hyp/s", "cores": nthreads, "kind": "port",
+            "sample": f"oracle, both calls of reference view {ref} (FIRST_INIT 8 iterations: {h1} NCC-hyps in "
+                      f"{t1:.1f}s; geometric REFINE_ITER 2 iterations from the GPU's states: {h2} in {t2:.1f}s) on "
+                      f"its {cw}x{ch} centre crop + 10 sources, {nthreads} threads",
+            "s_per_depth_map": (t1 + t2) * full,
+            "s_per_depth_map_note": f"the sample's two calls scaled by the view's pixel count (x{full:g})",
             "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
             "single_thread": {"value": hyps1 / dt1, "unit": "NCC plane-hyp/s",
-                              "sample": f"same call on a 240x180 centre crop, {hyps1} NCC-hyps in {dt1:.1f}s"}}
+                              "sample": f"FIRST_INIT call, 2 iterations, 240x180 centre crop, {hyps1} NCC-hyps in "
+                                        f"{dt1:.1f}s"}}
 
 
 if __name__ == "__main__":

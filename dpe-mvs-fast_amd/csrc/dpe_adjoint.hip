@@ -29,7 +29,9 @@ __device__ __forceinline__ float ncc_adjoint_view(const Dev &d, const SharedPatc
     const float2 pt = project(H, x, y);
     if (pt.x >= (float)vc.width || pt.x < 0.0f || pt.y >= (float)vc.height || pt.y < 0.0f) return 2.0f;
     float s_s = 0.0f, s_ss = 0.0f, s_rs = 0.0f;
-    window_sums_any<NT>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+    const int x0 = x + d.tap_off0, y0 = y + d.tap_off0, span = (NT - 1) * d.tap_inc;
+    window_sums_any<NT>(d, S, lane, x, y, v, H, refined_division_ok(H, x0, x0 + span, y0, y0 + span), s_s, s_ss,
+                        s_rs);
     taps += NT * NT;
     // ncc_from_sums with its intermediates
     const float inv = 1.0f / S.wsum[lane];

@@ -107,14 +107,16 @@ __device__ __forceinline__ float sample_fast(const char *__restrict__ cell0, flo
 // Every tap of the window [x0, x1] x [y0, y1] projects to [1/64, W-1-1/64] x
 // [1/64, H-1-1/64]: the projection is linear-fractional with w of one sign
 // (refined_division_ok), so its extremes over the window are at the corner
-// taps, and the margin covers rounding.  No tap then needs the clamp.
+// taps, and the margin covers rounding.  No tap then needs the clamp.  Called
+// only where refined_division_ok holds for the window, so the corners are
+// projected with the refined reciprocal (equal to project() there).
 __device__ __forceinline__ bool window_inside(const Dev &d, const float H[9], int x0, int x1, int y0, int y1) {
     const float m = 1.0f / 64.0f;
     const float ux = (float)(d.W - 1) - m, uy = (float)(d.H - 1) - m;
     bool ok = true;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const float2 p = project(H, (c & 1) ? x1 : x0, (c & 2) ? y1 : y0);
+        const float2 p = project_refined(H, (c & 1) ? x1 : x0, (c & 2) ? y1 : y0);
         ok = ok && p.x >= m && p.x <= ux && p.y >= m && p.y <= uy;
     }
     return ok;
@@ -239,12 +241,13 @@ __device__ __forceinline__ void shifted_window_sums(const Dev &d, const SharedPa
     }
 }
 
-// The window sums with the projection and quad layout this window and call allow.
+// The window sums with the projection and quad layout this window and call
+// allow; `refined` = refined_division_ok of the window.
 template <int NT>
 __device__ __forceinline__ void window_sums_any(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y, int v,
-                                                const float H[9], float &s_s, float &s_ss, float &s_rs) {
+                                                const float H[9], bool refined, float &s_s, float &s_ss,
+                                                float &s_rs) {
     const int x0 = x + d.tap_off0, y0 = y + d.tap_off0, span = (NT - 1) * d.tap_inc;
-    const bool refined = refined_division_ok(H, x0, x0 + span, y0, y0 + span);
     if (d.fast_gather && refined && window_inside(d, H, x0, x0 + span, y0, y0 + span)) {
         shared_window_sums<NT, true, QUAD_FP16, true>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
     } else if (d.quad_fmt == QUAD_FP16) {
@@ -292,10 +295,16 @@ __device__ __forceinline__ float ncc_shared(const Dev &d, const SharedPatch<NT> 
     const ViewConst &vc = d.vc[v];
     float H[9];
     homography(vc, d.cams[0], pl, H);
-    const float2 pt = project(H, x, y);
+    const int x0 = x + d.tap_off0, y0 = y + d.tap_off0, span = (NT - 1) * d.tap_inc;
+    const bool refined = refined_division_ok(H, x0, x0 + span, y0, y0 + span);
+    // the centre lies in the window's box when the last tap offset is >= 0, so
+    // the refined reciprocal is exact there as well
+    float2 pt;
+    if (refined && span >= -d.tap_off0) pt = project_refined(H, x, y);
+    else pt = project(H, x, y);
     if (pt.x >= (float)vc.width || pt.x < 0.0f || pt.y >= (float)vc.height || pt.y < 0.0f) return 2.0f;
     float s_s = 0.0f, s_ss = 0.0f, s_rs = 0.0f;
-    window_sums_any<NT>(d, S, lane, x, y, v, H, s_s, s_ss, s_rs);
+    window_sums_any<NT>(d, S, lane, x, y, v, H, refined, s_s, s_ss, s_rs);
     taps += NT * NT;
     return ncc_from_sums(S.sum_ref[lane], S.sum_ref_ref[lane], s_s, s_ss, s_rs, S.wsum[lane]);
 }

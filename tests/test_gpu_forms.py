@@ -264,3 +264,46 @@ def test_batched_form_is_bit_identical_to_single_calls():
             assert all(s.ncc_hyps > 0 for s in stats)
     finally:
         batch.close()
+
+
+def test_batched_form_at_the_batch_workload_sizes():
+    """The batched form at the batch workload's image sizes: one seeded
+    problem each of 640x480, 1024x768, 1600x1200 and 2048x1536 with 5-10
+    source views, FIRST_INIT with 8 iterations, in the combined form (fp16
+    gathers, 2 Gauss-Newton steps, autotuner, graph) on 4 lanes, run twice
+    (tiles sampled, then decided).  Every output equals the same problem's
+    single call in the same form bit for bit, and the plain single call's
+    depth agrees with ground truth about as well."""
+    rng = np.random.default_rng(0x5EED00C6)
+    sizes = [(640, 480), (1024, 768), (1600, 1200), (2048, 1536)]
+    pins, gts = [], []
+    for k, (w, h) in enumerate(sizes):
+        sc = scene.make_scene(w, h, 1500.0 * w / 1600, 11, 60.0, seed=0x5EED0060 + k, n_src=10, supersample=1)
+        ref = int(rng.integers(0, len(sc.views)))
+        nsrc = int(rng.integers(5, 11))
+        pins.append(host.first_call_inputs(sc, ref, sc.pairs[ref][:nsrc], max_iterations=8))
+        gts.append(sc.views[ref].depth)
+        print(f"problem {k}: {w}x{h}, reference {ref}, {nsrc} sources", flush=True)
+    form = abi.Form(precision=abi.PRECISION_FP16, gauss_newton_steps=2, autotune=1, graph=1)
+    ctx = host.Context(0)
+    try:
+        single = [host.run_hip(ctx, p, form) for p in pins]
+        plain = [host.run_hip(ctx, p) for p in pins]
+    finally:
+        ctx.close()
+    for k, p in enumerate(pins):
+        dmin, dmax = p.params.depth_min, p.params.depth_max
+        acc = [float((np.abs(host.assemble(o, dmin, dmax)["depth"] - gts[k]) / gts[k] < 0.01).mean())
+               for o in (single[k], plain[k])]
+        print(f"problem {k}: within 1 % of ground truth: combined form {acc[0]:.4f}, plain {acc[1]:.4f}")
+        assert acc[0] > acc[1] - 0.05
+    batch = host.Batch(0, lanes=4)
+    try:
+        for rep in range(2):
+            outs, stats, wall = batch.run(pins, form)
+            print(f"batch {rep}: {len(pins)} problems in {wall:.1f} ms; tiles "
+                  f"{[(s.tile_full, s.tile_weighted) for s in stats]}; graph {[s.graph for s in stats]}")
+            for k, (o, ref_out) in enumerate(zip(outs, single)):
+                assert _same(o, ref_out), (rep, k)
+    finally:
+        batch.close()
