@@ -1182,7 +1182,11 @@ __device__ __forceinline__ uint8_t edge_low_at(const Dev &d, long idx) {
     return d.edge_low[idx];
 }
 
-// BresenhamLine, DPE.cu:162-248 (reads outside the low-res map return 0)
+// BresenhamLine, DPE.cu:162-248 (reads outside the low-res map return 0).  The
+// result is whether any pixel a walk tests is an edge, and which pixels a walk
+// tests does not depend on the map, so each walk is taken in chunks of 8 steps:
+// the 8 positions first, then their 8 reads together (the reference's read,
+// test and branch per step would leave one read in flight at a time).
 __device__ bool bresenham(const Dev &d, int Ax, int Ay, int Bx, int By) {
     if (d.edge[Ax + Ay * d.W] || d.edge[Bx + By * d.W]) return false;
     const float scale_x = 1.0f * d.low_w / (float)d.W;
@@ -1201,21 +1205,36 @@ __device__ bool bresenham(const Dev &d, int Ax, int Ay, int Bx, int By) {
         int erro = (dx > dy ? dx : dy) / 2;
         int step = 0;
         bool tagx = true, tagy = true;
-        while (tagx || tagy) {
-            if (x0 == x1) tagx = false;
-            if (y0 == y1) tagy = false;
-            const int e2 = erro;
-            if (e2 > -dx) {
-                erro -= dy;
-                x0 += sx;
+        bool go = true;  // the walk's loop runs another step
+        while (go) {
+            long idx[8];
+            bool use[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                use[k] = go;
+                idx[k] = 0;
+                if (go) {
+                    if (x0 == x1) tagx = false;
+                    if (y0 == y1) tagy = false;
+                    const int e2 = erro;
+                    if (e2 > -dx) {
+                        erro -= dy;
+                        x0 += sx;
+                    }
+                    if (e2 < dy) {
+                        erro += dx;
+                        y0 += sy;
+                    }
+                    idx[k] = (long)x0 + (long)y0 * width;
+                    step += 1;
+                    go = step < max_step && (tagx || tagy);
+                }
             }
-            if (e2 < dy) {
-                erro += dx;
-                y0 += sy;
-            }
-            if (edge_low_at(d, (long)x0 + (long)y0 * width)) return true;
-            step += 1;
-            if (step >= max_step) break;
+            uint8_t hit = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (use[k]) hit |= edge_low_at(d, idx[k]);
+            if (hit) return true;
         }
     }
     return false;
@@ -1438,6 +1457,15 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
                     const float tx = (float)px + od.x * (float)radius;
                     const float ty = (float)py + od.y * (float)radius;
                     if (tx < 0 || ty < 0 || tx >= (float)W || ty >= (float)H) break;
+                    // The 4 jittered probes of this radius: their pixels and the state
+                    // and nearest-strong reads of all 4 first (the reads then overlap),
+                    // then the tests in probe order.  Each probe takes 4 draws and the
+                    // probes after the accepted one take none, so the stream position
+                    // is set back to just after the accepted probe.
+                    const uint32_t n0 = rng.n;
+                    short2 probe[4];
+                    bool inside[4];
+#pragma unroll
                     for (int ri = 0; ri < 4; ++ri) {
                         const uint32_t s1 = rng.next();
                         const uint32_t m1 = rng.next();
@@ -1447,25 +1475,33 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
                         const int rys = (int)(((s2 % 2 == 0) ? 1u : 0xFFFFFFFFu) * m2 % (uint32_t)d.gn_shift);
                         float2 dv = make_float2(od.x * 20 + (float)rxs, od.y * 20 + (float)rys);
                         normalize2(&dv);
-                        short2 np = make_short2((short)(int)((float)px + dv.x * (float)radius),
+                        probe[ri] = make_short2((short)(int)((float)px + dv.x * (float)radius),
                                                 (short)(int)((float)py + dv.y * (float)radius));
-                        if (np.x < min_margin || np.y < min_margin || np.x >= W - min_margin || np.y >= H - min_margin)
-                            continue;
-                        int nc = np.x + np.y * W;
-                        if (d.weak[nc] != DPE_STRONG) {
-                            np = d.nearest[nc];
-                            if (np.x == -1 || np.y == -1) continue;
-                            nc = np.x + np.y * W;
-                        }
+                        inside[ri] = !(probe[ri].x < min_margin || probe[ri].y < min_margin ||
+                                       probe[ri].x >= W - min_margin || probe[ri].y >= H - min_margin);
+                    }
+                    uint8_t state[4];
+#pragma unroll
+                    for (int ri = 0; ri < 4; ++ri)
+                        state[ri] = inside[ri] ? d.weak[probe[ri].x + probe[ri].y * W] : (uint8_t)DPE_STRONG;
+#pragma unroll
+                    for (int ri = 0; ri < 4; ++ri)
+                        if (inside[ri] && state[ri] != DPE_STRONG) probe[ri] = d.nearest[probe[ri].x + probe[ri].y * W];
+                    int accepted = -1;
+                    for (int ri = 0; ri < 4; ++ri) {
+                        const short2 np = probe[ri];
+                        if (!inside[ri] || np.x == -1 || np.y == -1) continue;
                         float2 td = make_float2((float)(np.x - px), (float)(np.y - py));
                         normalize2(&td);
                         const float ca = td.x * od.x + td.y * od.y;
                         if (ca > d.gn_thr && (!edge_limit || !bresenham(d, px, py, np.x, np.y))) {
                             pts[di].w = pack_pixel(np);
                             strong_point_size++;
+                            accepted = ri;
                             break;
                         }
                     }
+                    if (accepted >= 0) rng.n = n0 + 4u * (uint32_t)(accepted + 1);
                     if (pixel_valid(pts[di].w)) break;
                 }
                 float2 rd;
