@@ -161,15 +161,41 @@ __device__ __forceinline__ void weak_window_shape(const Dev &d, int center, int 
 }
 
 // Measured and not kept here: the clamp-free gather (18.0 -> 24.4 ms per weak
-// launch) and fully unrolled 3x3 / 6x6 windows (18.0 -> 18.7-19.2 ms).  This
-// kernel is bound by the fabric traffic of its scattered neighbour-window taps
-// (about 6 TB/s counted), not by instruction latency.
-template <bool REFINED, int FMT>
+// launch) and unrolled 6x6 centre windows (3 waves per SIMD at 158 VGPRs, or 2
+// with the 3x3 ones).  This kernel waits on its scattered neighbour-window
+// gathers (about 6 TB/s counted fabric traffic, 4x its algorithmic bytes).
+// NTW > 0: the window has NTW taps per side (2 radius / inc + 1 == NTW) and is
+// unrolled, so that its reads are in flight together (the neighbour windows,
+// 3x3: k_weak_update 19.4 -> 18.5 ms per launch at 1600x1200 at 3 waves per
+// SIMD, where unrolling only the rows at 4 waves, or capping the unrolled
+// kernel at 4 waves, measured slower).
+template <bool REFINED, int FMT, int NTW = 0>
 __device__ __forceinline__ float ncc_window_pre(const Dev &d, int v, const float H[9], int cx, int cy, int radius,
                                                 int inc, const float2 *__restrict__ tw, int n,
                                                 const float *__restrict__ ws, unsigned long long &taps) {
     float sum_src = 0.0f, sum_src_src = 0.0f, sum_ref_src = 0.0f;
     int t = 0;
+    if constexpr (NTW > 0) {
+#pragma unroll
+        for (int ii = 0; ii < NTW; ++ii) {
+            float r_src = 0.0f, r_ss = 0.0f, r_rs = 0.0f;
+#pragma unroll
+            for (int jj = 0; jj < NTW; ++jj) {
+                const int rx = cx - radius + ii * inc, ry = cy - radius + jj * inc;
+                const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
+                const float src = sample_src<FMT>(d, v, sp.x, sp.y);
+                const float2 e = tw[(size_t)(ii * NTW + jj) * n];
+                r_src = fmaf(e.x, src, r_src);
+                r_ss = fmaf(e.x * src, src, r_ss);
+                r_rs = fmaf(e.y, src, r_rs);
+            }
+            sum_src += r_src;
+            sum_src_src += r_ss;
+            sum_ref_src += r_rs;
+        }
+        taps += NTW * NTW;
+        return ncc_from_sums(ws[0], ws[n], sum_src, sum_src_src, sum_ref_src, ws[2 * (size_t)n]);
+    }
     for (int i = -radius; i <= radius; i += inc) {
         float r_src = 0.0f, r_ss = 0.0f, r_rs = 0.0f;
         for (int j = -radius; j <= radius; j += inc) {
@@ -190,13 +216,23 @@ __device__ __forceinline__ float ncc_window_pre(const Dev &d, int v, const float
     return ncc_from_sums(ws[0], ws[n], sum_src, sum_src_src, sum_ref_src, ws[2 * (size_t)n]);
 }
 
+// ncc_window_pre unrolled for 3x3 windows (the neighbour windows, radius 5 step 5)
+template <bool REFINED, int FMT>
+__device__ __forceinline__ float ncc_window_pre_unrolled(const Dev &d, int v, const float H[9], int cx, int cy, int radius,
+                                                     int inc, const float2 *__restrict__ tw, int n,
+                                                     const float *__restrict__ ws, unsigned long long &taps) {
+    const int nt = 2 * radius / inc + 1;
+    if (nt == 3) return ncc_window_pre<REFINED, FMT, 3>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
+    return ncc_window_pre<REFINED, FMT>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
+}
+
 __device__ float ncc_window_pre_any(const Dev &d, int v, const float H[9], int cx, int cy, int radius, int inc,
                                     const float2 *__restrict__ tw, int n, const float *__restrict__ ws,
                                     unsigned long long &taps) {
     const int last = -radius + ((2 * radius) / inc) * inc;
     const bool refined = refined_division_ok(H, cx - radius, cx + last, cy - radius, cy + last);
     if (d.quad_fmt == QUAD_FP16) {
-        if (refined) return ncc_window_pre<true, QUAD_FP16>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
+        if (refined) return ncc_window_pre_unrolled<true, QUAD_FP16>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
         return ncc_window_pre<false, QUAD_FP16>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
     }
     if (d.quad_fmt == QUAD_BF16) {
