@@ -1276,6 +1276,17 @@ __device__ bool bresenham(const Dev &d, int Ax, int Ay, int Bx, int By) {
     return false;
 }
 
+// Bit position of the unordered pair (i, j), i != j, of compacted RANSAC points
+// in a 2-bit-per-pair table (2 (hi (hi - 1) / 2 + lo)): the reference keeps the
+// BresenhamLine result of every pair it has walked (edge_test, DPE.cu:2311,
+// 2962; 0 = not walked, 1 = an edge between, 2 = none) and walks each pair
+// once.  BresenhamLine is symmetric in its endpoints (it tests both and walks
+// both ways), so one entry serves both orders.
+__device__ __forceinline__ int edge_pair_bit(int i, int j) {
+    const int lo = min(i, j), hi = max(i, j);
+    return hi * (hi - 1) + 2 * lo;
+}
+
 // PointinTriangle, DPE.cu:139-160
 __device__ bool point_in_triangle(short2 A, short2 B, short2 C, int px, int py) {
     const float abx = (float)(B.x - A.x), aby = (float)(B.y - A.y);
@@ -1445,7 +1456,10 @@ __device__ __forceinline__ short2 unpack_pixel(float f) {
 __device__ __forceinline__ bool pixel_valid(float f) { return __builtin_bit_cast(uint32_t, f) != 0xFFFFFFFFu; }
 
 // GenNeighbours, DPE.cu:2107-2467
-__global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
+// cache_words: 32-bit words per lane of the dynamic LDS pair table
+// (edge_pair_bit; word w of lane l at [w * 64 + l]), 0 = no table.
+__global__ __launch_bounds__(256) void k_gen_neighbours(Dev d, int cache_words) {
+    extern __shared__ uint32_t edge_cache[];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.weak_count) return;
     const int p = d.weak_list[i];
@@ -1641,6 +1655,21 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
         float temp_thr = ransac_threshold;
         bool has_consist = false;
         bool must_in_triangle = (d.prm.use_label && d.label[p] > 0 && edge_limit) ? false : true;
+        // the pair table of this lane, when its points fit
+        const bool cached = cache_words > 0 && vc * (vc - 1) <= 32 * cache_words;
+        uint32_t *table = edge_cache + (threadIdx.x & 63);
+        if (cached)
+            for (int w = 0; w < cache_words; ++w) table[w * 64] = 0u;
+        auto edge_between = [&](int i, int j, short2 P, short2 Qp) -> bool {
+            if (!cached) return bresenham(d, P.x, P.y, Qp.x, Qp.y);
+            const int pb = edge_pair_bit(i, j);
+            uint32_t &w = table[(pb >> 5) * 64];
+            const uint32_t e = (w >> (pb & 31)) & 3u;
+            if (e) return e == 1u;
+            const bool r = bresenham(d, P.x, P.y, Qp.x, Qp.y);
+            w |= (r ? 1u : 2u) << (pb & 31);
+            return r;
+        };
         while (iteration > 0 && max_iter > 0) {
             max_iter--;
             const int a = (int)(rng.next() % (uint32_t)vc);
@@ -1650,9 +1679,7 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d) {
             const short2 sa = unpack_pixel(pts[a].w), sb = unpack_pixel(pts[b].w), sc3 = unpack_pixel(pts[c].w);
             if (must_in_triangle && !point_in_triangle(sa, sb, sc3, px, py)) continue;
             if (edge_limit) {
-                if (bresenham(d, sa.x, sa.y, sb.x, sb.y) || bresenham(d, sb.x, sb.y, sc3.x, sc3.y) ||
-                    bresenham(d, sc3.x, sc3.y, sa.x, sa.y))
-                    continue;
+                if (edge_between(a, b, sa, sb) || edge_between(b, c, sb, sc3) || edge_between(c, a, sc3, sa)) continue;
             }
             bool normal_consistency = false;
             if (d.prm.geom_consistency && edge_limit) {
@@ -1826,6 +1853,7 @@ __global__ void k_ransac(Dev d, int iter) {
     float4 best = make_float4(0, 0, 0, 0);
     bool has_best = false, has_strong = false;
     bool must_in_triangle = (d.prm.use_label && d.label[p] > 0 && edge_limit) ? false : true;
+    uint64_t edge_table = 0;  // edge_pair_bit entries of this pixel's points
     while (iteration--) {
         const int a = (int)(rng.next() % (uint32_t)count);
         const int b = (int)(rng.next() % (uint32_t)count);
@@ -1842,9 +1870,16 @@ __global__ void k_ransac(Dev d, int iter) {
         }
         if (must_in_triangle && !point_in_triangle(sp[a], sp[b], sp[c], px, py)) continue;
         if (edge_limit) {
-            if (bresenham(d, sp[a].x, sp[a].y, sp[b].x, sp[b].y) || bresenham(d, sp[b].x, sp[b].y, sp[c].x, sp[c].y) ||
-                bresenham(d, sp[c].x, sp[c].y, sp[a].x, sp[a].y))
-                continue;
+            // the pair table (edge_pair_bit) of at most 8 points in one 64-bit word
+            auto edge_between = [&](int i, int j) -> bool {
+                const int pb = edge_pair_bit(i, j);
+                const uint32_t e = (uint32_t)(edge_table >> pb) & 3u;
+                if (e) return e == 1u;
+                const bool r = bresenham(d, sp[i].x, sp[i].y, sp[j].x, sp[j].y);
+                edge_table |= (uint64_t)(r ? 1u : 2u) << pb;
+                return r;
+            };
+            if (edge_between(a, b) || edge_between(b, c) || edge_between(c, a)) continue;
         }
         const float3 A = s3[a], B = s3[b], Cc = s3[c];
         const float3 AC = make_float3(A.x - Cc.x, A.y - Cc.y, A.z - Cc.z);
@@ -2382,7 +2417,14 @@ void launch_find_nearest_strong(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_find_nearest_strong, grid1(d.P), dim3(256), 0, s, d, (const int16_t *)g);
 }
 void launch_gen_neighbours(const Dev &d, hipStream_t s) {
-    if (d.weak_count > 0) hipLaunchKernelGGL(k_gen_neighbours, grid1(d.weak_count, 64), dim3(64), 0, s, d);
+    if (d.weak_count > 0) {
+        // GenNeighbours collects at most 8 rotate_time points (16 rotate_time with
+        // labels, at most 63); the pair table is kept in LDS when 32 points fit.
+        const int max_points = min(63, 8 * d.prm.rotate_time * (d.prm.use_label ? 2 : 1));
+        const int words = max_points <= 32 ? (max_points * (max_points - 1) + 31) / 32 : 0;
+        hipLaunchKernelGGL(k_gen_neighbours, grid1(d.weak_count, 64), dim3(64), (size_t)words * 64 * 4, s, d,
+                           words);
+    }
 }
 void launch_neighbour_update(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_neighbour_update, grid1(d.P), dim3(256), 0, s, d);
