@@ -630,11 +630,38 @@ __device__ __forceinline__ float weighted_sum(const Dev &d, const void *Sv, int 
     return acc;
 }
 
+// The wave's lanes see each other's LDS writes made before the call.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Set bits of the wave mask b in the lanes below this one.
+__device__ __forceinline__ int lanes_below(unsigned long long b) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+}
+
 // One workgroup = 64 pixels x WAVES waves; wave w evaluates slot WAVES*blockIdx.y + w.
+// The windows of a wave are its (pixel, view) pairs, packed densely: a pixel
+// sums over its own views only (the weighted ones, or the selected ones), and
+// the 64 pixels of a wave select different subsets (at 1600x1200 a pixel uses
+// ~6.6 of 10 views while the 64 use ~9.5 between them), so a loop over the
+// views with idle lanes runs ~9.5 windows per lane, the packed list ~7.  The
+// list is view-major (view 1's pixels in lane order, then view 2's, ...), so
+// the lanes of a round gather from one or two source views at neighbouring
+// pixels, as the per-view loop does (a pixel-major list spreads a round over
+// all views and touches ~2x the cache lines).  Round r evaluates pairs
+// 64r..64r+63, one per lane, the pixel's reference patch read from its LDS
+// column; then each pixel's lane folds its pairs of the round in view order,
+// the order of the sums above.
+// Dynamic LDS per wave (wtd_list_bytes): the round's 64 (cost, geometric
+// cost) results, per view the ballot of its pixels and its first list index,
+// and the list, entry = (pixel lane << 5) | (view - 1).
 template <int NT, int MODE, bool PREC, int WAVES>
 __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_weighted(Dev d, int colour, int Qn, int nslot,
                                                                   float *__restrict__ out) {
     __shared__ SharedPatch<NT> S;
+    extern __shared__ float2 wtd_lists[];
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = threadIdx.x / kLanes;
     const int q = blockIdx.x * kLanes + lane;
@@ -649,17 +676,88 @@ __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_weighted(Dev d, int colo
     float4 pl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float wn = 1.0f;
     uint32_t sel = 0;
+    const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
+    // views: bit v - 1 = view v enters this pixel's sum
+    uint32_t views = 0;
+    bool act = false, twin = false;
     if (any && slot < nslot && weighted_slot<MODE>(d, Qn, q, p, slot, pl, wn, sel)) {
-        const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
         const uint8_t fl = MODE == MODE_POST ? d.post_flags[p] : 0;
-        if (MODE == MODE_POST && post_has_twin(slot, fl)) {
-            float tw;
-            out[(size_t)slot * Qn + q] =
-                weighted_sum<NT, MODE, PREC>(d, &S, lane, x, y, slot, vw, sel, pl, wn, hyps, taps, &tw);
-            out[(size_t)(slot + kPostTwin) * Qn + q] = tw;
-        } else if (!(MODE == MODE_POST && post_is_twin(slot, fl))) {
-            out[(size_t)slot * Qn + q] = weighted_sum<NT, MODE, PREC>(d, &S, lane, x, y, slot, vw, sel, pl, wn, hyps, taps);
+        if (!(MODE == MODE_POST && post_is_twin(slot, fl))) {
+            act = true;
+            twin = MODE == MODE_POST && post_has_twin(slot, fl);
+            for (int v = 1; v < d.N; ++v) {
+                const bool use = (MODE == MODE_REFINE) ? (vw[v - 1] > 0) : (isSet(sel, v - 1) != 0);
+                if (use) views |= 1u << (v - 1);
+            }
         }
+    }
+    const int NV = d.N - 1;
+    char *lds = (char *)wtd_lists + (size_t)wave * wtd_wave_bytes(d.N);
+    float2 *res = (float2 *)lds;
+    unsigned long long *vball = (unsigned long long *)(lds + kLanes * sizeof(float2));
+    uint16_t *vstart = (uint16_t *)(vball + NV);
+    uint16_t *pairs = vstart + NV;
+    int total = 0;
+    for (int v = 0; v < NV; ++v) {
+        const bool mine = (views >> v) & 1u;
+        const unsigned long long b = __ballot(mine);
+        if (mine) pairs[total + lanes_below(b)] = (uint16_t)((lane << 5) | v);
+        if (lane == 0) {
+            vball[v] = b;
+            vstart[v] = (uint16_t)total;
+        }
+        total += __popcll(b);
+    }
+    wave_sync();
+    const bool geom = d.prm.geom_consistency != 0;
+    const float gf = d.prm.geom_factor;
+    float acc = 0.0f, acc2 = 0.0f;
+    uint32_t left = views;  // views not folded yet
+    for (int base = 0; base < total; base += kLanes) {
+        const int i = base + lane;
+        const uint32_t e = i < total ? pairs[i] : 0u;
+        const int src = (int)(e >> 5), v = (int)(e & 31u) + 1;
+        const int ix = __shfl(x, src, kLanes), iy = __shfl(y, src, kLanes);
+        const float4 ipl = make_float4(__shfl(pl.x, src, kLanes), __shfl(pl.y, src, kLanes),
+                                       __shfl(pl.z, src, kLanes), __shfl(pl.w, src, kLanes));
+        float c = 0.0f, g = 0.0f;
+        if (i < total) {
+            c = ncc_shared<NT, PREC>(d, S, src, ix, iy, v, ipl, taps);
+            hyps++;
+            if (MODE == MODE_POST && geom) g = geom_cost(d, ix, iy, v, ipl);
+        }
+        res[lane] = make_float2(c, g);
+        wave_sync();
+        // this lane's pairs of the round: its next views while their list index is below base + 64
+        while (left) {
+            const int vi = __builtin_ctz(left);
+            const int at = (int)vstart[vi] + lanes_below(vball[vi]) - base;
+            if (at >= kLanes) break;
+            left &= left - 1;
+            const float2 r = res[at];
+            const float wv = (float)vw[vi];
+            if (MODE == MODE_REFINE) {
+                acc += wv * r.x;
+            } else if (slot <= 61) {
+                float tc = r.x;
+                if (geom) tc += gf * r.y;
+                acc += tc * wv;
+                if (twin) {  // LocalRefine's order (DPE.cu:2822-2825)
+                    acc2 += r.x * wv;
+                    if (geom) acc2 += gf * r.y * wv;
+                }
+            } else {
+                acc += r.x * wv;
+                if (geom) acc += gf * r.y * wv;
+            }
+        }
+        wave_sync();
+    }
+    if (act) {
+        acc /= wn;
+        if (MODE == MODE_POST && slot < 61) out[(size_t)slot * Qn + q] = fminf(2.0f, acc);
+        else out[(size_t)slot * Qn + q] = acc;
+        if (twin) out[(size_t)(slot + kPostTwin) * Qn + q] = acc2 / wn;
     }
     count_add(d.counters, (MODE == MODE_REFINE ? kFamRefine : kFamPost), hyps, taps);
 }
@@ -879,7 +977,8 @@ __global__ void k_strong_candidates(Dev d, int colour, int iter) {
                     min_cost = costs[tc];
                 }
             }
-            cp[(size_t)(1 + dd) * Q] = min_cost < 3.402823466e+38f ? best : -1;
+            const int adaptive = min_cost < 3.402823466e+38f ? best : -1;
+            cp[(size_t)(1 + dd) * Q] = adaptive;
             int bestf = -1;
             if (!on_edge) {
                 float mc = 3.402823466e+38f;
@@ -895,7 +994,13 @@ __global__ void k_strong_candidates(Dev d, int colour, int iter) {
                 }
                 if (!(mc < 3.402823466e+38f)) bestf = -1;
             }
-            cp[(size_t)(9 + dd) * Q] = bestf;
+            // The fixed scan (11 steps of min_step_len) is a prefix of the adaptive
+            // one whenever step_len == min_step_len (at 1600x1200 always off edges),
+            // so both often pick the same pixel.  Its costs then equal the adaptive
+            // candidate's and never replace it in k_strong_select (the test needs
+            // more good or fewer bad views), so it is not evaluated: -1, which the
+            // selection skips the same way.
+            cp[(size_t)(9 + dd) * Q] = (bestf >= 0 && bestf == adaptive) ? -1 : bestf;
         }
     } else {
         int pos[8];
@@ -2459,9 +2564,11 @@ static void launch_full(const Dev &d, hipStream_t s, int colour, int Qn, int nsl
 }
 template <int MODE, bool PREC, int WAVES>
 static void wtd_t(const Dev &d, hipStream_t s, int colour, int Qn, int nslot, float *out) {
+    // the pair lists sit at the start of the dynamic LDS, inside the tile's pad
+    const size_t lds = std::max(wtd_list_bytes(WAVES, d.N), (size_t)d.wtd_tile.lds_pad);
     hipLaunchKernelGGL((k_ncc_weighted<6, MODE, PREC, WAVES>),
-                       dim3((Qn + kLanes - 1) / kLanes, (nslot + WAVES - 1) / WAVES), dim3(kLanes * WAVES),
-                       d.wtd_tile.lds_pad, s, d, colour, Qn, nslot, out);
+                       dim3((Qn + kLanes - 1) / kLanes, (nslot + WAVES - 1) / WAVES), dim3(kLanes * WAVES), lds, s, d,
+                       colour, Qn, nslot, out);
 }
 template <int MODE>
 static void launch_wtd(const Dev &d, hipStream_t s, int colour, int Qn, int nslot, float *out) {

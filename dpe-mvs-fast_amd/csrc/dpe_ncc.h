@@ -133,6 +133,16 @@ struct SharedPatch {
     float sum_refc[kLanes], sum_refc2[kLanes];
 };
 
+// Dynamic LDS of a weighted-sum launch over N images, per wave: the round's
+// 64 (cost, geometric cost) results, per view the 64-bit ballot of its pixels
+// and its first list index (16 bits), and up to 64 (N - 1) 16-bit (pixel,
+// view) list entries; 8-byte aligned.
+__host__ __device__ constexpr size_t wtd_wave_bytes(int N) {
+    return ((size_t)kLanes * sizeof(float2) + (size_t)(N - 1) * (8 + 2) + (size_t)kLanes * (N - 1) * 2 + 7) / 8 * 8;
+}
+__host__ __device__ constexpr size_t wtd_list_bytes(int waves, int N) { return (size_t)waves * wtd_wave_bytes(N); }
+static_assert(sizeof(SharedPatch<6>) + wtd_list_bytes(8, kMaxImages) <= 65536, "weighted tile exceeds 64 KiB of LDS");
+
 // WAVES = waves of the workgroup (the tile's slot width); they fill the taps together.
 template <int NT, int WAVES = kWaves>
 __device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int x, int y, SharedPatch<NT> &S) {

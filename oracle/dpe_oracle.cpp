@@ -894,7 +894,11 @@ void StrongCandidates(const Ctx &C, int px, int py, int iter, bool flag[8], int 
                 if (min_cost < FLT_MAX) {
                     flag[d] = true;
                     const int tpos = mcx + mcy * W;
+                    // the pixel the adaptive scan already chose: equal costs, which
+                    // never replace it below (a result the reference never uses)
+                    tl_family = (had && tpos == positions[d]) ? kFamDead : kFamFull;
                     CostVectorOld(C, px, py, planes[tpos], tca);
+                    tl_family = kFamFull;
                     int good[2] = {0, 0}, bad[2] = {0, 0};
                     for (int i = 0; i < 2; i++) {
                         for (int j = 0; j < C.N - 1; j++) {
