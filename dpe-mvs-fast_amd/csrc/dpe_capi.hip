@@ -536,9 +536,9 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
         up.to(d.weak, in->weak_info, P);
         int *dcnt = (int *)buffer(ctx, B_WEAKCOUNTS, sizeof(int) * 3);
         size_t temp_bytes = 0;
-        HIPCHK(weak_lists_temp_bytes((int)P, &temp_bytes));
+        HIPCHK(weak_lists_temp_bytes(d.W, d.H, &temp_bytes));
         void *temp = buffer(ctx, B_WEAKTEMP, std::max(temp_bytes, (size_t)1));
-        HIPCHK(launch_weak_lists(d.weak, d.W, (int)P, d.half_rows, nmap, dwl, dwl + P, dwl + P + half_cap, dcnt, temp,
+        HIPCHK(launch_weak_lists(d.weak, d.W, d.H, d.half_rows, nmap, dwl, dwl + P, dwl + P + half_cap, dcnt, temp,
                                  temp_bytes, ctx->stream));
         int cnt_h[3] = {0, 0, 0};
         HIPCHK(hipMemcpyAsync(cnt_h, dcnt, sizeof(cnt_h), hipMemcpyDeviceToHost, ctx->stream));

@@ -74,7 +74,7 @@ struct WeakSelect {  // a WEAK pixel, of colour `colour` in rows < half_rows (co
     const uint8_t *weak;
     int W, half_rows, colour;
     __device__ __forceinline__ bool operator()(int i) const {
-        if (weak[i] != DPE_WEAK) return false;
+        if (i < 0 || weak[i] != DPE_WEAK) return false;
         if (colour < 0) return true;
         const int x = i % W, y = i / W;
         return y < half_rows && ((x + y) & 1) == colour;
@@ -86,6 +86,32 @@ struct WeakFlag {
 };
 using PixelIter = hipcub::CountingInputIterator<int>;
 using WeakFlagIter = hipcub::TransformInputIterator<int, WeakFlag, PixelIter>;
+// The per-colour weak lists enumerate pixels tile by tile (kWeakTileW x
+// kWeakTileH tiles in raster order, raster order inside a tile), so the 64
+// lanes of a weak-update wave hold a compact 2-D patch of weak pixels, whose
+// GenNeighbours points and windows overlap, rather than a run of one row
+// (weak update at 1600x1200: 8x8 to 32x32 tiles all -7..-9 %, 16x16 kept).
+// Item i of the padded tile grid maps to its pixel index, -1 = padding.
+#ifndef DPE_WEAK_TILE_W
+#define DPE_WEAK_TILE_W 16
+#endif
+#ifndef DPE_WEAK_TILE_H
+#define DPE_WEAK_TILE_H 16
+#endif
+constexpr int kWeakTileW = DPE_WEAK_TILE_W, kWeakTileH = DPE_WEAK_TILE_H;
+struct TiledPixel {
+    int W, H, tiles_x;
+    __host__ __device__ __forceinline__ int operator()(int i) const {
+        const int t = i / (kWeakTileW * kWeakTileH), r = i - t * (kWeakTileW * kWeakTileH);
+        const int ty = t / tiles_x, tx = t - ty * tiles_x;
+        const int x = tx * kWeakTileW + r % kWeakTileW, y = ty * kWeakTileH + r / kWeakTileW;
+        return (x < W && y < H) ? x + y * W : -1;
+    }
+};
+using TiledIter = hipcub::TransformInputIterator<int, TiledPixel, PixelIter>;
+static inline int tiled_items(int W, int H) {
+    return ((W + kWeakTileW - 1) / kWeakTileW) * ((H + kWeakTileH - 1) / kWeakTileH) * kWeakTileW * kWeakTileH;
+}
 __global__ void k_nmap_clear_others(const uint8_t *__restrict__ weak, int *__restrict__ nmap, int P) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < P && weak[i] != DPE_WEAK) nmap[i] = 0;
@@ -2110,7 +2136,7 @@ __device__ __forceinline__ int weak_candidates(const Dev &d, int p, int position
 // CheckerboardPropagationWeak + PlaneHypothesisRefinementWeak, DPE.cu:1672-1866, 1124-1216
 __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int iter, unsigned long long &hyps,
                                                   unsigned long long &taps) {
-    // XCD-contiguous ranges of the raster-ordered weak list: the windows of
+    // XCD-contiguous ranges of the tile-ordered weak list: the windows of
     // nearby weak pixels share source cache lines (13 % faster at 1600x1200)
     const int li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (li >= d.weak_colour_count[colour]) return;
@@ -2490,25 +2516,35 @@ void launch_make_quads_h(const Dev &d, const float *img, uint2 *quads, int fmt, 
 void launch_check_fp16_exact(const float *img, size_t n, int *flag, hipStream_t s) {
     hipLaunchKernelGGL(k_check_fp16_exact, dim3(1024), dim3(256), 0, s, img, n, flag);
 }
-hipError_t weak_lists_temp_bytes(int P, size_t *bytes) {
-    size_t a = 0, b = 0;
+hipError_t weak_lists_temp_bytes(int W, int H, size_t *bytes) {
+    const int P = W * H;
+    size_t a = 0, b = 0, c = 0;
     hipError_t e = hipcub::DeviceSelect::If(nullptr, a, PixelIter(0), (int *)nullptr, (int *)nullptr, P,
                                             WeakSelect{nullptr, 1, 0, -1});
     if (e != hipSuccess) return e;
+    e = hipcub::DeviceSelect::If(nullptr, c, TiledIter(PixelIter(0), TiledPixel{W, H, 1}), (int *)nullptr,
+                                 (int *)nullptr, tiled_items(W, H), WeakSelect{nullptr, 1, 0, 0});
+    if (e != hipSuccess) return e;
     e = hipcub::DeviceScan::ExclusiveSum(nullptr, b, WeakFlagIter(PixelIter(0), WeakFlag{nullptr}), (int *)nullptr, P);
-    *bytes = std::max(a, b);
+    *bytes = std::max(std::max(a, b), c);
     return e;
 }
-hipError_t launch_weak_lists(const uint8_t *weak, int W, int P, int half_rows, int *nmap, int *all, int *col0,
+hipError_t launch_weak_lists(const uint8_t *weak, int W, int H, int half_rows, int *nmap, int *all, int *col0,
                              int *col1, int *counts, void *temp, size_t temp_bytes, hipStream_t s) {
+    const int P = W * H;
     hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, WeakFlagIter(PixelIter(0), WeakFlag{weak}),
                                                     nmap, P, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_nmap_clear_others, grid1(P), dim3(256), 0, s, weak, nmap, P);
-    int *out[3] = {all, col0, col1};
-    for (int k = 0; k < 3; ++k) {
-        e = hipcub::DeviceSelect::If(temp, temp_bytes, PixelIter(0), out[k], counts + k, P,
-                                     WeakSelect{weak, W, half_rows, k - 1}, s);
+    // all weak pixels in raster order (GenNeighbours, RANSAC); per colour tiled
+    e = hipcub::DeviceSelect::If(temp, temp_bytes, PixelIter(0), all, counts, P, WeakSelect{weak, W, half_rows, -1},
+                                 s);
+    if (e != hipSuccess) return e;
+    const TiledIter tiled(PixelIter(0), TiledPixel{W, H, (W + kWeakTileW - 1) / kWeakTileW});
+    int *out[2] = {col0, col1};
+    for (int k = 0; k < 2; ++k) {
+        e = hipcub::DeviceSelect::If(temp, temp_bytes, tiled, out[k], counts + 1 + k, tiled_items(W, H),
+                                     WeakSelect{weak, W, half_rows, k}, s);
         if (e != hipSuccess) return e;
     }
     return hipGetLastError();
