@@ -86,11 +86,12 @@ struct WeakFlag {
 };
 using PixelIter = hipcub::CountingInputIterator<int>;
 using WeakFlagIter = hipcub::TransformInputIterator<int, WeakFlag, PixelIter>;
-// The per-colour weak lists enumerate pixels tile by tile (kWeakTileW x
-// kWeakTileH tiles in raster order, raster order inside a tile), so the 64
-// lanes of a weak-update wave hold a compact 2-D patch of weak pixels, whose
-// GenNeighbours points and windows overlap, rather than a run of one row
-// (weak update at 1600x1200: 8x8 to 32x32 tiles all -7..-9 %, 16x16 kept).
+// The weak lists enumerate pixels tile by tile (kWeakTileW x kWeakTileH tiles
+// in raster order, raster order inside a tile), so the 64 lanes of a wave hold
+// a compact 2-D patch of weak pixels, whose GenNeighbours searches, points and
+// windows overlap, rather than a run of one row (at 1600x1200: weak update
+// -7..-9 % for 8x8 to 32x32 tiles, 16x16 kept; GenNeighbours and RANSAC
+// together -80 ms per REFINE call over 20 views).
 // Item i of the padded tile grid maps to its pixel index, -1 = padding.
 #ifndef DPE_WEAK_TILE_W
 #define DPE_WEAK_TILE_W 16
@@ -2536,15 +2537,11 @@ hipError_t launch_weak_lists(const uint8_t *weak, int W, int H, int half_rows, i
                                                     nmap, P, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_nmap_clear_others, grid1(P), dim3(256), 0, s, weak, nmap, P);
-    // all weak pixels in raster order (GenNeighbours, RANSAC); per colour tiled
-    e = hipcub::DeviceSelect::If(temp, temp_bytes, PixelIter(0), all, counts, P, WeakSelect{weak, W, half_rows, -1},
-                                 s);
-    if (e != hipSuccess) return e;
     const TiledIter tiled(PixelIter(0), TiledPixel{W, H, (W + kWeakTileW - 1) / kWeakTileW});
-    int *out[2] = {col0, col1};
-    for (int k = 0; k < 2; ++k) {
-        e = hipcub::DeviceSelect::If(temp, temp_bytes, tiled, out[k], counts + 1 + k, tiled_items(W, H),
-                                     WeakSelect{weak, W, half_rows, k}, s);
+    int *out[3] = {all, col0, col1};
+    for (int k = 0; k < 3; ++k) {
+        e = hipcub::DeviceSelect::If(temp, temp_bytes, tiled, out[k], counts + k, tiled_items(W, H),
+                                     WeakSelect{weak, W, half_rows, k - 1}, s);
         if (e != hipSuccess) return e;
     }
     return hipGetLastError();

@@ -127,11 +127,11 @@ void launch_make_quads_h(const Dev &d, const float *img, uint2 *quads, int fmt, 
 // sets *flag (int, zeroed by the caller) when a value of img[0..n) is not fp16-exact
 void launch_check_fp16_exact(const float *img, size_t n, int *flag, hipStream_t s);
 // The weak-pixel index map and lists of one call from its pixel states
-// weak[0..W*H): nmap[i] = rank of i among the WEAK pixels (0 for the others);
-// all = the WEAK pixels in raster order; col[c] = the WEAK pixels of colour
-// (x + y) & 1 == c in rows y < half_rows, in tiled order (tile by tile, raster
-// inside a tile); counts = the three list lengths.  col[c] has room for
-// (P + 1) / 2 entries.
+// weak[0..W*H): nmap[i] = rank of i among the WEAK pixels in raster order (0
+// for the others); all = the WEAK pixels; col[c] = the WEAK pixels of colour
+// (x + y) & 1 == c in rows y < half_rows; the lists in tiled order (tile by
+// tile, raster inside a tile); counts = the three list lengths.  col[c] has
+// room for (P + 1) / 2 entries.
 hipError_t weak_lists_temp_bytes(int W, int H, size_t *bytes);
 hipError_t launch_weak_lists(const uint8_t *weak, int W, int H, int half_rows, int *nmap, int *all, int *col0,
                              int *col1, int *counts, void *temp, size_t temp_bytes, hipStream_t s);
