@@ -81,7 +81,8 @@ enum BufId {
     B_POSTBL, B_POSTDISP, B_POSTWN, B_POSTFLAGS, B_PCOST, B_COUNTERS, B_GIVEN_PLANES, B_GIVEN_COSTS,
     B_F_VIEWS, B_F_DEPTH, B_F_NORMAL, B_F_STATES, B_F_COLOR, B_F_MASK, B_F_CLAIM, B_F_USED0, B_F_USED1, B_F_ACC0,
     B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_POSTPLANES, B_WEAKLIST, B_ADJ_WEIGHTS,
-    B_ADJ_GRADS, B_QUADS_H, B_FLAG, B_WPRE, B_WSUM, B_VC64, B_WEAKCOUNTS, B_WEAKTEMP, B_COUNT
+    B_ADJ_GRADS, B_QUADS_H, B_FLAG, B_WPRE, B_WSUM, B_VC64, B_WEAKCOUNTS, B_WEAKTEMP, B_NOWROW, B_CURCOST,
+    B_CURVALID, B_COUNT
 };
 
 struct Fail {
@@ -622,6 +623,9 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
     d.now_plane = (float4 *)buffer(ctx, B_NOWPLANE, sizeof(float4) * Q);
     d.now_depth = (float *)buffer(ctx, B_NOWDEPTH, sizeof(float) * Q);
     d.now_cost = (float *)buffer(ctx, B_NOWCOST, sizeof(float) * Q);
+    d.now_row = (int *)buffer(ctx, B_NOWROW, sizeof(int) * Q);
+    d.cur_cost = (float *)buffer(ctx, B_CURCOST, sizeof(float) * NV * P);
+    d.cur_valid = (uint32_t *)buffer(ctx, B_CURVALID, sizeof(uint32_t) * P);
     d.wnorm = (float *)buffer(ctx, B_WNORM, sizeof(float) * Q);
     d.post_op = (float4 *)buffer(ctx, B_POSTOP, sizeof(float4) * P);
     d.post_bl = (float *)buffer(ctx, B_POSTBL, sizeof(float) * P);

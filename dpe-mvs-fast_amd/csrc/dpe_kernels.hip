@@ -493,14 +493,17 @@ __device__ float geom_cost(const Dev &d, int x, int y, int v, const float4 &pl) 
 // Per-view cost matrix out[(slot*NV + v-1)*Qn + q] of the plane of every
 // (pixel, slot): plane_src[cand_pos[slot*Qn+q]] if cand_pos is given (a -1
 // entry skips the slot), else plane_src[slot*Qn + q].  With view_mask (indexed
-// by pixel, Qn == P) only the views whose bit is set are evaluated.  One
-// workgroup = 64 pixels x WAVES waves; wave w evaluates slots w, w+WAVES, ...
+// by pixel, Qn == P) only the views whose bit is set are evaluated.  With
+// `cached` (the strong launches; slot 0 = the pixel's current plane) the views
+// of cur_valid take their cost from cur_cost instead of a window: the same
+// plane was evaluated at this pixel when it was set, so the value is the same.
+// One workgroup = 64 pixels x WAVES waves; wave w evaluates slots w, w+WAVES, ...
 template <int NT, bool PREC, int WAVES>
 __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_full(Dev d, int colour, int Qn, int nslot,
                                                               const int *__restrict__ cand_pos,
                                                               const float4 *__restrict__ plane_src,
                                                               const uint32_t *__restrict__ view_mask,
-                                                              float *__restrict__ out) {
+                                                              float *__restrict__ out, bool cached) {
     __shared__ SharedPatch<NT> S;
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = threadIdx.x / kLanes;
@@ -529,9 +532,16 @@ __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_full(Dev d, int colour, 
         }
         if (active) {
             const uint32_t mask = view_mask ? view_mask[q] : 0xFFFFFFFFu;
+            const int p = x + y * d.W;
+            const uint32_t known = (cached && slot == 0) ? d.cur_valid[p] : 0u;
             for (int v = 1; v < d.N; ++v) {
                 if (!isSet(mask, v - 1)) continue;
-                out[((size_t)slot * d.NV + (v - 1)) * Qn + q] = ncc_shared<NT, PREC>(d, S, lane, x, y, v, pl, taps);
+                float *o = out + ((size_t)slot * d.NV + (v - 1)) * Qn + q;
+                if (isSet(known, v - 1)) {
+                    *o = d.cur_cost[(size_t)(v - 1) * d.P + p];
+                    continue;
+                }
+                *o = ncc_shared<NT, PREC>(d, S, lane, x, y, v, pl, taps);
                 hyps++;
             }
         }
@@ -890,13 +900,16 @@ __global__ void k_random_init_planes(Dev d) {
 }
 
 // ComputeMultiViewInitialCostandSelectedViews (DPE.cu:784-830) /
-// ComputeMultiViewInitialCost (DPE.cu:832-861) from the slot-0 cost matrix.
+// ComputeMultiViewInitialCost (DPE.cu:832-861) from the current planes' costs
+// (cur_cost, every view in FIRST_INIT, the prior-selected ones otherwise).
 __global__ void k_random_init_costs(Dev d) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.P) return;
     const int NV = d.NV;
     const float cost_max = 2.0f;
-    const float *cm = d.cost_mat;  // [NV][P]
+    const float *cm = d.cur_cost;  // [NV][P]
+    const uint32_t all = NV >= 32 ? 0xFFFFFFFFu : (1u << NV) - 1u;
+    d.cur_valid[p] = d.prm.state == DPE_FIRST_INIT ? all : (d.selected[p] & all);
     if (d.prm.state == DPE_FIRST_INIT) {
         float cv[32];
         int num_valid = 0;
@@ -1277,12 +1290,14 @@ __global__ void k_strong_select(Dev d, int colour, int iter) {
     const float4 cur = d.planes[p];
     float depth_now = depth_from_plane(cam, cur, x, y);
     float4 plane_now = cur;
+    int now_row = 0;
     if (row_slot[mi] >= 0) {
         const float4 cand = d.planes[d.cand_pos[(size_t)row_slot[mi] * Q + q]];
         const float db = depth_from_plane(cam, cand, x, y);
         if (db >= d.prm.depth_min && db <= d.prm.depth_max && final_costs[mi] < cost_now) {
             depth_now = db;
             plane_now = cand;
+            now_row = row_slot[mi];
             cost_now = final_costs[mi];
             d.selected[p] = tsel;
         }
@@ -1306,9 +1321,15 @@ __global__ void k_strong_select(Dev d, int colour, int iter) {
     d.now_plane[q] = plane_now;
     d.now_depth[q] = depth_now;
     d.now_cost[q] = cost_now;
+    d.now_row[q] = now_row;
     d.wnorm[q] = wn;
 }
 
+// Bitwise equality of two planes.
+__device__ __forceinline__ bool same_plane(const float4 &a, const float4 &b) {
+    return __float_as_uint(a.x) == __float_as_uint(b.x) && __float_as_uint(a.y) == __float_as_uint(b.y) &&
+           __float_as_uint(a.z) == __float_as_uint(b.z) && __float_as_uint(a.w) == __float_as_uint(b.w);
+}
 // Acceptance of the refinement hypotheses and the RunState write-back
 // (DPE.cu:1115-1121, 1660-1669).
 __global__ void k_strong_accept(Dev d, int colour) {
@@ -1333,15 +1354,31 @@ __global__ void k_strong_accept(Dev d, int colour) {
             cost = tc;
         }
     }
+    const float4 cur = d.planes[p];
+    float4 fin = cur;
     if (d.prm.state == DPE_REFINE_INIT) {
         if ((double)cost < (double)d.costs[p] - 0.1) {
             d.costs[p] = cost;
             d.planes[p] = plane;
+            fin = plane;
         }
     } else {
         d.costs[p] = cost;
         d.planes[p] = plane;
+        fin = plane;
     }
+    // per-view costs of the plane the pixel now holds: the current plane's
+    // (cost_mat row 0) or the adopted candidate's (row now_row), all views;
+    // a refinement hypothesis was evaluated on the weighted views only
+    const float4 np = d.now_plane[q];
+    const int row = same_plane(fin, np) ? d.now_row[q] : same_plane(fin, cur) ? 0 : -1;
+    const int NV = d.NV;
+    if (row < 0) {
+        d.cur_valid[p] = 0u;
+        return;
+    }
+    for (int v = 0; v < NV; ++v) d.cur_cost[(size_t)v * d.P + p] = d.cost_mat[((size_t)row * NV + v) * Q + q];
+    d.cur_valid[p] = NV >= 32 ? 0xFFFFFFFFu : (1u << NV) - 1u;
 }
 
 // ------------------------------------------------------ weak-texture path
@@ -2578,21 +2615,21 @@ static inline dim3 grid_direct(int n, int nslot) { return dim3((n + 255) / 256, 
 // Every tile gives bit-identical results; only the schedule differs.
 template <bool PREC, int WAVES>
 static void full_t(const Dev &d, hipStream_t s, int colour, int Qn, int nslot, const int *cand, const float4 *src,
-                   const uint32_t *mask, float *out) {
+                   const uint32_t *mask, float *out, bool cached) {
     hipLaunchKernelGGL((k_ncc_full<6, PREC, WAVES>), dim3((Qn + kLanes - 1) / kLanes), dim3(kLanes * WAVES),
-                       d.full_tile.lds_pad, s, d, colour, Qn, nslot, cand, src, mask, out);
+                       d.full_tile.lds_pad, s, d, colour, Qn, nslot, cand, src, mask, out, cached);
 }
 static void launch_full(const Dev &d, hipStream_t s, int colour, int Qn, int nslot, const int *cand,
-                        const float4 *src, const uint32_t *mask, float *out) {
+                        const float4 *src, const uint32_t *mask, float *out, bool cached = false) {
     const int w = d.full_tile.waves;
     if (d.prec) {
-        if (w == 2) full_t<true, 2>(d, s, colour, Qn, nslot, cand, src, mask, out);
-        else if (w == 8) full_t<true, 8>(d, s, colour, Qn, nslot, cand, src, mask, out);
-        else full_t<true, 4>(d, s, colour, Qn, nslot, cand, src, mask, out);
+        if (w == 2) full_t<true, 2>(d, s, colour, Qn, nslot, cand, src, mask, out, cached);
+        else if (w == 8) full_t<true, 8>(d, s, colour, Qn, nslot, cand, src, mask, out, cached);
+        else full_t<true, 4>(d, s, colour, Qn, nslot, cand, src, mask, out, cached);
     } else {
-        if (w == 2) full_t<false, 2>(d, s, colour, Qn, nslot, cand, src, mask, out);
-        else if (w == 8) full_t<false, 8>(d, s, colour, Qn, nslot, cand, src, mask, out);
-        else full_t<false, 4>(d, s, colour, Qn, nslot, cand, src, mask, out);
+        if (w == 2) full_t<false, 2>(d, s, colour, Qn, nslot, cand, src, mask, out, cached);
+        else if (w == 8) full_t<false, 8>(d, s, colour, Qn, nslot, cand, src, mask, out, cached);
+        else full_t<false, 4>(d, s, colour, Qn, nslot, cand, src, mask, out, cached);
     }
 }
 template <int MODE, bool PREC, int WAVES>
@@ -2622,10 +2659,10 @@ void launch_ncc_current(const Dev &d, hipStream_t s) {
     // prior-selected ones (ComputeMultiViewInitialCost, DPE.cu:832-861)
     const uint32_t *mask = d.prm.state == DPE_FIRST_INIT ? nullptr : d.selected;
     if (d.ntap == 6)
-        launch_full(d, s, -1, d.P, 1, nullptr, d.planes, mask, d.cost_mat);
+        launch_full(d, s, -1, d.P, 1, nullptr, d.planes, mask, d.cur_cost);
     else
         hipLaunchKernelGGL(k_ncc_full_direct, grid_direct(d.P, 1), dim3(256), 0, s, d, -1, d.P, nullptr, d.planes,
-                           mask, d.cost_mat);
+                           mask, d.cur_cost);
 }
 void launch_random_init_costs(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_random_init_costs, grid1(d.P), dim3(256), 0, s, d);
@@ -2636,7 +2673,7 @@ void launch_strong_candidates(const Dev &d, int colour, int iter, hipStream_t s)
 void launch_ncc_strong(const Dev &d, int colour, hipStream_t s) {
     const int nslot = d.prm.use_edge ? kStrongSlots : 9;
     if (d.ntap == 6)
-        launch_full(d, s, colour, d.Q, nslot, d.cand_pos, d.planes, nullptr, d.cost_mat);
+        launch_full(d, s, colour, d.Q, nslot, d.cand_pos, d.planes, nullptr, d.cost_mat, true);
     else
         hipLaunchKernelGGL(k_ncc_full_direct, grid_direct(d.Q, nslot), dim3(256), 0, s, d, colour, d.Q, d.cand_pos,
                            d.planes, (const uint32_t *)nullptr, d.cost_mat);

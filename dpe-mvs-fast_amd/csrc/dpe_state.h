@@ -11,6 +11,9 @@
 //   costs, selected, view_weight (32 B/px), weak, fit planes, ... as DataPassHelper
 //   cand_pos   int[17][Q]              candidate pixel of every hypothesis slot
 //   cost_mat   float[17][NV][Q]        per-view NCC cost of every slot
+//   cur_cost   float[NV][P], cur_valid u32[P]   per-view NCC cost of each pixel's
+//              current plane, for the views whose bit is set (known from the
+//              evaluation that put the plane there)
 //   rplane     float4[5][Q], rcost float[5][Q]   refinement hypotheses
 #pragma once
 #include "dpe_device.h"
@@ -74,6 +77,9 @@ struct Dev {
     float4 *now_plane;
     float *now_depth;
     float *now_cost;
+    int *now_row;         // cost_mat row of now_plane (0 = the current plane)
+    float *cur_cost;      // per-view costs of the current planes (layout above)
+    uint32_t *cur_valid;  // views of cur_cost that hold the current plane's cost
     float *wnorm;
     // post-pass scratch (DepthToWeak / LocalRefine)
     float4 *post_op;

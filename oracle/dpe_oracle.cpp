@@ -86,6 +86,10 @@ struct Ctx {
     std::vector<F4> snap_planes;
     std::vector<float> snap_costs;
     std::vector<uint32_t> snap_selected;
+    // views whose NCC cost for the pixel's current plane an earlier evaluation
+    // already gave (the strong propagation's re-evaluation of the current plane
+    // counts as dead work for them; values are not reused here)
+    std::vector<uint32_t> known;
     // GenNeighbours constants computed as DPE.cu:2151-2157 does (host double math)
     float gn_cos_angle = 0, gn_sin_angle = 0, gn_threshold = 0;
     int gn_shift_range = 1;
@@ -651,6 +655,9 @@ float InitialCost(Ctx &C, int px, int py) {
                             [&](int i) { return ComputeBilateralNCCOld(C, px, py, i, plane); });
 }
 
+uint32_t all_views(const Ctx &C) { return C.N - 1 >= 32 ? 0xFFFFFFFFu : (1u << (C.N - 1)) - 1u; }
+bool same_plane(const F4 &a, const F4 &b) { return std::memcmp(&a, &b, sizeof(F4)) == 0; }
+
 // RandomInitialization, DPE.cu:1039-1067
 void RandomInitialization(Ctx &C, int center) {
     tl_family = kFamFull;
@@ -660,11 +667,13 @@ void RandomInitialization(Ctx &C, int center) {
         Rng rng(C.prm.seed, center, PH_RANDOM_INIT, C.prm.call_index);
         C.planes[center] = GenerateRandomPlaneHypothesis(cam, px, py, rng, C.prm.depth_min, C.prm.depth_max);
         C.costs[center] = InitialCostAndSelectedViews(C, px, py);
+        C.known[center] = all_views(C);
     } else {
         F4 plane = TransformNormal2RefCam(cam, C.planes[center]);
         const float depth = plane.w;
         plane.w = GetDistance2Origin(cam, px, py, depth, plane);
         C.planes[center] = plane;
+        C.known[center] = C.selected[center] & all_views(C);  // the views InitialCost evaluates
         C.costs[center] = InitialCost(C, px, py);
     }
 }
@@ -1089,7 +1098,11 @@ void CheckerboardPropagationStrong(Ctx &C, int px, int py, int iter) {
     const int mi = FindMinCostIndex(final_costs, 8);
     float cvn[32] = {2.0f};
     const F4 cur = C.planes[center];
-    CostVectorOld(C, px, py, cur, cvn);
+    for (int i = 1; i < C.N; ++i) {  // CostVectorOld
+        tl_family = isSet(C.known[center], i - 1) ? kFamDead : kFamFull;
+        cvn[i - 1] = ComputeBilateralNCCOld(C, px, py, i, cur);
+    }
+    tl_family = kFamFull;
     float cost_now = 0.0f;
     for (int i = 0; i < NV; ++i) cost_now += vw[i] * cvn[i];
     cost_now /= wnorm;
@@ -1106,6 +1119,7 @@ void CheckerboardPropagationStrong(Ctx &C, int px, int py, int iter) {
             C.selected[center] = tsel;
         }
     }
+    const F4 adopted = plane_now;  // every view of it was evaluated (cvn or cost_array[mi])
     RefinementStrong(C, &plane_now, &depth_now, &cost_now, rng, vw, wnorm, px, py);
     if (C.prm.state == DPE_REFINE_INIT) {
         if ((double)cost_now < (double)C.costs[center] - 0.1) {
@@ -1116,6 +1130,8 @@ void CheckerboardPropagationStrong(Ctx &C, int px, int py, int iter) {
         C.costs[center] = cost_now;
         C.planes[center] = plane_now;
     }
+    const F4 fin = C.planes[center];
+    C.known[center] = (same_plane(fin, adopted) || same_plane(fin, cur)) ? all_views(C) : 0u;
 }
 
 // CheckerboardPropagationWeak, DPE.cu:1672-1866
@@ -2141,6 +2157,7 @@ int setup(Ctx &C, const dpe_params *params, const dpe_inputs *in, int nthreads) 
     C.planes.assign(P, F4{0, 0, 0, 0});
     C.costs.assign(P, 0.0f);
     C.selected.assign(P, 0u);
+    C.known.assign(P, 0u);
     C.view_weight.assign(P * MAXIMG, 0);
     C.fit_planes.assign(P, F4{0, 0, 0, 0});
     C.weak.assign(P, DPE_STRONG);
