@@ -245,27 +245,32 @@ __device__ __forceinline__ float4 to_camera(const dpe_camera &cam, const float4 
 
 // Plane-dependent half of ComputeHomography (DPE.cu:487-516); the
 // plane-independent half (R_relative, t_relative) is precomputed per view.
+// The reference's build (--use_fast_math, CMakeLists.txt:26) divides as
+// x * rcp(y); here x * RN(1/y), with one correctly rounded reciprocal per plane
+// (1/d) and per focal length, instead of 19 correctly rounded quotients
+// (~290 -> ~90 instructions per window, the loop-invariant reciprocals hoisted).
 __device__ __forceinline__ void homography(const ViewConst &v, const dpe_camera &ref, const float4 &pl, float H[9]) {
+    const float iw = 1.0f / pl.w, ik0 = 1.0f / ref.K[0], ik4 = 1.0f / ref.K[4];
     float G[9];
-    G[0] = v.Rr[0] - v.tr[0] * pl.x / pl.w;
-    G[1] = v.Rr[1] - v.tr[0] * pl.y / pl.w;
-    G[2] = v.Rr[2] - v.tr[0] * pl.z / pl.w;
-    G[3] = v.Rr[3] - v.tr[1] * pl.x / pl.w;
-    G[4] = v.Rr[4] - v.tr[1] * pl.y / pl.w;
-    G[5] = v.Rr[5] - v.tr[1] * pl.z / pl.w;
-    G[6] = v.Rr[6] - v.tr[2] * pl.x / pl.w;
-    G[7] = v.Rr[7] - v.tr[2] * pl.y / pl.w;
-    G[8] = v.Rr[8] - v.tr[2] * pl.z / pl.w;
+    G[0] = v.Rr[0] - v.tr[0] * pl.x * iw;
+    G[1] = v.Rr[1] - v.tr[0] * pl.y * iw;
+    G[2] = v.Rr[2] - v.tr[0] * pl.z * iw;
+    G[3] = v.Rr[3] - v.tr[1] * pl.x * iw;
+    G[4] = v.Rr[4] - v.tr[1] * pl.y * iw;
+    G[5] = v.Rr[5] - v.tr[1] * pl.z * iw;
+    G[6] = v.Rr[6] - v.tr[2] * pl.x * iw;
+    G[7] = v.Rr[7] - v.tr[2] * pl.y * iw;
+    G[8] = v.Rr[8] - v.tr[2] * pl.z * iw;
     float t[9];
-    t[0] = G[0] / ref.K[0];
-    t[1] = G[1] / ref.K[4];
-    t[2] = -G[0] * ref.K[2] / ref.K[0] - G[1] * ref.K[5] / ref.K[4] + G[2];
-    t[3] = G[3] / ref.K[0];
-    t[4] = G[4] / ref.K[4];
-    t[5] = -G[3] * ref.K[2] / ref.K[0] - G[4] * ref.K[5] / ref.K[4] + G[5];
-    t[6] = G[6] / ref.K[0];
-    t[7] = G[7] / ref.K[4];
-    t[8] = -G[6] * ref.K[2] / ref.K[0] - G[7] * ref.K[5] / ref.K[4] + G[8];
+    t[0] = G[0] * ik0;
+    t[1] = G[1] * ik4;
+    t[2] = -G[0] * ref.K[2] * ik0 - G[1] * ref.K[5] * ik4 + G[2];
+    t[3] = G[3] * ik0;
+    t[4] = G[4] * ik4;
+    t[5] = -G[3] * ref.K[2] * ik0 - G[4] * ref.K[5] * ik4 + G[5];
+    t[6] = G[6] * ik0;
+    t[7] = G[7] * ik4;
+    t[8] = -G[6] * ref.K[2] * ik0 - G[7] * ref.K[5] * ik4 + G[8];
     H[0] = v.Ks[0] * t[0] + v.Ks[2] * t[6];
     H[1] = v.Ks[0] * t[1] + v.Ks[2] * t[7];
     H[2] = v.Ks[0] * t[2] + v.Ks[2] * t[8];
@@ -400,6 +405,20 @@ __device__ __forceinline__ float ncc_from_sums(float sum_ref, float sum_ref_ref,
     const float var_src = sum_src_src - sum_src * sum_src;
     if (var_ref < 1e-5f || var_src < 1e-5f) return 2.0f;
     const float covar = sum_ref_src - sum_ref * sum_src;
+    const float var_rs = sqrtf(var_ref * var_src);
+    return fmaxf(0.0f, fminf(2.0f, 1.0f - covar / var_rs));
+}
+// ncc_from_sums with the reference-only terms computed once per pixel:
+// inv = 1 / wsum, mean_ref = sum_ref * inv, var_ref = sum_ref_ref * inv - mean_ref^2
+// (the same operations, so the same result).
+__device__ __forceinline__ float ncc_from_ref_terms(float mean_ref, float var_ref, float inv, float sum_src,
+                                                    float sum_src_src, float sum_ref_src) {
+    sum_src *= inv;
+    sum_src_src *= inv;
+    sum_ref_src *= inv;
+    const float var_src = sum_src_src - sum_src * sum_src;
+    if (var_ref < 1e-5f || var_src < 1e-5f) return 2.0f;
+    const float covar = sum_ref_src - mean_ref * sum_src;
     const float var_rs = sqrtf(var_ref * var_src);
     return fmaxf(0.0f, fminf(2.0f, 1.0f - covar / var_rs));
 }

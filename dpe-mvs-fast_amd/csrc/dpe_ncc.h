@@ -131,6 +131,9 @@ struct SharedPatch {
     float sum_ref[kLanes], sum_ref_ref[kLanes], wsum[kLanes];
     // precision form: sums of w (ref - rc) and w (ref - rc)^2, rc = the centre texel
     float sum_refc[kLanes], sum_refc2[kLanes];
+    // the reference-only terms of ncc_from_sums: 1 / wsum, the weighted mean and
+    // variance of the reference patch
+    float inv_wsum[kLanes], mean_ref[kLanes], var_ref[kLanes];
 };
 
 // Dynamic LDS of a weighted-sum launch over N images, per wave: the round's
@@ -190,6 +193,10 @@ __device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int 
         S.wsum[lane] = s_w;
         S.sum_refc[lane] = s_c;
         S.sum_refc2[lane] = s_c2;
+        const float inv = 1.0f / s_w, mr = s_r * inv;
+        S.inv_wsum[lane] = inv;
+        S.mean_ref[lane] = mr;
+        S.var_ref[lane] = s_rr * inv - mr * mr;
     }
     __syncthreads();
 }
@@ -316,7 +323,7 @@ __device__ __forceinline__ float ncc_shared(const Dev &d, const SharedPatch<NT> 
     float s_s = 0.0f, s_ss = 0.0f, s_rs = 0.0f;
     window_sums_any<NT>(d, S, lane, x, y, v, H, refined, s_s, s_ss, s_rs);
     taps += NT * NT;
-    return ncc_from_sums(S.sum_ref[lane], S.sum_ref_ref[lane], s_s, s_ss, s_rs, S.wsum[lane]);
+    return ncc_from_ref_terms(S.mean_ref[lane], S.var_ref[lane], S.inv_wsum[lane], s_s, s_ss, s_rs);
 }
 
 

@@ -319,25 +319,28 @@ void ComputeHomography(const dpe_camera &ref, const dpe_camera &src, const F4 &p
     tr[0] = src.R[0] * Cr[0] + src.R[1] * Cr[1] + src.R[2] * Cr[2];
     tr[1] = src.R[3] * Cr[0] + src.R[4] * Cr[1] + src.R[5] * Cr[2];
     tr[2] = src.R[6] * Cr[0] + src.R[7] * Cr[1] + src.R[8] * Cr[2];
-    H[0] = Rr[0] - tr[0] * plane.x / plane.w;
-    H[1] = Rr[1] - tr[0] * plane.y / plane.w;
-    H[2] = Rr[2] - tr[0] * plane.z / plane.w;
-    H[3] = Rr[3] - tr[1] * plane.x / plane.w;
-    H[4] = Rr[4] - tr[1] * plane.y / plane.w;
-    H[5] = Rr[5] - tr[1] * plane.z / plane.w;
-    H[6] = Rr[6] - tr[2] * plane.x / plane.w;
-    H[7] = Rr[7] - tr[2] * plane.y / plane.w;
-    H[8] = Rr[8] - tr[2] * plane.z / plane.w;
+    // divisions as x * RN(1/y): the reference's --use_fast_math build
+    // (CMakeLists.txt:26) divides as x * rcp(y)
+    const float iw = 1.0f / plane.w, ik0 = 1.0f / ref.K[0], ik4 = 1.0f / ref.K[4];
+    H[0] = Rr[0] - tr[0] * plane.x * iw;
+    H[1] = Rr[1] - tr[0] * plane.y * iw;
+    H[2] = Rr[2] - tr[0] * plane.z * iw;
+    H[3] = Rr[3] - tr[1] * plane.x * iw;
+    H[4] = Rr[4] - tr[1] * plane.y * iw;
+    H[5] = Rr[5] - tr[1] * plane.z * iw;
+    H[6] = Rr[6] - tr[2] * plane.x * iw;
+    H[7] = Rr[7] - tr[2] * plane.y * iw;
+    H[8] = Rr[8] - tr[2] * plane.z * iw;
     float tmp[9];
-    tmp[0] = H[0] / ref.K[0];
-    tmp[1] = H[1] / ref.K[4];
-    tmp[2] = -H[0] * ref.K[2] / ref.K[0] - H[1] * ref.K[5] / ref.K[4] + H[2];
-    tmp[3] = H[3] / ref.K[0];
-    tmp[4] = H[4] / ref.K[4];
-    tmp[5] = -H[3] * ref.K[2] / ref.K[0] - H[4] * ref.K[5] / ref.K[4] + H[5];
-    tmp[6] = H[6] / ref.K[0];
-    tmp[7] = H[7] / ref.K[4];
-    tmp[8] = -H[6] * ref.K[2] / ref.K[0] - H[7] * ref.K[5] / ref.K[4] + H[8];
+    tmp[0] = H[0] * ik0;
+    tmp[1] = H[1] * ik4;
+    tmp[2] = -H[0] * ref.K[2] * ik0 - H[1] * ref.K[5] * ik4 + H[2];
+    tmp[3] = H[3] * ik0;
+    tmp[4] = H[4] * ik4;
+    tmp[5] = -H[3] * ref.K[2] * ik0 - H[4] * ref.K[5] * ik4 + H[5];
+    tmp[6] = H[6] * ik0;
+    tmp[7] = H[7] * ik4;
+    tmp[8] = -H[6] * ref.K[2] * ik0 - H[7] * ref.K[5] * ik4 + H[8];
     H[0] = src.K[0] * tmp[0] + src.K[2] * tmp[6];
     H[1] = src.K[0] * tmp[1] + src.K[2] * tmp[7];
     H[2] = src.K[0] * tmp[2] + src.K[2] * tmp[8];

@@ -242,9 +242,11 @@ def test_ncc_invariants(orc):
     ref = rng.integers(0, 256, (H, W)).astype(np.float32)
     planes = _fronto_planes(H, W)
     inner = (slice(6, H - 6), slice(6, W - 6))
-    # identical views: cost 0
+    # identical views: cost 0 up to rounding (the homography's divisions are
+    # x * RN(1/y), as the reference's fast-math x * rcp(y), so H is identity
+    # only to ~1 ulp and the taps land ~1e-5 px off the texels)
     c = _ncc(orc, _two_view_inputs(ref, ref.copy()), planes)[0, 0]
-    assert np.abs(c[inner]).max() < 1e-5
+    assert np.abs(c[inner]).max() < 1e-4
     # affine intensity change (a > 0): cost 0
     c = _ncc(orc, _two_view_inputs(ref, 0.5 * ref + 20.0), planes)[0, 0]
     assert np.abs(c[inner]).max() < 1e-4
