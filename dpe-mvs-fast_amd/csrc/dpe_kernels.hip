@@ -2642,14 +2642,22 @@ static void wtd_t(const Dev &d, hipStream_t s, int colour, int Qn, int nslot, fl
 }
 template <int MODE>
 static void launch_wtd(const Dev &d, hipStream_t s, int colour, int Qn, int nslot, float *out) {
-    const int w = d.wtd_tile.waves;
+    int w = d.wtd_tile.waves;
+    // The refinement's default tile is one wave per hypothesis (5): with 4 its
+    // 5 slots take two workgroups per 64 pixels, the second with one busy wave
+    // and a second patch fill (refinement launches -5 % at 1600x1200).  The
+    // post-pass keeps 4 (5 waves measured 8-14 % slower there).
+    static_assert(kRefineSlots == 5, "refinement tile");
+    if (MODE == MODE_REFINE && w == 4 && d.wtd_tile.lds_pad == 0) w = kRefineSlots;
     if (d.prec) {
         if (w == 2) wtd_t<MODE, true, 2>(d, s, colour, Qn, nslot, out);
         else if (w == 8) wtd_t<MODE, true, 8>(d, s, colour, Qn, nslot, out);
+        else if (w == 5) wtd_t<MODE, true, 5>(d, s, colour, Qn, nslot, out);
         else wtd_t<MODE, true, 4>(d, s, colour, Qn, nslot, out);
     } else {
         if (w == 2) wtd_t<MODE, false, 2>(d, s, colour, Qn, nslot, out);
         else if (w == 8) wtd_t<MODE, false, 8>(d, s, colour, Qn, nslot, out);
+        else if (w == 5) wtd_t<MODE, false, 5>(d, s, colour, Qn, nslot, out);
         else wtd_t<MODE, false, 4>(d, s, colour, Qn, nslot, out);
     }
 }
