@@ -1020,9 +1020,11 @@ dpe_status fuse_impl(dpe_ctx *ctx, const dpe_fuse_view *views, int nv, const dpe
         HIPCHK(hipMemsetAsync(acc[0], 0, P, s));
         HIPCHK(hipMemsetAsync(used[0], 0xFF, sizeof(int) * (size_t)P * std::max(ns, 1), s));
         launch_fuse_fill_claims(f, i, ns, s);
+        // A pixel reads only claims of smaller raster index, so after round k the first k pixels hold their
+        // sequential result and round P + 1 at the latest finds nothing to change; more rounds mean a bug.
         int cur = 0;
         for (int round = 0;; ++round) {
-            if (round > 4096) return fail(ctx, DPE_ERR_HIP, "fusion fixpoint did not converge");
+            if (round > P) return fail(ctx, DPE_ERR_INTERNAL, "fusion fixpoint exceeded its P + 1 round bound");
             HIPCHK(hipMemsetAsync(changed, 0, sizeof(int), s));
             launch_fuse_eval(f, i, P, used[cur], acc[cur], used[1 - cur], acc[1 - cur], changed, s);
             cur = 1 - cur;

@@ -10,9 +10,10 @@ MAX_IMAGES = 32
 NEIGHBOUR_NUM = 9
 
 # dpe_status
-OK, ERR_INVALID_ARGUMENT, ERR_TOO_MANY_IMAGES, ERR_MISSING_INPUT, ERR_HIP, ERR_NO_DEVICE, ERR_UNSUPPORTED = range(7)
+OK, ERR_INVALID_ARGUMENT, ERR_TOO_MANY_IMAGES, ERR_MISSING_INPUT, ERR_HIP, ERR_NO_DEVICE, ERR_UNSUPPORTED, ERR_INTERNAL = range(8)
 STATUS_NAMES = {0: "DPE_OK", 1: "DPE_ERR_INVALID_ARGUMENT", 2: "DPE_ERR_TOO_MANY_IMAGES",
-                3: "DPE_ERR_MISSING_INPUT", 4: "DPE_ERR_HIP", 5: "DPE_ERR_NO_DEVICE", 6: "DPE_ERR_UNSUPPORTED"}
+                3: "DPE_ERR_MISSING_INPUT", 4: "DPE_ERR_HIP", 5: "DPE_ERR_NO_DEVICE", 6: "DPE_ERR_UNSUPPORTED",
+                7: "DPE_ERR_INTERNAL"}
 # RunState (main.h:69-73), PixelState (main.h:75-79)
 FIRST_INIT, REFINE_INIT, REFINE_ITER = 0, 1, 2
 WEAK, STRONG, UNKNOWN = 0, 1, 2

@@ -34,7 +34,8 @@ typedef enum dpe_status {
     DPE_ERR_MISSING_INPUT = 3,    /* an input the params require is null: DPE.cpp:995-998 */
     DPE_ERR_HIP = 4,              /* HIP runtime error: replaces CudaSafeCall's exit, DPE.cpp:646-679 */
     DPE_ERR_NO_DEVICE = 5,
-    DPE_ERR_UNSUPPORTED = 6
+    DPE_ERR_UNSUPPORTED = 6,
+    DPE_ERR_INTERNAL = 7          /* a proven bound of the library was exceeded (e.g. fusion rounds); a bug */
 } dpe_status;
 
 /* RunState, main.h:69-73 */
