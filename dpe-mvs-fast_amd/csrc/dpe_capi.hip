@@ -82,7 +82,7 @@ enum BufId {
     B_F_VIEWS, B_F_DEPTH, B_F_NORMAL, B_F_STATES, B_F_COLOR, B_F_MASK, B_F_CLAIM, B_F_USED0, B_F_USED1, B_F_ACC0,
     B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_POSTPLANES, B_WEAKLIST, B_ADJ_WEIGHTS,
     B_ADJ_GRADS, B_QUADS_H, B_FLAG, B_WPRE, B_WSUM, B_VC64, B_WEAKCOUNTS, B_WEAKTEMP, B_NOWROW, B_CURCOST,
-    B_CURVALID, B_COUNT
+    B_CURVALID, B_LASTCAND, B_COUNT
 };
 
 struct Fail {
@@ -385,7 +385,11 @@ struct EventTimer {
     }
     // Per-family launch times and the counters of the NCC kernels into `stt`;
     // gpu_ms is left to the caller.  Needs the stream synchronised.
-    void fill(dpe_stats &stt, const unsigned long long *cnt) {
+    // cnt: the kCounterWords striped counters as the kernels left them
+    void fill(dpe_stats &stt, const unsigned long long *striped) {
+        unsigned long long cnt[2 * kNumFamilies] = {};
+        for (int k = 0; k < kCounterStripes; ++k)
+            for (int i = 0; i < 2 * kNumFamilies; ++i) cnt[i] += striped[(size_t)k * kCounterStride + i];
         for (const Span &sp : ncc) {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, ctx->events[sp.a], ctx->events[sp.b]));
@@ -405,7 +409,7 @@ struct EventTimer {
     }
 };
 
-constexpr int kNumCounters = 2 * kNumFamilies;
+constexpr int kNumCounters = kCounterWords;
 
 // The autotuner's tiles (index = dpe_stats.tile_full / tile_weighted):
 //   0: 4 waves            1: 8 waves            2: 2 waves
@@ -616,7 +620,13 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
     }
     // half-iteration and post-pass scratch
     d.cand_pos = (int *)buffer(ctx, B_CANDPOS, sizeof(int) * kStrongSlots * Q);
-    d.cost_mat = (float *)buffer(ctx, B_COSTMAT, sizeof(float) * std::max((size_t)kStrongSlots * NV * Q, (size_t)NV * P));
+    // one cost matrix per colour: a candidate repeating its slot's last plane keeps
+    // the row from the colour's previous half-iteration (k_ncc_full)
+    float *cost_mat =
+        (float *)buffer(ctx, B_COSTMAT, sizeof(float) * std::max((size_t)2 * kStrongSlots * NV * Q, (size_t)NV * P));
+    d.cost_mat = cost_mat;
+    d.last_cand = (float4 *)buffer(ctx, B_LASTCAND, sizeof(float4) * 2 * kStrongSlots * Q);
+    HIPCHK(hipMemsetAsync(d.last_cand, 0xFF, sizeof(float4) * 2 * kStrongSlots * Q, ctx->stream));
     d.nsel = (uint32_t *)buffer(ctx, B_NSEL, sizeof(uint32_t) * 4 * Q);
     d.rplane = (float4 *)buffer(ctx, B_RPLANE, sizeof(float4) * kRefineSlots * Q);
     d.rcost = (float *)buffer(ctx, B_RCOST, sizeof(float) * kRefineSlots * Q);
@@ -697,6 +707,7 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
                 d.wtd_tile = d.full_tile;
             }
             half++;
+            d.cost_mat = cost_mat + (size_t)c * kStrongSlots * NV * Q;
             launch_strong_candidates(d, c, it, s);
             tm.ncc_launch(kFamFull, [&] { launch_ncc_strong(d, c, s); }, tile);
             launch_strong_select(d, c, it, s);
@@ -743,12 +754,12 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
     copy_out(ctx, out->costs, d.costs, sizeof(float) * P, dev_out);
     copy_out(ctx, out->pixel_states, d.weak, P, dev_out);
     copy_out(ctx, out->selected_views, d.selected, sizeof(uint32_t) * P, dev_out);
-    unsigned long long cnt[kNumCounters] = {};
-    HIPCHK(hipMemcpyAsync(cnt, d.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    std::vector<unsigned long long> cnt(kNumCounters);
+    HIPCHK(hipMemcpyAsync(cnt.data(), d.counters, sizeof(unsigned long long) * kNumCounters, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     dpe_stats stt{};
     HIPCHK(hipEventElapsedTime(&stt.gpu_ms, ev_start, ev_stop));
-    tm.fill(stt, cnt);
+    tm.fill(stt, cnt.data());
     stt.quad_bytes = d.quad_fmt == QUAD_FP32 ? 16 : 8;
     stt.tile_full = stt.tile_weighted = 0;
     stt.graph = use_graph ? 1 : 0;
@@ -804,11 +815,12 @@ dpe_status ncc_adjoint_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_input
         HIPCHK(hipMemcpyAsync(costs, co, sizeof(float) * P, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipMemcpyAsync(grads, gr, sizeof(float4) * P, hipMemcpyDeviceToHost, ctx->stream));
     }
-    unsigned long long cnt[kNumCounters] = {};
-    HIPCHK(hipMemcpyAsync(cnt, d.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<unsigned long long> cnt(kNumCounters);
+    HIPCHK(hipMemcpyAsync(cnt.data(), d.counters, sizeof(unsigned long long) * kNumCounters, hipMemcpyDeviceToHost,
+                          ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     dpe_stats stt{};
-    tm.fill(stt, cnt);
+    tm.fill(stt, cnt.data());
     stt.quad_bytes = d.quad_fmt == QUAD_FP32 ? 16 : 8;
     stt.gpu_ms = stt.ncc_ms;
     ctx->stats = stt;
@@ -891,11 +903,12 @@ dpe_status ncc_cost_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *fo
     HIPCHK(hipMemsetAsync(d.counters, 0, sizeof(unsigned long long) * kNumCounters, ctx->stream));
     tm.ncc_launch(kFamFull, [&] { launch_ncc_given(d, num_hyps, pl, co, ctx->stream); });
     if (!dev) HIPCHK(hipMemcpyAsync(costs, co, sizeof(float) * P * num_hyps * d.NV, hipMemcpyDeviceToHost, ctx->stream));
-    unsigned long long cnt[kNumCounters] = {};
-    HIPCHK(hipMemcpyAsync(cnt, d.counters, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<unsigned long long> cnt(kNumCounters);
+    HIPCHK(hipMemcpyAsync(cnt.data(), d.counters, sizeof(unsigned long long) * kNumCounters, hipMemcpyDeviceToHost,
+                          ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     dpe_stats stt{};
-    tm.fill(stt, cnt);
+    tm.fill(stt, cnt.data());
     stt.quad_bytes = d.quad_fmt == QUAD_FP32 ? 16 : 8;
     stt.gpu_ms = stt.ncc_ms;
     ctx->stats = stt;

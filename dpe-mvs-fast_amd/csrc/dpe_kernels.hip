@@ -490,6 +490,36 @@ __device__ float geom_cost(const Dev &d, int x, int y, int v, const float4 &pl) 
 }
 
 // ---------------------------------------------------- NCC cost kernels (hot)
+// The wave's lanes see each other's LDS writes made before the call.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Set bits of the wave mask b in the lanes below this one.
+__device__ __forceinline__ int lanes_below(unsigned long long b) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+}
+// Bitwise equality of two planes.
+__device__ __forceinline__ bool same_plane(const float4 &a, const float4 &b) {
+    return __float_as_uint(a.x) == __float_as_uint(b.x) && __float_as_uint(a.y) == __float_as_uint(b.y) &&
+           __float_as_uint(a.z) == __float_as_uint(b.z) && __float_as_uint(a.w) == __float_as_uint(b.w);
+}
+// A candidate row (the NCC costs of one plane at one pixel over the views) is a
+// pure function of (pixel, plane, view).  last[slot * Qn + q] holds the plane of
+// the row that slot `slot` of pixel q last wrote into its colour's cost matrix
+// in this call (all-ones bits: none), so a candidate repeating it is not
+// evaluated: its row is still there.  Returns true for such a repeat and
+// records pl otherwise.
+__device__ __forceinline__ bool repeated_row(float4 *__restrict__ last, size_t at, const float4 &pl) {
+    const float4 l = last[at];
+    const bool none = (__float_as_uint(l.x) & __float_as_uint(l.y) & __float_as_uint(l.z) & __float_as_uint(l.w)) ==
+                      0xFFFFFFFFu;
+    if (!none && same_plane(l, pl)) return true;
+    last[at] = pl;
+    return false;
+}
+
 // Per-view cost matrix out[(slot*NV + v-1)*Qn + q] of the plane of every
 // (pixel, slot): plane_src[cand_pos[slot*Qn+q]] if cand_pos is given (a -1
 // entry skips the slot), else plane_src[slot*Qn + q].  With view_mask (indexed
@@ -497,14 +527,22 @@ __device__ float geom_cost(const Dev &d, int x, int y, int v, const float4 &pl) 
 // `cached` (the strong launches; slot 0 = the pixel's current plane) the views
 // of cur_valid take their cost from cur_cost instead of a window: the same
 // plane was evaluated at this pixel when it was set, so the value is the same.
+// With `last` (the strong launches), candidate slots 1.. whose plane repeats
+// the slot's last evaluated one keep their row (repeated_row).
 // One workgroup = 64 pixels x WAVES waves; wave w evaluates slots w, w+WAVES, ...
+// A wave's windows of one slot are its (pixel, view) pairs still to evaluate,
+// packed view-major onto its lanes (round r = pairs 64r..64r+63; pixel lane and
+// view from the pair list in LDS): pixels whose slot has no candidate, views
+// already known and repeated rows leave no idle lanes behind.
 template <int NT, bool PREC, int WAVES>
 __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_full(Dev d, int colour, int Qn, int nslot,
                                                               const int *__restrict__ cand_pos,
                                                               const float4 *__restrict__ plane_src,
                                                               const uint32_t *__restrict__ view_mask,
-                                                              float *__restrict__ out, bool cached) {
+                                                              float *__restrict__ out, bool cached,
+                                                              float4 *__restrict__ last) {
     __shared__ SharedPatch<NT> S;
+    extern __shared__ float2 full_lists[];
     const int lane = threadIdx.x & (kLanes - 1);
     const int wave = threadIdx.x / kLanes;
     const int q = blockIdx.x * kLanes + lane;
@@ -518,10 +556,16 @@ __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_full(Dev d, int colour, 
     }
     fill_shared_patch<NT, WAVES>(d, any_slot, x, y, S);
     unsigned long long hyps = 0, taps = 0;
+    const int NV = d.N - 1;
+    const uint32_t all = NV >= 32 ? 0xFFFFFFFFu : (1u << NV) - 1u;
+    const int p = x + y * d.W;
+    char *lds = (char *)full_lists + (size_t)wave * wtd_wave_bytes(d.N);
+    uint16_t *pairs = (uint16_t *)lds;
     for (int slot = wave; slot < nslot; slot += WAVES) {
-        bool active = any_slot;
         float4 pl = make_float4(0, 0, 0, 0);
-        if (active) {
+        uint32_t views = 0;  // bit v - 1: view v to evaluate for this lane's pixel
+        if (any_slot) {
+            bool active = true;
             if (cand_pos) {
                 const int pos = cand_pos[(size_t)slot * Qn + q];
                 active = pos >= 0;
@@ -529,22 +573,40 @@ __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_full(Dev d, int colour, 
             } else {
                 pl = plane_src[(size_t)slot * Qn + q];
             }
-        }
-        if (active) {
-            const uint32_t mask = view_mask ? view_mask[q] : 0xFFFFFFFFu;
-            const int p = x + y * d.W;
-            const uint32_t known = (cached && slot == 0) ? d.cur_valid[p] : 0u;
-            for (int v = 1; v < d.N; ++v) {
-                if (!isSet(mask, v - 1)) continue;
-                float *o = out + ((size_t)slot * d.NV + (v - 1)) * Qn + q;
-                if (isSet(known, v - 1)) {
-                    *o = d.cur_cost[(size_t)(v - 1) * d.P + p];
-                    continue;
+            if (active && !(last && slot > 0 && repeated_row(last, (size_t)slot * Qn + q, pl))) {
+                views = (view_mask ? view_mask[q] : 0xFFFFFFFFu) & all;
+                if (cached && slot == 0) {
+                    const uint32_t known = views & d.cur_valid[p];
+                    for (uint32_t k = known; k; k &= k - 1) {
+                        const int v = __builtin_ctz(k);
+                        out[((size_t)slot * NV + v) * Qn + q] = d.cur_cost[(size_t)v * d.P + p];
+                    }
+                    views &= ~known;
                 }
-                *o = ncc_shared<NT, PREC>(d, S, lane, x, y, v, pl, taps);
+            }
+        }
+        int total = 0;
+        for (int v = 0; v < NV; ++v) {
+            const bool mine = (views >> v) & 1u;
+            const unsigned long long b = __ballot(mine);
+            if (mine) pairs[total + lanes_below(b)] = (uint16_t)((lane << 5) | v);
+            total += __popcll(b);
+        }
+        wave_sync();
+        for (int base = 0; base < total; base += kLanes) {
+            const int i = base + lane;
+            const uint32_t e = i < total ? pairs[i] : 0u;
+            const int src = (int)(e >> 5), v = (int)(e & 31u) + 1;
+            const int ix = __shfl(x, src, kLanes), iy = __shfl(y, src, kLanes);
+            const float4 ipl = make_float4(__shfl(pl.x, src, kLanes), __shfl(pl.y, src, kLanes),
+                                           __shfl(pl.z, src, kLanes), __shfl(pl.w, src, kLanes));
+            if (i < total) {
+                out[((size_t)slot * NV + (v - 1)) * Qn + blockIdx.x * kLanes + src] =
+                    ncc_shared<NT, PREC>(d, S, src, ix, iy, v, ipl, taps);
                 hyps++;
             }
         }
+        wave_sync();  // the next slot rewrites the list
     }
     count_add(d.counters, kFamFull, hyps, taps);
 }
@@ -553,7 +615,7 @@ __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_full(Dev d, int colour, 
 __global__ __launch_bounds__(256) void k_ncc_full_direct(Dev d, int colour, int Qn, const int *__restrict__ cand_pos,
                                                          const float4 *__restrict__ plane_src,
                                                          const uint32_t *__restrict__ view_mask,
-                                                         float *__restrict__ out) {
+                                                         float *__restrict__ out, float4 *__restrict__ last) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int slot = blockIdx.y;
     int x = 0, y = 0;
@@ -567,6 +629,8 @@ __global__ __launch_bounds__(256) void k_ncc_full_direct(Dev d, int colour, int 
         } else {
             pl = plane_src[(size_t)slot * Qn + q];
         }
+        // a candidate repeating its slot's last plane keeps its row (k_ncc_full)
+        if (active && last && slot > 0) active = !repeated_row(last, (size_t)slot * Qn + q, pl);
     }
     unsigned long long hyps = 0, taps = 0;
     if (active) {
@@ -667,16 +731,6 @@ __device__ __forceinline__ float weighted_sum(const Dev &d, const void *Sv, int 
     return acc;
 }
 
-// The wave's lanes see each other's LDS writes made before the call.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-// Set bits of the wave mask b in the lanes below this one.
-__device__ __forceinline__ int lanes_below(unsigned long long b) {
-    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
-}
 
 // One workgroup = 64 pixels x WAVES waves; wave w evaluates slot WAVES*blockIdx.y + w.
 // The windows of a wave are its (pixel, view) pairs, packed densely: a pixel
@@ -1325,11 +1379,6 @@ __global__ void k_strong_select(Dev d, int colour, int iter) {
     d.wnorm[q] = wn;
 }
 
-// Bitwise equality of two planes.
-__device__ __forceinline__ bool same_plane(const float4 &a, const float4 &b) {
-    return __float_as_uint(a.x) == __float_as_uint(b.x) && __float_as_uint(a.y) == __float_as_uint(b.y) &&
-           __float_as_uint(a.z) == __float_as_uint(b.z) && __float_as_uint(a.w) == __float_as_uint(b.w);
-}
 // Acceptance of the refinement hypotheses and the RunState write-back
 // (DPE.cu:1115-1121, 1660-1669).
 __global__ void k_strong_accept(Dev d, int colour) {
@@ -2615,21 +2664,24 @@ static inline dim3 grid_direct(int n, int nslot) { return dim3((n + 255) / 256, 
 // Every tile gives bit-identical results; only the schedule differs.
 template <bool PREC, int WAVES>
 static void full_t(const Dev &d, hipStream_t s, int colour, int Qn, int nslot, const int *cand, const float4 *src,
-                   const uint32_t *mask, float *out, bool cached) {
-    hipLaunchKernelGGL((k_ncc_full<6, PREC, WAVES>), dim3((Qn + kLanes - 1) / kLanes), dim3(kLanes * WAVES),
-                       d.full_tile.lds_pad, s, d, colour, Qn, nslot, cand, src, mask, out, cached);
+                   const uint32_t *mask, float *out, bool cached, float4 *last) {
+    // the pair lists sit at the start of the dynamic LDS, inside the tile's pad
+    const size_t lds = std::max(wtd_list_bytes(WAVES, d.N), (size_t)d.full_tile.lds_pad);
+    hipLaunchKernelGGL((k_ncc_full<6, PREC, WAVES>), dim3((Qn + kLanes - 1) / kLanes), dim3(kLanes * WAVES), lds, s,
+                       d, colour, Qn, nslot, cand, src, mask, out, cached, last);
 }
 static void launch_full(const Dev &d, hipStream_t s, int colour, int Qn, int nslot, const int *cand,
-                        const float4 *src, const uint32_t *mask, float *out, bool cached = false) {
+                        const float4 *src, const uint32_t *mask, float *out, bool cached = false,
+                        float4 *last = nullptr) {
     const int w = d.full_tile.waves;
     if (d.prec) {
-        if (w == 2) full_t<true, 2>(d, s, colour, Qn, nslot, cand, src, mask, out, cached);
-        else if (w == 8) full_t<true, 8>(d, s, colour, Qn, nslot, cand, src, mask, out, cached);
-        else full_t<true, 4>(d, s, colour, Qn, nslot, cand, src, mask, out, cached);
+        if (w == 2) full_t<true, 2>(d, s, colour, Qn, nslot, cand, src, mask, out, cached, last);
+        else if (w == 8) full_t<true, 8>(d, s, colour, Qn, nslot, cand, src, mask, out, cached, last);
+        else full_t<true, 4>(d, s, colour, Qn, nslot, cand, src, mask, out, cached, last);
     } else {
-        if (w == 2) full_t<false, 2>(d, s, colour, Qn, nslot, cand, src, mask, out, cached);
-        else if (w == 8) full_t<false, 8>(d, s, colour, Qn, nslot, cand, src, mask, out, cached);
-        else full_t<false, 4>(d, s, colour, Qn, nslot, cand, src, mask, out, cached);
+        if (w == 2) full_t<false, 2>(d, s, colour, Qn, nslot, cand, src, mask, out, cached, last);
+        else if (w == 8) full_t<false, 8>(d, s, colour, Qn, nslot, cand, src, mask, out, cached, last);
+        else full_t<false, 4>(d, s, colour, Qn, nslot, cand, src, mask, out, cached, last);
     }
 }
 template <int MODE, bool PREC, int WAVES>
@@ -2670,7 +2722,7 @@ void launch_ncc_current(const Dev &d, hipStream_t s) {
         launch_full(d, s, -1, d.P, 1, nullptr, d.planes, mask, d.cur_cost);
     else
         hipLaunchKernelGGL(k_ncc_full_direct, grid_direct(d.P, 1), dim3(256), 0, s, d, -1, d.P, nullptr, d.planes,
-                           mask, d.cur_cost);
+                           mask, d.cur_cost, (float4 *)nullptr);
 }
 void launch_random_init_costs(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_random_init_costs, grid1(d.P), dim3(256), 0, s, d);
@@ -2680,11 +2732,12 @@ void launch_strong_candidates(const Dev &d, int colour, int iter, hipStream_t s)
 }
 void launch_ncc_strong(const Dev &d, int colour, hipStream_t s) {
     const int nslot = d.prm.use_edge ? kStrongSlots : 9;
+    float4 *last = d.last_cand + (size_t)colour * kStrongSlots * d.Q;
     if (d.ntap == 6)
-        launch_full(d, s, colour, d.Q, nslot, d.cand_pos, d.planes, nullptr, d.cost_mat, true);
+        launch_full(d, s, colour, d.Q, nslot, d.cand_pos, d.planes, nullptr, d.cost_mat, true, last);
     else
         hipLaunchKernelGGL(k_ncc_full_direct, grid_direct(d.Q, nslot), dim3(256), 0, s, d, colour, d.Q, d.cand_pos,
-                           d.planes, (const uint32_t *)nullptr, d.cost_mat);
+                           d.planes, (const uint32_t *)nullptr, d.cost_mat, last);
 }
 void launch_strong_select(const Dev &d, int colour, int iter, hipStream_t s) {
     hipLaunchKernelGGL(k_strong_select, grid1(d.Q, 64), dim3(64), 0, s, d, colour, iter);
@@ -2736,7 +2789,7 @@ void launch_ncc_given(const Dev &d, int num_hyps, const float4 *planes, float *c
         launch_full(d, s, -1, d.P, num_hyps, nullptr, planes, nullptr, costs);
     else
         hipLaunchKernelGGL(k_ncc_full_direct, grid_direct(d.P, num_hyps), dim3(256), 0, s, d, -1, d.P, nullptr, planes,
-                           (const uint32_t *)nullptr, costs);
+                           (const uint32_t *)nullptr, costs, (float4 *)nullptr);
 }
 
 // An empty launch: HIP loads this file's code object at its first kernel launch

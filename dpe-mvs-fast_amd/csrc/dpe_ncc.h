@@ -20,11 +20,13 @@ __device__ __forceinline__ bool colour_pixel(const Dev &d, int colour, int q, in
     return y < d.half_rows && x < d.W;
 }
 
-// Wave-reduced NCC counts of one kernel family; every lane of the wave must call.
+// Wave-reduced NCC counts of one kernel family into the wave's counter stripe;
+// every lane of the wave must call.
 __device__ __forceinline__ void count_add(unsigned long long *counters, int family, unsigned long long hyps,
                                           unsigned long long taps) {
     if (!counters) return;
-    unsigned long long *c = counters + 2 * family;
+    const uint32_t wave_id = (blockIdx.x + gridDim.x * blockIdx.y) * ((blockDim.x + 63) / 64) + threadIdx.x / 64;
+    unsigned long long *c = counters + (size_t)(wave_id % kCounterStripes) * kCounterStride + 2 * family;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         hyps += __shfl_down(hyps, off, 64);
@@ -61,8 +63,9 @@ __device__ __forceinline__ float texel16(uint32_t h) {
 }
 template <int FMT>
 __device__ __forceinline__ float4 quad_at(const Dev &d, int v, int ix, int iy) {
-    // 32-bit byte offsets: (W + 1) * (H + 1) * 16 < 2^32 is checked at set-up
-    const uint32_t cell = (uint32_t)(iy + 1) * (uint32_t)(d.W + 1) + (uint32_t)(ix + 1);
+    // 32-bit byte offsets: (W + 1) * (H + 1) * 16 < 2^32 is checked at set-up; both
+    // factors are below 2^24 (W <= 65536), so the full-rate 24-bit multiply serves
+    const uint32_t cell = __umul24((uint32_t)(iy + 1), (uint32_t)(d.W + 1)) + (uint32_t)(ix + 1);
     if constexpr (FMT != QUAD_FP32) {
         const uint2 h = *(const uint2 *)((const char *)(d.quads_h + (size_t)v * d.qs) + cell * 8u);
         // word 0 = (t00, t01), word 1 = (t10, t11)
@@ -72,6 +75,7 @@ __device__ __forceinline__ float4 quad_at(const Dev &d, int v, int ix, int iy) {
         return *(const float4 *)((const char *)(d.quads + (size_t)v * d.qs) + cell * 16u);
     }
 }
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 template <int FMT>
 __device__ __forceinline__ float sample_src(const Dev &d, int v, float u, float w) {
     const float fx = floorf(u), fy = floorf(w);
@@ -88,14 +92,15 @@ __device__ __forceinline__ float sample_src_any(const Dev &d, int v, float u, fl
 }
 
 // Clamp-free bilinear gather (Dev::fast_gather, a window inside the source):
-// the cell offset comes from the floored floats (exact below 2^24), and the
+// the window's taps lie in [1/64, W - 1 - 1/64] x [1/64, H - 1 - 1/64], so
+// floor is truncation, the cell index is a 24-bit multiply-add of the truncated
+// coordinates, and u - floor(u) is exact, which v_fract_f32 returns (5
+// instructions where floor, subtract and a float cell offset took 6); the
 // texel differences t.y - t.x, t.w - t.z are taken in packed fp16, exact on the
 // quarter grid, so the sample equals sample_src's bit for bit.
-typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ float sample_fast(const char *__restrict__ cell0, float row_cells, float u, float w) {
-    const float fx = floorf(u), fy = floorf(w);
-    const float a = u - fx, b = w - fy;
-    const uint32_t cell = (uint32_t)fmaf(fy, row_cells, fx);
+__device__ __forceinline__ float sample_fast(const char *__restrict__ cell0, uint32_t row_cells, float u, float w) {
+    const float a = __builtin_amdgcn_fractf(u), b = __builtin_amdgcn_fractf(w);
+    const uint32_t cell = __umul24((uint32_t)w, row_cells) + (uint32_t)u;
     const uint2 h = *(const uint2 *)(cell0 + cell * 8u);
     const half2_t p = __builtin_bit_cast(half2_t, h.x);  // (t.x, t.z)
     const half2_t q = __builtin_bit_cast(half2_t, h.y);  // (t.y, t.w)
@@ -209,7 +214,7 @@ __device__ __forceinline__ void shared_window_sums(const Dev &d, const SharedPat
                                                    int v, const float H[9], float &s_s, float &s_ss, float &s_rs) {
     // FAST: cell (0, 0) of the fp16 quads and the cells per row
     const char *cell0 = (const char *)(d.quads_h + (size_t)v * d.qs) + (size_t)(d.W + 2) * 8;
-    const float row_cells = (float)(d.W + 1);
+    const uint32_t row_cells = (uint32_t)(d.W + 1);
 #pragma unroll 1
     for (int ki = 0; ki < NT; ++ki) {
         float r_s = 0.0f, r_ss = 0.0f, r_rs = 0.0f;

@@ -10,7 +10,9 @@
 //   planes     float4[P]               (n, d) camera frame during iterations
 //   costs, selected, view_weight (32 B/px), weak, fit planes, ... as DataPassHelper
 //   cand_pos   int[17][Q]              candidate pixel of every hypothesis slot
-//   cost_mat   float[17][NV][Q]        per-view NCC cost of every slot
+//   cost_mat   float[2][17][NV][Q]     per-view NCC cost of every slot, per colour
+//              (a colour's rows stay for its next half-iteration)
+//   last_cand  float4[2][17][Q]        plane of each slot's last evaluated row
 //   cur_cost   float[NV][P], cur_valid u32[P]   per-view NCC cost of each pixel's
 //              current plane, for the views whose bit is set (known from the
 //              evaluation that put the plane there)
@@ -70,7 +72,8 @@ struct Dev {
     const int *label;
     // strong half-iteration scratch
     int *cand_pos;
-    float *cost_mat;
+    float *cost_mat;      // the launched colour's [17][NV][Q] rows
+    float4 *last_cand;    // [2][17][Q] plane of each slot's last evaluated row (all-ones: none)
     uint32_t *nsel;
     float4 *rplane;
     float *rcost;
@@ -100,7 +103,7 @@ struct Dev {
     // GenNeighbours constants (DPE.cu:2151-2157, host double math)
     float gn_cos, gn_sin, gn_thr;
     int gn_shift;
-    unsigned long long *counters;  // [2*family] NCC hyps, [2*family+1] taps; family = NccFamily
+    unsigned long long *counters;  // [stripe][2*family] NCC hyps, [..][2*family+1] taps (kCounterStripes)
     // 1: fp16 quads of quarter-grid texels and (W+1)(H+1) < 2^24, so windows
     // that project inside the source take the clamp-free gather (gather_fast)
     int fast_gather;
@@ -121,6 +124,13 @@ enum NccFamily {
     kFamWeak = 3,    // k_weak_update: weak-pixel propagation (ComputeBilateralNCCNew windows)
     kNumFamilies = 4
 };
+// The NCC counters (per family: hyps, taps) are striped: every wave adds its
+// counts to one of kCounterStripes copies, 128 B apart, and the host sums them.
+// (One copy took about two million atomic adds per post-pass launch on one
+// address, 7 ms of its 46 ms at 1600x1200.)
+constexpr int kCounterStripes = 64;
+constexpr int kCounterStride = 16;  // 64-bit words per stripe (2 * kNumFamilies used)
+constexpr int kCounterWords = kCounterStripes * kCounterStride;
 
 // Launch interface (dpe_kernels.hip); every function enqueues on `stream`.
 // Kernels named ncc_* evaluate bilateral NCC windows (the timed hot path).

@@ -90,6 +90,13 @@ struct Ctx {
     // already gave (the strong propagation's re-evaluation of the current plane
     // counts as dead work for them; values are not reused here)
     std::vector<uint32_t> known;
+    // Per candidate slot (1..16: GPU slot 1 + d of the 8 scanned candidates, 9 + d
+    // of the 8 fixed-step ones) and pixel, the plane of the slot's last evaluated
+    // NCC row in this call (all-ones bits: none).  A row is a pure function of
+    // (pixel, plane, view), so a candidate repeating that plane gives the same
+    // costs; the GPU keeps such rows from the pixel's previous half-iteration
+    // and does not evaluate them, so they count as dead work here.
+    mutable std::vector<F4> last_cand;
     // GenNeighbours constants computed as DPE.cu:2151-2157 does (host double math)
     float gn_cos_angle = 0, gn_sin_angle = 0, gn_threshold = 0;
     int gn_shift_range = 1;
@@ -661,6 +668,18 @@ float InitialCost(Ctx &C, int px, int py) {
 uint32_t all_views(const Ctx &C) { return C.N - 1 >= 32 ? 0xFFFFFFFFu : (1u << (C.N - 1)) - 1u; }
 bool same_plane(const F4 &a, const F4 &b) { return std::memcmp(&a, &b, sizeof(F4)) == 0; }
 
+constexpr int kCandSlots = 17;
+// NCC family of candidate slot `slot`'s row at `center` with plane pl (see
+// Ctx::last_cand); records pl as the slot's last evaluated plane.
+int candidate_family(const Ctx &C, int slot, int center, const F4 &pl) {
+    F4 &l = C.last_cand[(size_t)slot * C.P + center];
+    F4 none;
+    std::memset(&none, 0xFF, sizeof(F4));
+    const bool repeat = same_plane(l, pl) && !same_plane(l, none);
+    l = pl;
+    return repeat ? kFamDead : kFamFull;
+}
+
 // RandomInitialization, DPE.cu:1039-1067
 void RandomInitialization(Ctx &C, int center) {
     tl_family = kFamFull;
@@ -875,7 +894,9 @@ void StrongCandidates(const Ctx &C, int px, int py, int iter, bool flag[8], int 
             if (min_cost < FLT_MAX) {
                 flag[d] = true;
                 positions[d] = mcx + mcy * W;
+                tl_family = candidate_family(C, 1 + d, center, planes[positions[d]]);
                 CostVectorOld(C, px, py, planes[positions[d]], cost_array[d]);
+                tl_family = kFamFull;
             }
         }
         if (!edge[center]) {
@@ -908,7 +929,7 @@ void StrongCandidates(const Ctx &C, int px, int py, int iter, bool flag[8], int 
                     const int tpos = mcx + mcy * W;
                     // the pixel the adaptive scan already chose: equal costs, which
                     // never replace it below (a result the reference never uses)
-                    tl_family = (had && tpos == positions[d]) ? kFamDead : kFamFull;
+                    tl_family = (had && tpos == positions[d]) ? kFamDead : candidate_family(C, 9 + d, center, planes[tpos]);
                     CostVectorOld(C, px, py, planes[tpos], tca);
                     tl_family = kFamFull;
                     int good[2] = {0, 0}, bad[2] = {0, 0};
@@ -942,7 +963,9 @@ void StrongCandidates(const Ctx &C, int px, int py, int iter, bool flag[8], int 
                 }
             }
             up_far = costMinPoint;
+            tl_family = candidate_family(C, 2, center, planes[up_far]);
             CostVectorOld(C, px, py, planes[up_far], cost_array[1]);
+            tl_family = kFamFull;
         }
         if (py < H - 3) {
             flag[3] = true;
@@ -955,7 +978,9 @@ void StrongCandidates(const Ctx &C, int px, int py, int iter, bool flag[8], int 
                 }
             }
             down_far = costMinPoint;
+            tl_family = candidate_family(C, 4, center, planes[down_far]);
             CostVectorOld(C, px, py, planes[down_far], cost_array[3]);
+            tl_family = kFamFull;
         }
         if (px > 2) {
             flag[5] = true;
@@ -968,7 +993,9 @@ void StrongCandidates(const Ctx &C, int px, int py, int iter, bool flag[8], int 
                 }
             }
             left_far = costMinPoint;
+            tl_family = candidate_family(C, 6, center, planes[left_far]);
             CostVectorOld(C, px, py, planes[left_far], cost_array[5]);
+            tl_family = kFamFull;
         }
         if (px < W - 3) {
             flag[7] = true;
@@ -981,7 +1008,9 @@ void StrongCandidates(const Ctx &C, int px, int py, int iter, bool flag[8], int 
                 }
             }
             right_far = costMinPoint;
+            tl_family = candidate_family(C, 8, center, planes[right_far]);
             CostVectorOld(C, px, py, planes[right_far], cost_array[7]);
+            tl_family = kFamFull;
         }
         if (py > 0) {
             flag[0] = true;
@@ -998,7 +1027,9 @@ void StrongCandidates(const Ctx &C, int px, int py, int iter, bool flag[8], int 
                 }
             }
             up_near = costMinPoint;
+            tl_family = candidate_family(C, 1, center, planes[up_near]);
             CostVectorOld(C, px, py, planes[up_near], cost_array[0]);
+            tl_family = kFamFull;
         }
         if (py < H - 1) {
             flag[2] = true;
@@ -1015,7 +1046,9 @@ void StrongCandidates(const Ctx &C, int px, int py, int iter, bool flag[8], int 
                 }
             }
             down_near = costMinPoint;
+            tl_family = candidate_family(C, 3, center, planes[down_near]);
             CostVectorOld(C, px, py, planes[down_near], cost_array[2]);
+            tl_family = kFamFull;
         }
         if (px > 0) {
             flag[4] = true;
@@ -1032,7 +1065,9 @@ void StrongCandidates(const Ctx &C, int px, int py, int iter, bool flag[8], int 
                 }
             }
             left_near = costMinPoint;
+            tl_family = candidate_family(C, 5, center, planes[left_near]);
             CostVectorOld(C, px, py, planes[left_near], cost_array[4]);
+            tl_family = kFamFull;
         }
         if (px < W - 1) {
             flag[6] = true;
@@ -1049,7 +1084,9 @@ void StrongCandidates(const Ctx &C, int px, int py, int iter, bool flag[8], int 
                 }
             }
             right_near = costMinPoint;
+            tl_family = candidate_family(C, 7, center, planes[right_near]);
             CostVectorOld(C, px, py, planes[right_near], cost_array[6]);
+            tl_family = kFamFull;
         }
         positions[0] = up_near;
         positions[1] = up_far;
@@ -2161,6 +2198,11 @@ int setup(Ctx &C, const dpe_params *params, const dpe_inputs *in, int nthreads) 
     C.costs.assign(P, 0.0f);
     C.selected.assign(P, 0u);
     C.known.assign(P, 0u);
+    {
+        F4 none;
+        std::memset(&none, 0xFF, sizeof(F4));
+        C.last_cand.assign((size_t)kCandSlots * P, none);
+    }
     C.view_weight.assign(P * MAXIMG, 0);
     C.fit_planes.assign(P, F4{0, 0, 0, 0});
     C.weak.assign(P, DPE_STRONG);
