@@ -311,7 +311,7 @@ Dev setup_views(dpe_ctx *ctx, const dpe_params *prm, const dpe_inputs *in, int p
         HIPCHK(hipMemcpyAsync(&not_exact, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
         fmt = (not_exact & 1) ? QUAD_FP32 : QUAD_FP16;
-        d.fast_gather = fmt == QUAD_FP16 && !(not_exact & 2) && (size_t)(d.W + 1) * (d.H + 1) < ((size_t)1 << 24);
+        d.fast_gather = fmt == QUAD_FP16 && !(not_exact & 2);
     }
     d.quad_fmt = fmt;
     d.prec = precision;

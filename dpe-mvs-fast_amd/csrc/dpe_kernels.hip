@@ -529,11 +529,11 @@ __device__ __forceinline__ bool repeated_row(float4 *__restrict__ last, size_t a
 // plane was evaluated at this pixel when it was set, so the value is the same.
 // With `last` (the strong launches), candidate slots 1.. whose plane repeats
 // the slot's last evaluated one keep their row (repeated_row).
-// One workgroup = 64 pixels x WAVES waves; wave w evaluates slots w, w+WAVES, ...
-// A wave's windows of one slot are its (pixel, view) pairs still to evaluate,
-// packed view-major onto its lanes (round r = pairs 64r..64r+63; pixel lane and
-// view from the pair list in LDS): pixels whose slot has no candidate, views
-// already known and repeated rows leave no idle lanes behind.
+// One workgroup = 64 pixels x WAVES waves; wave w takes slots w, w+WAVES, ...
+// The wave's (slot, pixel) items that still need windows are packed onto its
+// lanes (pixels whose slot has no candidate, or repeats its row, leave no idle
+// lane), 64 items per pass; a pass loops over the views, the same view in
+// every lane, so the view's constants and quad base stay wave-uniform.
 template <int NT, bool PREC, int WAVES>
 __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_full(Dev d, int colour, int Qn, int nslot,
                                                               const int *__restrict__ cand_pos,
@@ -558,55 +558,55 @@ __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_full(Dev d, int colour, 
     unsigned long long hyps = 0, taps = 0;
     const int NV = d.N - 1;
     const uint32_t all = NV >= 32 ? 0xFFFFFFFFu : (1u << NV) - 1u;
+    const uint32_t mask = (valid && view_mask ? view_mask[q] : 0xFFFFFFFFu) & all;
     const int p = x + y * d.W;
-    char *lds = (char *)full_lists + (size_t)wave * wtd_wave_bytes(d.N);
-    uint16_t *pairs = (uint16_t *)lds;
-    for (int slot = wave; slot < nslot; slot += WAVES) {
-        float4 pl = make_float4(0, 0, 0, 0);
-        uint32_t views = 0;  // bit v - 1: view v to evaluate for this lane's pixel
+    // the wave's item list: (slot index within the wave << 6) | pixel lane; without
+    // cand_pos every valid pixel of every slot is an item and the list is implicit
+    const bool dense = cand_pos == nullptr;
+    uint16_t *items = (uint16_t *)((char *)full_lists + (size_t)wave * full_list_bytes(nslot, WAVES));
+    int total = dense ? (nslot - wave + WAVES - 1) / WAVES * kLanes : 0;
+    for (int k = 0, slot = wave; !dense && slot < nslot; ++k, slot += WAVES) {
+        bool need = false;
         if (any_slot) {
-            bool active = true;
-            if (cand_pos) {
-                const int pos = cand_pos[(size_t)slot * Qn + q];
-                active = pos >= 0;
-                if (active) pl = plane_src[pos];
-            } else {
-                pl = plane_src[(size_t)slot * Qn + q];
-            }
-            if (active && !(last && slot > 0 && repeated_row(last, (size_t)slot * Qn + q, pl))) {
-                views = (view_mask ? view_mask[q] : 0xFFFFFFFFu) & all;
-                if (cached && slot == 0) {
-                    const uint32_t known = views & d.cur_valid[p];
-                    for (uint32_t k = known; k; k &= k - 1) {
-                        const int v = __builtin_ctz(k);
+            const int pos = cand_pos ? cand_pos[(size_t)slot * Qn + q] : 0;
+            if (pos >= 0) {
+                const float4 pl = cand_pos ? plane_src[pos] : plane_src[(size_t)slot * Qn + q];
+                need = !(last && slot > 0 && repeated_row(last, (size_t)slot * Qn + q, pl));
+                if (need && cached && slot == 0) {
+                    const uint32_t known = mask & d.cur_valid[p];
+                    for (uint32_t b = known; b; b &= b - 1) {
+                        const int v = __builtin_ctz(b);
                         out[((size_t)slot * NV + v) * Qn + q] = d.cur_cost[(size_t)v * d.P + p];
                     }
-                    views &= ~known;
+                    need = (mask & ~known) != 0;
                 }
             }
         }
-        int total = 0;
-        for (int v = 0; v < NV; ++v) {
-            const bool mine = (views >> v) & 1u;
-            const unsigned long long b = __ballot(mine);
-            if (mine) pairs[total + lanes_below(b)] = (uint16_t)((lane << 5) | v);
-            total += __popcll(b);
+        const unsigned long long b = __ballot(need);
+        if (need) items[total + lanes_below(b)] = (uint16_t)((k << 6) | lane);
+        total += __popcll(b);
+    }
+    wave_sync();
+    for (int base = 0; base < total; base += kLanes) {
+        const int i = base + lane;
+        const bool act = dense ? valid : i < total;
+        const uint32_t e = dense ? (uint32_t)i : act ? items[i] : 0u;
+        const int src = (int)(e & 63u), slot = wave + WAVES * (int)(e >> 6);
+        const int ix = __shfl(x, src, kLanes), iy = __shfl(y, src, kLanes);
+        const int iq = blockIdx.x * kLanes + src;
+        uint32_t views = __shfl(mask, src, kLanes);
+        float4 pl = make_float4(0, 0, 0, 0);
+        if (act) {
+            pl = cand_pos ? plane_src[cand_pos[(size_t)slot * Qn + iq]] : plane_src[(size_t)slot * Qn + iq];
+            if (cached && slot == 0) views &= ~d.cur_valid[ix + iy * d.W];
+        } else {
+            views = 0;
         }
-        wave_sync();
-        for (int base = 0; base < total; base += kLanes) {
-            const int i = base + lane;
-            const uint32_t e = i < total ? pairs[i] : 0u;
-            const int src = (int)(e >> 5), v = (int)(e & 31u) + 1;
-            const int ix = __shfl(x, src, kLanes), iy = __shfl(y, src, kLanes);
-            const float4 ipl = make_float4(__shfl(pl.x, src, kLanes), __shfl(pl.y, src, kLanes),
-                                           __shfl(pl.z, src, kLanes), __shfl(pl.w, src, kLanes));
-            if (i < total) {
-                out[((size_t)slot * NV + (v - 1)) * Qn + blockIdx.x * kLanes + src] =
-                    ncc_shared<NT, PREC>(d, S, src, ix, iy, v, ipl, taps);
-                hyps++;
-            }
+        for (int v = 1; v < d.N; ++v) {
+            if (!isSet(views, v - 1)) continue;
+            out[((size_t)slot * NV + (v - 1)) * Qn + iq] = ncc_shared<NT, PREC>(d, S, src, ix, iy, v, pl, taps);
+            hyps++;
         }
-        wave_sync();  // the next slot rewrites the list
     }
     count_add(d.counters, kFamFull, hyps, taps);
 }
@@ -1927,7 +1927,11 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d, int cache_words) 
                 const float fit_depth = -cr.w / (cr.x * q.x + cr.y * q.y + cr.z);
                 return fabsf(fit_depth - q.z);
             };
-            for (int si = 0; si < vc; ++si) {
+            // A count below max(6, max_count) changes nothing below (no plane, no
+            // flag, no threshold), so the scan stops once the remaining points
+            // cannot lift the count to it: the same decisions, fewer residuals.
+            const int needed = max(6, max_count);
+            for (int si = 0; si < vc && temp_count + (vc - si) >= needed; ++si) {
                 const float distance = residual(si);
                 if (distance < temp_thr) {
                     temp_count++;
@@ -2110,8 +2114,10 @@ __global__ void k_ransac(Dev d, int iter) {
         normalize3(&cr);
         cr.w = -(cr.x * A.x + cr.y * A.y + cr.z * A.z);
         if (!has_strong && is_strong) has_strong = true;
+        // The partial sums of non-negative terms never decrease (and NaN stays),
+        // so once tcost < min_cost fails it fails at the end: the scan stops there.
         float tcost = 0.0f;
-        for (int si = 0; si < count; ++si) {
+        for (int si = 0; si < count && tcost < min_cost; ++si) {
             if (si == a || si == b || si == c) continue;
             const float fx = ((float)sp[si].x - cam.K[2]) / cam.K[0];
             const float fy = ((float)sp[si].y - cam.K[5]) / cam.K[4];
@@ -2665,8 +2671,9 @@ static inline dim3 grid_direct(int n, int nslot) { return dim3((n + 255) / 256, 
 template <bool PREC, int WAVES>
 static void full_t(const Dev &d, hipStream_t s, int colour, int Qn, int nslot, const int *cand, const float4 *src,
                    const uint32_t *mask, float *out, bool cached, float4 *last) {
-    // the pair lists sit at the start of the dynamic LDS, inside the tile's pad
-    const size_t lds = std::max(wtd_list_bytes(WAVES, d.N), (size_t)d.full_tile.lds_pad);
+    // the item lists (strong launches only) sit at the start of the dynamic LDS, inside the tile's pad
+    const size_t lists = cand ? (size_t)WAVES * full_list_bytes(nslot, WAVES) : 0;
+    const size_t lds = std::max(lists, (size_t)d.full_tile.lds_pad);
     hipLaunchKernelGGL((k_ncc_full<6, PREC, WAVES>), dim3((Qn + kLanes - 1) / kLanes), dim3(kLanes * WAVES), lds, s,
                        d, colour, Qn, nslot, cand, src, mask, out, cached, last);
 }

@@ -98,6 +98,8 @@ __device__ __forceinline__ float sample_src_any(const Dev &d, int v, float u, fl
 // instructions where floor, subtract and a float cell offset took 6); the
 // texel differences t.y - t.x, t.w - t.z are taken in packed fp16, exact on the
 // quarter grid, so the sample equals sample_src's bit for bit.
+// cell0 = cell (0, 0) of the view: wave-uniform where the view is (the
+// gather then takes a scalar base and a 32-bit offset per tap).
 __device__ __forceinline__ float sample_fast(const char *__restrict__ cell0, uint32_t row_cells, float u, float w) {
     const float a = __builtin_amdgcn_fractf(u), b = __builtin_amdgcn_fractf(w);
     const uint32_t cell = __umul24((uint32_t)w, row_cells) + (uint32_t)u;
@@ -149,6 +151,11 @@ __host__ __device__ constexpr size_t wtd_wave_bytes(int N) {
     return ((size_t)kLanes * sizeof(float2) + (size_t)(N - 1) * (8 + 2) + (size_t)kLanes * (N - 1) * 2 + 7) / 8 * 8;
 }
 __host__ __device__ constexpr size_t wtd_list_bytes(int waves, int N) { return (size_t)waves * wtd_wave_bytes(N); }
+// Dynamic LDS of a k_ncc_full launch per wave: one 16-bit (slot, pixel lane)
+// item per pixel of each of the wave's slots; 8-byte aligned.
+__host__ __device__ constexpr size_t full_list_bytes(int nslot, int waves) {
+    return ((size_t)(nslot + waves - 1) / waves * kLanes * 2 + 7) / 8 * 8;
+}
 static_assert(sizeof(SharedPatch<6>) + wtd_list_bytes(8, kMaxImages) <= 65536, "weighted tile exceeds 64 KiB of LDS");
 
 // WAVES = waves of the workgroup (the tile's slot width); they fill the taps together.

@@ -104,8 +104,8 @@ struct Dev {
     float gn_cos, gn_sin, gn_thr;
     int gn_shift;
     unsigned long long *counters;  // [stripe][2*family] NCC hyps, [..][2*family+1] taps (kCounterStripes)
-    // 1: fp16 quads of quarter-grid texels and (W+1)(H+1) < 2^24, so windows
-    // that project inside the source take the clamp-free gather (gather_fast)
+    // 1: fp16 quads of quarter-grid texels, so windows that project inside the
+    // source take the clamp-free gather (sample_fast)
     int fast_gather;
     // tiles of the shared-patch NCC launches: k_ncc_full (candidates, initial
     // costs) and k_ncc_weighted (refinement, post-pass)
