@@ -561,10 +561,18 @@ __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_full(Dev d, int colour, 
     const uint32_t mask = (valid && view_mask ? view_mask[q] : 0xFFFFFFFFu) & all;
     const int p = x + y * d.W;
     // the wave's item list: (slot index within the wave << 6) | pixel lane; without
-    // cand_pos every valid pixel of every slot is an item and the list is implicit
+    // cand_pos every valid pixel of every slot is an item and the list is implicit,
+    // and with fewer slots than waves every wave takes every slot and the views
+    // v with (v - 1) % WAVES == wave instead
     const bool dense = cand_pos == nullptr;
+    const bool split = dense && nslot < WAVES;
+    uint32_t wave_views = 0xFFFFFFFFu;
+    if (split) {
+        wave_views = 0u;
+        for (int v = wave; v < 32; v += WAVES) wave_views |= 1u << v;
+    }
     uint16_t *items = (uint16_t *)((char *)full_lists + (size_t)wave * full_list_bytes(nslot, WAVES));
-    int total = dense ? (nslot - wave + WAVES - 1) / WAVES * kLanes : 0;
+    int total = dense ? (split ? nslot : (nslot - wave + WAVES - 1) / WAVES) * kLanes : 0;
     for (int k = 0, slot = wave; !dense && slot < nslot; ++k, slot += WAVES) {
         bool need = false;
         if (any_slot) {
@@ -591,10 +599,10 @@ __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_full(Dev d, int colour, 
         const int i = base + lane;
         const bool act = dense ? valid : i < total;
         const uint32_t e = dense ? (uint32_t)i : act ? items[i] : 0u;
-        const int src = (int)(e & 63u), slot = wave + WAVES * (int)(e >> 6);
+        const int src = (int)(e & 63u), slot = split ? (int)(e >> 6) : wave + WAVES * (int)(e >> 6);
         const int ix = __shfl(x, src, kLanes), iy = __shfl(y, src, kLanes);
         const int iq = blockIdx.x * kLanes + src;
-        uint32_t views = __shfl(mask, src, kLanes);
+        uint32_t views = __shfl(mask, src, kLanes) & wave_views;
         float4 pl = make_float4(0, 0, 0, 0);
         if (act) {
             pl = cand_pos ? plane_src[cand_pos[(size_t)slot * Qn + iq]] : plane_src[(size_t)slot * Qn + iq];
