@@ -180,6 +180,19 @@ struct Rng {
     __device__ float uniform() { return (float)((next() >> 8) + 1u) * 5.9604644775390625e-08f; }
 };
 
+// x % d (d >= 1) with one multiply-high: for d >= 2, m = floor((2^32 - 1) / d) + 1
+// exceeds 2^32 / d by less than 1, so mulhi(x, m) is floor(x / d) or one more
+// (x (m - 2^32 / d) / 2^32 < 1) and a wrapped remainder is put back by adding d.
+// Equal to x % d for every 32-bit x and d = 2..64 (checked exhaustively).
+struct UMod {
+    uint32_t d, m;
+    __device__ explicit UMod(uint32_t d_) : d(d_), m(d_ > 1u ? 0xFFFFFFFFu / d_ + 1u : 0u) {}
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const {
+        const uint32_t r = x - __umulhi(x, m) * d;
+        return d == 1u ? 0u : (r >= d ? r + d : r);
+    }
+};
+
 // ---------------------------------------------------------------- geometry
 struct ViewConst {
     float Rr[9];  // R_relative = R_src R_ref^T   (DPE.cu:471-479)

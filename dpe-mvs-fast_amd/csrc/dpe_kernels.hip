@@ -1709,6 +1709,7 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d, int cache_words) 
     for (int i = 0; i < max_pt_num; ++i) pts[i] = make_float4(0.0f, 0.0f, 0.0f, pack_pixel(make_short2(-1, -1)));
     int odi = -1;
     int strong_point_size = 0;
+    const UMod shift_mod((uint32_t)d.gn_shift);
     const int rotate_time = d.prm.rotate_time;
     const float ransac_threshold = d.prm.ransac_threshold * depth_diff;
     bool edge_limit = false;
@@ -1745,10 +1746,10 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d, int cache_words) 
                     for (int ri = 0; ri < 4; ++ri) {
                         const uint32_t s1 = rng.next();
                         const uint32_t m1 = rng.next();
-                        const int rxs = (int)(((s1 % 2 == 0) ? 1u : 0xFFFFFFFFu) * m1 % (uint32_t)d.gn_shift);
+                        const int rxs = (int)shift_mod(((s1 % 2 == 0) ? 1u : 0xFFFFFFFFu) * m1);
                         const uint32_t s2 = rng.next();
                         const uint32_t m2 = rng.next();
-                        const int rys = (int)(((s2 % 2 == 0) ? 1u : 0xFFFFFFFFu) * m2 % (uint32_t)d.gn_shift);
+                        const int rys = (int)shift_mod(((s2 % 2 == 0) ? 1u : 0xFFFFFFFFu) * m2);
                         float2 dv = make_float2(od.x * 20 + (float)rxs, od.y * 20 + (float)rys);
                         normalize2(&dv);
                         probe[ri] = make_short2((short)(int)((float)px + dv.x * (float)radius),
@@ -1896,11 +1897,12 @@ __global__ __launch_bounds__(256) void k_gen_neighbours(Dev d, int cache_words) 
             w |= (r ? 1u : 2u) << (pb & 31);
             return r;
         };
+        const UMod pick((uint32_t)vc);
         while (iteration > 0 && max_iter > 0) {
             max_iter--;
-            const int a = (int)(rng.next() % (uint32_t)vc);
-            const int b = (int)(rng.next() % (uint32_t)vc);
-            const int c = (int)(rng.next() % (uint32_t)vc);
+            const int a = (int)pick(rng.next());
+            const int b = (int)pick(rng.next());
+            const int c = (int)pick(rng.next());
             if (a == b || b == c || a == c) continue;
             const short2 sa = unpack_pixel(pts[a].w), sb = unpack_pixel(pts[b].w), sc3 = unpack_pixel(pts[c].w);
             if (must_in_triangle && !point_in_triangle(sa, sb, sc3, px, py)) continue;
@@ -2084,10 +2086,11 @@ __global__ void k_ransac(Dev d, int iter) {
     bool has_best = false, has_strong = false;
     bool must_in_triangle = (d.prm.use_label && d.label[p] > 0 && edge_limit) ? false : true;
     uint64_t edge_table = 0;  // edge_pair_bit entries of this pixel's points
+    const UMod pick((uint32_t)count);
     while (iteration--) {
-        const int a = (int)(rng.next() % (uint32_t)count);
-        const int b = (int)(rng.next() % (uint32_t)count);
-        const int c = (int)(rng.next() % (uint32_t)count);
+        const int a = (int)pick(rng.next());
+        const int b = (int)pick(rng.next());
+        const int c = (int)pick(rng.next());
         if (a == b || b == c || a == c) continue;
         bool is_strong = false;
         if (d.prm.geom_consistency && edge_limit) {
