@@ -563,7 +563,7 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
     d.weak_colour_count[1] = cc[1];
     {
         const size_t n = (size_t)cc[0] + cc[1] + 1;
-        float2 *wpre = (float2 *)buffer(ctx, B_WPRE, sizeof(float2) * kWeakPatchTaps * n);
+        float *wpre = (float *)buffer(ctx, B_WPRE, sizeof(float) * kWeakPatchTaps * n);
         float *wsum = (float *)buffer(ctx, B_WSUM, sizeof(float) * 3 * kNeighbourNum * n);
         d.weak_wpre[0] = wpre;
         d.weak_wpre[1] = wpre + (size_t)kWeakPatchTaps * cc[0];

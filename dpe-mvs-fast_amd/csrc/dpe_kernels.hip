@@ -198,7 +198,7 @@ __device__ __forceinline__ void weak_window_shape(const Dev &d, int center, int 
 // kernel at 4 waves, measured slower).
 template <bool REFINED, int FMT, int NTW = 0>
 __device__ __forceinline__ float ncc_window_pre(const Dev &d, int v, const float H[9], int cx, int cy, int radius,
-                                                int inc, const float2 *__restrict__ tw, int n,
+                                                int inc, const float *__restrict__ tw, int n,
                                                 const float *__restrict__ ws, unsigned long long &taps) {
     float sum_src = 0.0f, sum_src_src = 0.0f, sum_ref_src = 0.0f;
     int t = 0;
@@ -211,7 +211,8 @@ __device__ __forceinline__ float ncc_window_pre(const Dev &d, int v, const float
                 const int rx = cx - radius + ii * inc, ry = cy - radius + jj * inc;
                 const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
                 const float src = sample_src<FMT>(d, v, sp.x, sp.y);
-                const float2 e = tw[(size_t)(ii * NTW + jj) * n];
+                const float wt = tw[(size_t)(ii * NTW + jj) * n];
+                const float2 e = make_float2(wt, wt * texel(d.ref, d.W, d.H, rx, ry));
                 r_src = fmaf(e.x, src, r_src);
                 r_ss = fmaf(e.x * src, src, r_ss);
                 r_rs = fmaf(e.y, src, r_rs);
@@ -229,7 +230,8 @@ __device__ __forceinline__ float ncc_window_pre(const Dev &d, int v, const float
             const int rx = cx + i, ry = cy + j;
             const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
             const float src = sample_src<FMT>(d, v, sp.x, sp.y);
-            const float2 e = tw[(size_t)t * n];
+            const float wt = tw[(size_t)t * n];
+            const float2 e = make_float2(wt, wt * texel(d.ref, d.W, d.H, rx, ry));
             r_src = fmaf(e.x, src, r_src);
             r_ss = fmaf(e.x * src, src, r_ss);
             r_rs = fmaf(e.y, src, r_rs);
@@ -246,7 +248,7 @@ __device__ __forceinline__ float ncc_window_pre(const Dev &d, int v, const float
 // ncc_window_pre unrolled for 3x3 windows (the neighbour windows, radius 5 step 5)
 template <bool REFINED, int FMT>
 __device__ __forceinline__ float ncc_window_pre_unrolled(const Dev &d, int v, const float H[9], int cx, int cy, int radius,
-                                                     int inc, const float2 *__restrict__ tw, int n,
+                                                     int inc, const float *__restrict__ tw, int n,
                                                      const float *__restrict__ ws, unsigned long long &taps) {
     const int nt = 2 * radius / inc + 1;
     if (nt == 3) return ncc_window_pre<REFINED, FMT, 3>(d, v, H, cx, cy, radius, inc, tw, n, ws, taps);
@@ -254,7 +256,7 @@ __device__ __forceinline__ float ncc_window_pre_unrolled(const Dev &d, int v, co
 }
 
 __device__ float ncc_window_pre_any(const Dev &d, int v, const float H[9], int cx, int cy, int radius, int inc,
-                                    const float2 *__restrict__ tw, int n, const float *__restrict__ ws,
+                                    const float *__restrict__ tw, int n, const float *__restrict__ ws,
                                     unsigned long long &taps) {
     const int last = -radius + ((2 * radius) / inc) * inc;
     const bool refined = refined_division_ok(H, cx - radius, cx + last, cy - radius, cy + last);
@@ -360,7 +362,7 @@ __device__ float ncc_new_pre(const Dev &d, int x, int y, int v, const float4 &pl
     if (pt.x >= (float)vc.width || pt.x < 0.0f || pt.y >= (float)vc.height || pt.y < 0.0f) return cost_max;
     float center_cost = 0.0f, strong_cost = 0.0f;
     int strong_count = 0;
-    const float2 *tw = d.weak_wpre[colour] + i;
+    const float *tw = d.weak_wpre[colour] + i;
     for (int k = 0; k < kNeighbourNum; ++k) {
         const short2 np = neighbour_point(d, center, k);
         if (np.x == -1 || np.y == -1) continue;
@@ -402,7 +404,7 @@ __global__ __launch_bounds__(256) void k_weak_patch(Dev d, int colour) {
     if (i >= n) return;
     const int p = d.weak_colour_list[colour][i];
     const int x = p % d.W, y = p / d.W;
-    float2 *tw = d.weak_wpre[colour] + i;
+    float *tw = d.weak_wpre[colour] + i;
     float *ws = d.weak_wsum[colour] + i;
     if (d.weak[p] != DPE_WEAK) return;
     const float ss = d.prm.sigma_spatial, sc = d.prm.sigma_color;
@@ -431,7 +433,7 @@ __global__ __launch_bounds__(256) void k_weak_patch(Dev d, int colour) {
                 r_ref = fmaf(w, ref, r_ref);
                 r_rr = fmaf(wr, ref, r_rr);
                 r_w = r_w + w;
-                tw[(size_t)t * n] = make_float2(w, wr);
+                tw[(size_t)t * n] = w;
                 t++;
             }
             sum_ref += r_ref;

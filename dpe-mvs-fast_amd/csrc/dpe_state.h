@@ -98,7 +98,7 @@ struct Dev {
     const int *weak_colour_list[2];
     int weak_colour_count[2];
     // reference halves of the weak windows per colour-list entry (k_weak_patch)
-    float2 *weak_wpre[2];   // [kWeakPatchTaps][count]
+    float *weak_wpre[2];    // [kWeakPatchTaps][count] bilateral weight of every tap
     float *weak_wsum[2];    // [3 * kNeighbourNum][count]
     // GenNeighbours constants (DPE.cu:2151-2157, host double math)
     float gn_cos, gn_sin, gn_thr;
