@@ -97,6 +97,8 @@ struct Ctx {
     // costs; the GPU keeps such rows from the pixel's previous half-iteration
     // and does not evaluate them, so they count as dead work here.
     mutable std::vector<F4> last_cand;
+    // oracle_process_fp64: the multi-view cost is evaluated in double
+    bool cost64 = false;
     // GenNeighbours constants computed as DPE.cu:2151-2157 does (host double math)
     float gn_cos_angle = 0, gn_sin_angle = 0, gn_threshold = 0;
     int gn_shift_range = 1;
@@ -476,8 +478,14 @@ float BilateralNCCWindow(const Ctx &C, int src_idx, const float *H, int cx, int 
     return std::fmax(0.0f, std::fmin(2.0f, 1.0f - covar / var_rs));
 }
 
+float ComputeBilateralNCCOld64(const Ctx &C, int px, int py, int src_idx, const F4 &plane);
+float ComputeBilateralNCCNew64(const Ctx &C, int px, int py, int src_idx, const F4 &plane,
+                               const std::vector<uint32_t> &selected);
+float ComputeGeomConsistencyCost64(const Ctx &C, int px, int py, int src_idx, const F4 &plane);
+
 // ComputeBilateralNCCOld, DPE.cu:696-782
 float ComputeBilateralNCCOld(const Ctx &C, int px, int py, int src_idx, const F4 &plane) {
+    if (C.cost64) return ComputeBilateralNCCOld64(C, px, py, src_idx, plane);
     const dpe_camera &src = C.cams[src_idx];
     float H[9];
     ComputeHomography(C.cams[0], src, plane, H);
@@ -495,6 +503,7 @@ inline S2 GetNeighbourPoint(const Ctx &C, int center, int index) {
 // ComputeBilateralNCCNew, DPE.cu:561-694 (WEAK centre pixels only)
 float ComputeBilateralNCCNew(const Ctx &C, int px, int py, int src_idx, const F4 &plane,
                              const std::vector<uint32_t> &selected) {
+    if (C.cost64) return ComputeBilateralNCCNew64(C, px, py, src_idx, plane, selected);
     const dpe_camera &src = C.cams[src_idx];
     const int center = px + py * C.W;
     const float cost_max = 2.0f;
@@ -585,6 +594,7 @@ void ProjectonCamera_cu(const F3 &X, const dpe_camera &cam, F2 &pt, float &depth
 
 // ComputeGeomConsistencyCost, DPE.cu:919-957 (depth lookup at truncated coords)
 float ComputeGeomConsistencyCost(const Ctx &C, int px, int py, int src_idx, const F4 &plane) {
+    if (C.cost64) return ComputeGeomConsistencyCost64(C, px, py, src_idx, plane);
     const dpe_camera &ref = C.cams[0];
     const dpe_camera &src = C.cams[src_idx];
     const float max_cost = 3.0f;
@@ -604,6 +614,215 @@ float ComputeGeomConsistencyCost(const Ctx &C, int px, int py, int src_idx, cons
     const float dr = (float)py - bp.y;
     const float cc = std::sqrt(dc * dc + dr * dr);
     return std::fmin(max_cost, cc);
+}
+
+// ------------------------------------------------------------ FP64 cost path
+// oracle_process_fp64 runs the same PatchMatch with the multi-view cost --
+// homography, projection, bilinear sample, bilateral weights, window sums, NCC
+// and the geometric-consistency cost -- evaluated in double from the fp32
+// inputs (images, cameras and the fp32 plane hypotheses), with exact divisions
+// and square roots; a cost is rounded to fp32 where the algorithm stores it.
+// States, random streams and decision rules are the fp32 oracle's.  It is the
+// reference the precision form's depth error is measured against.
+
+// ComputeHomography (DPE.cu:457-517) in double.
+void ComputeHomography64(const dpe_camera &ref, const dpe_camera &src, const F4 &plane, double *H) {
+    double rR[9], sR[9], rt[3], st[3], rK[9], sK[9];
+    for (int i = 0; i < 9; ++i) {
+        rR[i] = ref.R[i];
+        sR[i] = src.R[i];
+        rK[i] = ref.K[i];
+        sK[i] = src.K[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        rt[i] = ref.t[i];
+        st[i] = src.t[i];
+    }
+    double ref_C[3], src_C[3];
+    for (int i = 0; i < 3; ++i) {
+        ref_C[i] = -(rR[i] * rt[0] + rR[3 + i] * rt[1] + rR[6 + i] * rt[2]);
+        src_C[i] = -(sR[i] * st[0] + sR[3 + i] * st[1] + sR[6 + i] * st[2]);
+    }
+    double Rr[9], tr[3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            Rr[3 * r + c] = sR[3 * r] * rR[3 * c] + sR[3 * r + 1] * rR[3 * c + 1] + sR[3 * r + 2] * rR[3 * c + 2];
+    for (int r = 0; r < 3; ++r)
+        tr[r] = sR[3 * r] * (ref_C[0] - src_C[0]) + sR[3 * r + 1] * (ref_C[1] - src_C[1]) +
+                sR[3 * r + 2] * (ref_C[2] - src_C[2]);
+    const double n[3] = {plane.x, plane.y, plane.z}, dist = plane.w;
+    double M[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) M[3 * r + c] = Rr[3 * r + c] - tr[r] * n[c] / dist;
+    // M K_ref^-1 (K_ref = [[fx 0 cx] [0 fy cy] [0 0 1]])
+    double T[9];
+    for (int r = 0; r < 3; ++r) {
+        T[3 * r] = M[3 * r] / rK[0];
+        T[3 * r + 1] = M[3 * r + 1] / rK[4];
+        T[3 * r + 2] = -M[3 * r] * rK[2] / rK[0] - M[3 * r + 1] * rK[5] / rK[4] + M[3 * r + 2];
+    }
+    for (int c = 0; c < 3; ++c) {
+        H[c] = sK[0] * T[c] + sK[2] * T[6 + c];
+        H[3 + c] = sK[4] * T[3 + c] + sK[5] * T[6 + c];
+        H[6 + c] = sK[8] * T[6 + c];
+    }
+}
+
+inline void Project64(const double *H, double x, double y, double &u, double &v) {
+    const double w = H[6] * x + H[7] * y + H[8];
+    u = (H[0] * x + H[1] * y + H[2]) / w;
+    v = (H[3] * x + H[4] * y + H[5]) / w;
+}
+
+inline int d2i_clamp(double f, int lo, int hi) {
+    if (!(f >= (double)lo)) return lo;
+    if (f > (double)hi) return hi;
+    return (int)f;
+}
+
+// bilinear() in double: same texel addressing (clamp to edge), exact weights.
+inline double bilinear64(const float *im, int W, int H, double u, double v) {
+    const double fx = std::floor(u), fy = std::floor(v);
+    const double a = u - fx, b = v - fy;
+    const int ix = d2i_clamp(fx, -1, W - 1), iy = d2i_clamp(fy, -1, H - 1);
+    const double t00 = texel(im, W, H, ix, iy), t10 = texel(im, W, H, ix + 1, iy);
+    const double t01 = texel(im, W, H, ix, iy + 1), t11 = texel(im, W, H, ix + 1, iy + 1);
+    const double top = t00 + a * (t10 - t00), bot = t01 + a * (t11 - t01);
+    return top + b * (bot - top);
+}
+
+// BilateralNCCWindow in double.
+double BilateralNCCWindow64(const Ctx &C, int src_idx, const double *H, int cx, int cy, double ref_center_pix,
+                            int radius, int increment) {
+    const dpe_camera &src = C.cams[src_idx];
+    const float *refim = C.img[0];
+    const float *srcim = C.img[src_idx];
+    const double ss = C.prm.sigma_spatial, sc = C.prm.sigma_color;
+    double s_r = 0, s_rr = 0, s_s = 0, s_ss = 0, s_rs = 0, s_w = 0;
+    int taps = 0;
+    for (int i = -radius; i <= radius; i += increment) {
+        for (int j = -radius; j <= radius; j += increment) {
+            const int rx = cx + i, ry = cy + j;
+            const double r = texel(refim, C.W, C.H, rx, ry);
+            double u, v;
+            Project64(H, rx, ry, u, v);
+            const double s = bilinear64(srcim, src.width, src.height, u, v);
+            const double w = o_exp(-std::sqrt((double)(i * i + j * j)) / (2.0 * ss * ss) -
+                                   std::fabs(r - ref_center_pix) / (2.0 * sc * sc));
+            s_r += w * r;
+            s_rr += w * r * r;
+            s_s += w * s;
+            s_ss += w * s * s;
+            s_rs += w * r * s;
+            s_w += w;
+            ++taps;
+        }
+    }
+    const_cast<Ctx &>(C).tally[tl_worker].taps[tl_family] += taps;
+    s_r /= s_w;
+    s_rr /= s_w;
+    s_s /= s_w;
+    s_ss /= s_w;
+    s_rs /= s_w;
+    const double var_ref = s_rr - s_r * s_r, var_src = s_ss - s_s * s_s;
+    if (var_ref < 1e-5 || var_src < 1e-5) return 2.0;
+    const double covar = s_rs - s_r * s_s;
+    return std::fmax(0.0, std::fmin(2.0, 1.0 - covar / std::sqrt(var_ref * var_src)));
+}
+
+float ComputeBilateralNCCOld64(const Ctx &C, int px, int py, int src_idx, const F4 &plane) {
+    const dpe_camera &src = C.cams[src_idx];
+    double H[9];
+    ComputeHomography64(C.cams[0], src, plane, H);
+    const_cast<Ctx &>(C).tally[tl_worker].hyps[tl_family]++;
+    double u, v;
+    Project64(H, px, py, u, v);
+    if (u >= src.width || u < 0.0 || v >= src.height || v < 0.0) return 2.0f;
+    const double rc = texel(C.img[0], C.W, C.H, px, py);
+    return (float)BilateralNCCWindow64(C, src_idx, H, px, py, rc, C.prm.strong_radius, C.prm.strong_increment);
+}
+
+float ComputeBilateralNCCNew64(const Ctx &C, int px, int py, int src_idx, const F4 &plane,
+                               const std::vector<uint32_t> &selected) {
+    const dpe_camera &src = C.cams[src_idx];
+    const int center = px + py * C.W;
+    const double cost_max = 2.0;
+    double H[9];
+    ComputeHomography64(C.cams[0], src, plane, H);
+    const_cast<Ctx &>(C).tally[tl_worker].hyps[tl_family]++;
+    double u, v;
+    Project64(H, px, py, u, v);
+    if (u >= src.width || u < 0.0 || v >= src.height || v < 0.0) return 2.0f;
+    if (C.weak[center] != DPE_WEAK) return 0.0f;
+    const double rc = texel(C.img[0], C.W, C.H, px, py);
+    double center_cost = 0.0, strong_cost = 0.0;
+    int strong_count = 0;
+    for (int k = 0; k < NNUM; ++k) {
+        const S2 np = GetNeighbourPoint(C, center, k);
+        if (np.x == -1 || np.y == -1) continue;
+        double nu, nv;
+        Project64(H, np.x, np.y, nu, nv);
+        if (nu < 0 || nv < 0 || nu >= C.W || nv >= C.H) {
+            if (k == 0) return 2.0f;
+            if (isSet(selected[np.x + np.y * C.W], src_idx - 1)) {
+                strong_cost += cost_max;
+                strong_count++;
+            }
+            continue;
+        }
+        int radius = (k == 0 ? C.prm.strong_radius : C.prm.weak_radius);
+        int increment = (k == 0 ? C.prm.strong_increment : C.prm.weak_increment);
+        if (C.prm.use_radius && k == 0) {
+            radius = C.radius[center];
+            increment = std::max(2, (int)(2.0 * radius / 5.0));
+        }
+        const double tc = BilateralNCCWindow64(C, src_idx, H, np.x, np.y, rc, radius, increment);
+        if (k == 0) {
+            center_cost = tc;
+        } else {
+            strong_cost += tc;
+            strong_count++;
+        }
+    }
+    if (strong_count == 0) return (float)center_cost;
+    strong_cost = std::min(strong_cost / strong_count, cost_max);
+    return (float)(0.25 * center_cost + 0.75 * strong_cost);
+}
+
+// ComputeGeomConsistencyCost (DPE.cu:919-957) in double; the source depth is
+// looked up at the truncated projection as in the fp32 path.
+float ComputeGeomConsistencyCost64(const Ctx &C, int px, int py, int src_idx, const F4 &plane) {
+    const dpe_camera &ref = C.cams[0];
+    const dpe_camera &src = C.cams[src_idx];
+    auto world = [](double x, double y, double depth, const dpe_camera &cam, double *X) {
+        const double cx = depth * (x - cam.K[2]) / cam.K[0], cy = depth * (y - cam.K[5]) / cam.K[4];
+        for (int i = 0; i < 3; ++i)
+            X[i] = (double)cam.R[i] * cx + (double)cam.R[3 + i] * cy + (double)cam.R[6 + i] * depth + cam.c[i];
+    };
+    auto project = [](const double *X, const dpe_camera &cam, double &u, double &v) {
+        double t[3];
+        for (int i = 0; i < 3; ++i)
+            t[i] = (double)cam.R[3 * i] * X[0] + (double)cam.R[3 * i + 1] * X[1] + (double)cam.R[3 * i + 2] * X[2] +
+                   cam.t[i];
+        const double z = (double)cam.K[6] * t[0] + (double)cam.K[7] * t[1] + (double)cam.K[8] * t[2];
+        u = ((double)cam.K[0] * t[0] + (double)cam.K[1] * t[1] + (double)cam.K[2] * t[2]) / z;
+        v = ((double)cam.K[3] * t[0] + (double)cam.K[4] * t[1] + (double)cam.K[5] * t[2]) / z;
+    };
+    const double depth = -(double)plane.w * ref.K[0] /
+                         ((px - (double)ref.K[2]) * plane.x + ((double)ref.K[0] / ref.K[4]) * (py - (double)ref.K[5]) *
+                                                                  plane.y +
+                          (double)ref.K[0] * plane.z);
+    double X[3], u, v;
+    world(px, py, depth, ref, X);
+    project(X, src, u, v);
+    const int ix = d2i_clamp(u, -1, C.W), iy = d2i_clamp(v, -1, C.H);
+    const double src_depth = texel(C.gdepth[src_idx], C.W, C.H, ix, iy);
+    if (src_depth == 0.0) return 3.0f;
+    double S[3], bu, bv;
+    world(u, v, src_depth, src, S);
+    project(S, ref, bu, bv);
+    const double dc = px - bu, dr = py - bv;
+    return (float)std::fmin(3.0, std::sqrt(dc * dc + dr * dr));
 }
 
 // ComputeMultiViewInitialCostandSelectedViews, DPE.cu:784-830
@@ -2359,16 +2578,11 @@ void fuse_proj(const FV3<T> &X, const dpe_camera &cam, FV2<T> &pt, T &depth) {
     pt.x = ((T)K[0] * t.x + (T)K[1] * t.y + (T)K[2] * t.z) / depth;
     pt.y = ((T)K[3] * t.x + (T)K[4] * t.y + (T)K[5] * t.z) / depth;
 }
-}  // namespace
-}  // namespace oracle
 
-using namespace oracle;
-
-extern "C" {
-
-int oracle_process(const dpe_params *params, const dpe_inputs *in, dpe_outputs *out, int nthreads,
-                   int64_t *counters) {
+int process_impl(const dpe_params *params, const dpe_inputs *in, dpe_outputs *out, int nthreads, int64_t *counters,
+                 bool cost64) {
     Ctx C;
+    C.cost64 = cost64;
     const int st = setup(C, params, in, nthreads < 1 ? 1 : nthreads);
     if (st != DPE_OK) return st;
     if (!out || !out->planes || !out->costs || !out->pixel_states || !out->selected_views)
@@ -2399,14 +2613,33 @@ int oracle_process(const dpe_params *params, const dpe_inputs *in, dpe_outputs *
     }
     return DPE_OK;
 }
+}  // namespace
+}  // namespace oracle
+
+using namespace oracle;
+
+extern "C" {
+
+// One process call (DPE.cu:3130-3262) on the fp32 oracle.
+int oracle_process(const dpe_params *params, const dpe_inputs *in, dpe_outputs *out, int nthreads,
+                   int64_t *counters) {
+    return process_impl(params, in, out, nthreads, counters, false);
+}
+
+// The same call with the multi-view cost evaluated in double (FP64 cost path).
+int oracle_process_fp64(const dpe_params *params, const dpe_inputs *in, dpe_outputs *out, int nthreads,
+                        int64_t *counters) {
+    return process_impl(params, in, out, nthreads, counters, true);
+}
 
 // ComputeMultiViewCostVectorOld over given camera-frame planes (test access).
-int oracle_ncc_cost(const dpe_params *params, const dpe_inputs *in, int num_hyps, const float *planes, float *costs,
-                    int nthreads) {
+static int ncc_cost_impl(const dpe_params *params, const dpe_inputs *in, int num_hyps, const float *planes,
+                         float *costs, int nthreads, bool cost64) {
     dpe_params p = *params;
     p.use_APD = 0; p.use_edge = 0; p.use_limit = 0; p.use_label = 0; p.geom_consistency = 0;
     p.state = DPE_FIRST_INIT;
     Ctx C;
+    C.cost64 = cost64;
     const int st = setup(C, &p, in, nthreads < 1 ? 1 : nthreads);
     if (st != DPE_OK) return st;
     const int NV = C.N - 1;
@@ -2421,6 +2654,17 @@ int oracle_ncc_cost(const dpe_params *params, const dpe_inputs *in, int num_hyps
         for (int v = 0; v < NV; ++v) costs[((size_t)h * NV + v) * P + i] = cv[v];
     });
     return DPE_OK;
+}
+
+int oracle_ncc_cost(const dpe_params *params, const dpe_inputs *in, int num_hyps, const float *planes, float *costs,
+                    int nthreads) {
+    return ncc_cost_impl(params, in, num_hyps, planes, costs, nthreads, false);
+}
+
+// The same cost vectors through the FP64 cost path.
+int oracle_ncc_cost_fp64(const dpe_params *params, const dpe_inputs *in, int num_hyps, const float *planes,
+                         float *costs, int nthreads) {
+    return ncc_cost_impl(params, in, num_hyps, planes, costs, nthreads, true);
 }
 
 }  // extern "C"
@@ -2703,9 +2947,10 @@ void oracle_gen_edge_inform(int W, int H, const uint8_t *edge, const int32_t *la
 // ---- test access to single functions of the restatement (each checked by
 // tests/test_oracle_pins.py against closed forms, brute force or invariants)
 
-float oracle_geom_cost(const dpe_camera *ref, const dpe_camera *src, int W, int H, const float *src_depth, int px,
-                       int py, const float *plane) {
+static float geom_cost_impl(const dpe_camera *ref, const dpe_camera *src, int W, int H, const float *src_depth,
+                            int px, int py, const float *plane, bool cost64) {
     Ctx C;
+    C.cost64 = cost64;
     C.W = W;
     C.H = H;
     C.P = W * H;
@@ -2715,6 +2960,14 @@ float oracle_geom_cost(const dpe_camera *ref, const dpe_camera *src, int W, int 
     C.gdepth[0] = nullptr;
     C.gdepth[1] = src_depth;
     return ComputeGeomConsistencyCost(C, px, py, 1, F4{plane[0], plane[1], plane[2], plane[3]});
+}
+float oracle_geom_cost(const dpe_camera *ref, const dpe_camera *src, int W, int H, const float *src_depth, int px,
+                       int py, const float *plane) {
+    return geom_cost_impl(ref, src, W, H, src_depth, px, py, plane, false);
+}
+float oracle_geom_cost_fp64(const dpe_camera *ref, const dpe_camera *src, int W, int H, const float *src_depth, int px,
+                            int py, const float *plane) {
+    return geom_cost_impl(ref, src, W, H, src_depth, px, py, plane, true);
 }
 
 float oracle_initial_cost(const float *view_costs, int num_images, uint32_t *selected) {

@@ -13,9 +13,17 @@ extern "C" {
  * reads (the GPU library skips them). */
 int oracle_process(const dpe_params *params, const dpe_inputs *in, dpe_outputs *out, int nthreads,
                    int64_t *counters);
+/* The same call with the multi-view cost (homography, projection, bilinear sample, bilateral weights, window
+ * sums, NCC, geometric-consistency cost) evaluated in double from the fp32 inputs and plane hypotheses; states,
+ * random streams and decision rules as in oracle_process.  The reference of the precision form's depth error. */
+int oracle_process_fp64(const dpe_params *params, const dpe_inputs *in, dpe_outputs *out, int nthreads,
+                        int64_t *counters);
 /* ComputeMultiViewCostVectorOld for given camera-frame planes (layout of dpe_ncc_cost). */
 int oracle_ncc_cost(const dpe_params *params, const dpe_inputs *in, int num_hyps, const float *planes,
                     float *costs, int nthreads);
+/* oracle_ncc_cost through the FP64 cost path. */
+int oracle_ncc_cost_fp64(const dpe_params *params, const dpe_inputs *in, int num_hyps, const float *planes,
+                         float *costs, int nthreads);
 /* RunFusion, DPE.cpp:1387-1548. */
 int oracle_fuse(const dpe_fuse_view *views, int num_views, const dpe_fuse_params *fp, dpe_point *points,
                 int64_t capacity, int64_t *num_points);
@@ -53,6 +61,9 @@ uint8_t oracle_depth_to_weak_state(const float *costs61, int weak_peak_radius);
  * source camera and its W x H depth map */
 float oracle_geom_cost(const dpe_camera *ref, const dpe_camera *src, int W, int H, const float *src_depth, int px,
                        int py, const float *plane);
+/* oracle_geom_cost through the FP64 cost path. */
+float oracle_geom_cost_fp64(const dpe_camera *ref, const dpe_camera *src, int W, int H, const float *src_depth,
+                            int px, int py, const float *plane);
 /* the view loop of ComputeMultiViewInitialCost (DPE.cu:832-861) over given costs of source views 1..num_images-1;
  * *selected is the prior mask in, the updated mask out; returns the cost */
 float oracle_initial_cost(const float *view_costs, int num_images, uint32_t *selected);

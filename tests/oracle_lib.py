@@ -24,8 +24,12 @@ def load_oracle():
     P = C.POINTER
     lib.oracle_process.argtypes = [P(abi.Params), P(abi.Inputs), P(abi.Outputs), C.c_int, P(C.c_int64)]
     lib.oracle_process.restype = C.c_int
+    lib.oracle_process_fp64.argtypes = [P(abi.Params), P(abi.Inputs), P(abi.Outputs), C.c_int, P(C.c_int64)]
+    lib.oracle_process_fp64.restype = C.c_int
     lib.oracle_ncc_cost.argtypes = [P(abi.Params), P(abi.Inputs), C.c_int, C.c_void_p, C.c_void_p, C.c_int]
     lib.oracle_ncc_cost.restype = C.c_int
+    lib.oracle_ncc_cost_fp64.argtypes = lib.oracle_ncc_cost.argtypes
+    lib.oracle_ncc_cost_fp64.restype = C.c_int
     lib.oracle_fuse.argtypes = [P(abi.FuseView), C.c_int, P(abi.FuseParams), C.c_void_p, C.c_int64, P(C.c_int64)]
     lib.oracle_fuse.restype = C.c_int
     lib.oracle_fuse_tat.argtypes = [P(abi.FuseView), C.c_int, C.c_void_p, P(abi.FuseTatParams), C.c_void_p,
@@ -70,6 +74,8 @@ def load_oracle():
     lib.oracle_geom_cost.argtypes = [P(abi.Camera), P(abi.Camera), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
                                      C.c_void_p]
     lib.oracle_geom_cost.restype = C.c_float
+    lib.oracle_geom_cost_fp64.argtypes = lib.oracle_geom_cost.argtypes
+    lib.oracle_geom_cost_fp64.restype = C.c_float
     lib.oracle_initial_cost.argtypes = [C.c_void_p, C.c_int, P(C.c_uint32)]
     lib.oracle_initial_cost.restype = C.c_float
     lib.oracle_random_normal.argtypes = [P(abi.Camera), C.c_int, C.c_int, C.c_float, C.c_uint64, C.c_uint32,
@@ -95,13 +101,15 @@ def load_oracle():
 ORACLE_FAMILIES = 5  # the GPU's four NCC families + evaluations the reference never reads
 
 
-def run_oracle(lib, pin, nthreads=8, families=None):
+def run_oracle(lib, pin, nthreads=8, families=None, fp64=False):
     """One oracle process call on ProblemInputs; returns (ProblemOutputs, (hyps, taps)).
-    If `families` is a list, it receives (hyps, taps) of each NCC family."""
+    If `families` is a list, it receives (hyps, taps) of each NCC family.
+    fp64: the FP64 cost path (oracle_process_fp64)."""
     from dpe_amd import host
     out = host.ProblemOutputs(pin.width, pin.height)
     cnt = (C.c_int64 * (2 + 2 * ORACLE_FAMILIES))()
-    st = lib.oracle_process(C.byref(pin.params), C.byref(pin.struct), C.byref(out.struct), nthreads, cnt)
+    fn = lib.oracle_process_fp64 if fp64 else lib.oracle_process
+    st = fn(C.byref(pin.params), C.byref(pin.struct), C.byref(out.struct), nthreads, cnt)
     if st != 0:
         raise RuntimeError(f"oracle_process failed with status {st}")
     if families is not None:

@@ -162,6 +162,62 @@ def test_precision_form_depth_maps():
         ctx.close()
 
 
+def test_precision_form_depth_error_against_fp64_oracle():
+    """Depth error of the plain path and the precision form against the FP64
+    oracle (oracle_process_fp64: the same PatchMatch with the multi-view cost --
+    homography, projection, bilinear sample, weights, window sums, NCC -- in
+    double; states, random streams and decisions as in the fp32 oracle), on
+    config-1 inputs (320x240, 4 sources, FIRST_INIT with 4 iterations).
+
+    Reported per form: the fraction of pixels whose depth lies within 1e-4,
+    1e-3 and 1e-2 (relative) of the FP64 oracle's, the p90 relative error and
+    the ground-truth accuracy.  PatchMatch is a random search: an argmin or
+    threshold decision flips on a cost difference of rounding size, and the
+    pixel then follows another sample path, so a form departs from the FP64
+    depth map on a fraction of pixels rather than by a uniform small error.
+    The plain GPU path equals the fp32 oracle bit for bit (asserted), so its
+    figure is the fp32 arithmetic's own.  Bars: each 16-bit form within 3
+    points of plain at 1e-2 and within 1 point of the FP64 oracle's
+    ground-truth accuracy."""
+    import oracle_lib
+    from parity_util import bitwise_fractions, depth_agreement
+    s = scene.config1()
+    ref = s.meta["ref_mid"]
+    gt = s.views[ref].depth
+    pin = host.first_call_inputs(s, ref, s.pairs[ref], max_iterations=4)
+    dmin, dmax = pin.params.depth_min, pin.params.depth_max
+    orc = oracle_lib.load_oracle()
+    out64, _ = oracle_lib.run_oracle(orc, pin, 16, fp64=True)
+    out32, _ = oracle_lib.run_oracle(orc, pin, 16)
+    d64 = host.assemble(out64, dmin, dmax)["depth"].astype(np.float64)
+
+    def report(name, d):
+        d = np.asarray(d, np.float64)
+        agree = {tol: depth_agreement(d, d64, tol) for tol in (1e-4, 1e-3, 1e-2)}
+        with np.errstate(divide="ignore", invalid="ignore"):
+            rel = np.where(d64 > 0, np.abs(d - d64) / d64, 0.0)
+        acc = float((np.abs(d - gt) / gt < 0.01).mean())
+        print(f"{name}: vs FP64 oracle within 1e-4 {agree[1e-4]:.4f}, 1e-3 {agree[1e-3]:.4f}, 1e-2 "
+              f"{agree[1e-2]:.4f}, p90 rel {np.quantile(rel, 0.9):.2e}; ground truth within 1 % {acc:.4f}")
+        return agree, acc
+
+    _, acc64 = report("FP64 oracle", d64)
+    res = {}
+    ctx = host.Context(0)
+    try:
+        for name, prec in PRECISIONS.items():
+            out = host.run_hip(ctx, pin, abi.Form(precision=prec))
+            if name == "plain":
+                assert bitwise_fractions(out, out32) == dict(planes=1.0, costs=1.0, pixel_states=1.0,
+                                                             selected_views=1.0)
+            res[name] = report(name, host.assemble(out, dmin, dmax)["depth"])
+    finally:
+        ctx.close()
+    for name in ("fp16", "bf16"):
+        assert res[name][0][1e-2] >= res["plain"][0][1e-2] - 0.03, name
+        assert res[name][1] >= acc64 - 0.01, name
+
+
 def _same(a, b):
     """Bit-identical outputs (NaN payloads included)."""
     return all(np.array_equal(getattr(a, n).view(np.uint8), getattr(b, n).view(np.uint8))

@@ -377,6 +377,69 @@ def test_vectorised_float64_cost_matches_oracle(orc):
         assert abs(exp - c64[0][t]) < 1e-3, (x, y, exp, c64[0][t])
 
 
+def test_fp64_cost_path_matches_float64_restatement(orc):
+    """The oracle's FP64 cost path (oracle_ncc_cost_fp64, the reference of the
+    precision form's depth error) equals the float64 NumPy restatement to
+    double rounding, on the pixels where both take the same tap cells and
+    branches; the fp32 path differs from both by its own rounding."""
+    from ncc64 import view_costs64
+    s = scene.make_scene(160, 120, 150.0, 3, 20.0, seed=22)
+    ref = 1
+    ids = [ref] + s.pairs[ref]
+    v = s.views[ref]
+    cams = host.scene_cameras(s, ids)
+    prm = host.call_params(2, 6, use_APD=0, use_edge=0, use_limit=0, use_label=0)
+    imgs = [s.views[i].image.astype(np.float32) for i in ids]
+    pin = host.ProblemInputs(imgs, cams, prm)
+    rng = np.random.default_rng(29)
+    n = v.normal @ v.R.T
+    depth = v.depth * (1 + 0.01 * rng.standard_normal(v.depth.shape))
+    ys, xs = np.mgrid[0:120, 0:160]
+    X = np.stack([depth * (xs - v.K[0, 2]) / v.K[0, 0], depth * (ys - v.K[1, 2]) / v.K[1, 1], depth], -1)
+    planes = np.concatenate([n, -np.sum(n * X, -1)[..., None]], -1).astype(np.float32)[None]
+    got64 = np.zeros((1, len(ids) - 1, 120, 160), np.float32)
+    assert orc.oracle_ncc_cost_fp64(C.byref(pin.params), C.byref(pin.struct), 1, planes.ctypes.data,
+                                    got64.ctypes.data, 4) == 0
+    got32 = _ncc(orc, pin, planes)
+    iy, ix = np.mgrid[6:114, 6:154]
+    pix = np.stack([ix.ravel(), iy.ravel()], -1)
+    pl = planes[0, pix[:, 1], pix[:, 0]].astype(np.float64)
+    c64, _ = view_costs64(pix, pl[:, :3], pl[:, 3], imgs, cams)
+    for k in range(len(ids) - 1):
+        exp = c64[k].astype(np.float32)  # the FP64 path rounds the cost to fp32 once
+        e64 = np.abs(got64[0, k][pix[:, 1], pix[:, 0]].astype(np.float64) - exp)
+        e32 = np.abs(got32[0, k][pix[:, 1], pix[:, 0]].astype(np.float64) - exp)
+        # equal to a few fp32 ulps of the cost (< 2) wherever the centre projects inside
+        assert (e64 <= 5e-7).mean() >= 0.999, (k, np.quantile(e64, 0.999))
+        assert np.median(e32) > 10 * max(np.median(e64), 1e-9)
+
+
+def test_fp64_geom_cost_matches_float64_definition(orc):
+    """oracle_geom_cost_fp64 equals the float64 evaluation of the geometric-
+    consistency cost's definition (the fp32 path is within 2e-3 px of it)."""
+    from test_oracle_geometry_pins import _cam, _rot, _plane_depth_map, _cam_plane, _geom_cost64
+    ref = _cam(np.eye(3), [0, 0, 0])
+    src = _cam(_rot("y", 0.12) @ _rot("x", -0.05), [0.6, 0.1, 0.2])
+    rng = np.random.default_rng(83)
+    checked = 0
+    for trial in range(3):
+        n_w = np.array([rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), -1.0])
+        n_w /= np.linalg.norm(n_w)
+        p0 = np.array([0.0, 0.0, rng.uniform(3.0, 5.0)])
+        dmap = (_plane_depth_map(src, n_w, p0) * (1 + 0.01 * rng.standard_normal((240, 320)))).astype(np.float32)
+        plane = _cam_plane(ref, n_w, p0)
+        for _ in range(200):
+            px, py = int(rng.integers(20, 300)), int(rng.integers(20, 220))
+            exp, u, v = _geom_cost64(ref, src, dmap, px, py, plane)
+            if min(abs(u - round(u)), abs(v - round(v))) < 1e-6 or not (0 <= u < 319 and 0 <= v < 239):
+                continue
+            got = orc.oracle_geom_cost_fp64(C.byref(ref), C.byref(src), 320, 240, dmap.ctypes.data, px, py,
+                                            plane.ctypes.data)
+            assert got == np.float32(exp), (trial, px, py, got, exp)
+            checked += 1
+    assert checked > 400
+
+
 # ------------------------------------------------------------ process calls
 def test_oracle_process_status_codes(orc):
     s = scene.make_scene(48, 40, 45.0, 3, 20.0, seed=5, supersample=1)
