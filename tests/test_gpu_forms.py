@@ -176,9 +176,11 @@ def test_precision_form_depth_error_against_fp64_oracle():
     pixel then follows another sample path, so a form departs from the FP64
     depth map on a fraction of pixels rather than by a uniform small error.
     The plain GPU path equals the fp32 oracle bit for bit (asserted), so its
-    figure is the fp32 arithmetic's own.  Bars: each 16-bit form within 3
-    points of plain at 1e-2 and within 1 point of the FP64 oracle's
-    ground-truth accuracy."""
+    figure is the fp32 arithmetic's own.  Measured (one MI355X): plain 77.6 /
+    86.6 / 91.9 % within 1e-4 / 1e-3 / 1e-2; fp16 and bf16 (the 8-bit texels
+    are exact in both) 98.1 / 99.2 / 99.7 %.  Bars: each 16-bit form within
+    1e-3 of the FP64 depth on >= 97 % of pixels and on 5 points more than
+    plain, and within 1 point of the FP64 oracle's ground-truth accuracy."""
     import oracle_lib
     from parity_util import bitwise_fractions, depth_agreement
     s = scene.config1()
@@ -214,7 +216,7 @@ def test_precision_form_depth_error_against_fp64_oracle():
     finally:
         ctx.close()
     for name in ("fp16", "bf16"):
-        assert res[name][0][1e-2] >= res["plain"][0][1e-2] - 0.03, name
+        assert res[name][0][1e-3] >= 0.97 and res[name][0][1e-3] >= res["plain"][0][1e-3] + 0.05, name
         assert res[name][1] >= acc64 - 0.01, name
 
 
