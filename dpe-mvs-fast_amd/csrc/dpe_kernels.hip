@@ -2239,6 +2239,61 @@ __device__ __forceinline__ int weak_candidates(const Dev &d, int p, int position
     return u;
 }
 
+// Candidate cost rows of the weak propagation (ComputeMultiViewCostVectorNew
+// of the STRONG GenNeighbours points, DPE.cu:1700-1711), evaluated apart from
+// the rest of the weak update so that every lane has a window to evaluate: a
+// wave takes 64 consecutive entries of the colour's weak list, lists their
+// distinct candidates as (entry, slot) items in LDS, entry by entry (the
+// candidates of one pixel share its neighbour-window positions, so their
+// gathers fall on nearby source texels), and evaluates 64 items at a time,
+// each lane looping over the views.  Row (slot i, view v) of entry li goes to
+// weak_cost[(i * NV + v - 1) * n + li]; k_weak_update reads it back.
+constexpr int kWeakCandWaves = 4;
+__global__ __launch_bounds__(kLanes * kWeakCandWaves) void k_weak_cand(Dev d, int colour) {
+    __shared__ uint16_t items[kWeakCandWaves][kLanes * 8];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
+    const int n = d.weak_colour_count[colour];
+    const int *list = d.weak_colour_list[colour];
+    const int li0 = (xcd_block(blockIdx.x, gridDim.x) * kWeakCandWaves + wave) * kLanes;
+    const int li = li0 + lane;
+    uint32_t mask = 0;  // bit i: slot i holds a distinct STRONG candidate
+    if (li < n && d.weak[list[li]] == DPE_WEAK) {
+        bool flag[8];
+        int positions[8], src[8];
+        weak_candidates(d, list[li], positions, flag, src);
+        for (int i = 0; i < 8; ++i)
+            if (flag[i] && src[i] == i) mask |= 1u << i;
+    }
+    // exclusive prefix sum of the per-lane counts (0..8, four bits) over the wave
+    const uint32_t cnt = (uint32_t)__popc(mask);
+    int first = 0, total = 0;
+    for (int b = 0; b < 4; ++b) {
+        const unsigned long long bb = __ballot((cnt >> b) & 1u);
+        first += lanes_below(bb) << b;
+        total += __popcll(bb) << b;
+    }
+    uint16_t *it = items[wave];
+    for (uint32_t m = mask; m; m &= m - 1) it[first++] = (uint16_t)((lane << 3) | __builtin_ctz(m));
+    wave_sync();
+    unsigned long long hyps = 0, taps = 0;
+    const int NV = d.NV;
+    for (int base = 0; base < total; base += kLanes) {
+        const int k = base + lane;
+        if (k < total) {
+            const uint32_t e = it[k];
+            const int lk = li0 + (int)(e >> 3), i = (int)(e & 7u);
+            const int p = list[lk];
+            const int x = p % d.W, y = p / d.W;
+            const short2 np = neighbour_point(d, p, i + 1);
+            const float4 pl = d.planes[np.x + np.y * d.W];
+            float *row = d.weak_cost + (size_t)i * NV * n + lk;
+            for (int v = 1; v < d.N; ++v) row[(size_t)(v - 1) * n] = ncc_new_pre(d, x, y, v, pl, colour, lk, hyps, taps);
+        }
+    }
+    count_add(d.counters, kFamWeak, hyps, taps);
+}
+
 // CheckerboardPropagationWeak + PlaneHypothesisRefinementWeak, DPE.cu:1672-1866, 1124-1216
 __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int iter, unsigned long long &hyps,
                                                   unsigned long long &taps) {
@@ -2254,29 +2309,29 @@ __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int 
     const dpe_camera &cam = d.cams[0];
     const bool geom = d.prm.geom_consistency != 0;
     const float gf = d.prm.geom_factor;
-    float cost_array[8][32];
-    for (int i = 0; i < 8; ++i)
-        for (int j = 0; j < 32; ++j) cost_array[i][j] = 0.0f;
-    cost_array[0][0] = 2.0f;
     bool flag[8];
     int positions[8], src[8];
     float4 new_plane[8];
     weak_candidates(d, p, positions, flag, src);
-    int uniq[8], nu = 0;
-    for (int i = 0; i < 8; ++i) {
-        new_plane[i] = flag[i] ? d.planes[positions[i]] : make_float4(0, 0, 0, 0);
-        if (flag[i] && src[i] == i) uniq[nu++] = i;
-    }
-    // the distinct candidates back to back (a wave loops to its lanes' largest
-    // count rather than over all 8 slots), then the repeated ones copied
-    for (int j = 0; j < nu; ++j) {
-        const int i = uniq[j];
-        const float4 pl = new_plane[i];
-        for (int v = 1; v < d.N; ++v) cost_array[i][v - 1] = ncc_new_pre(d, x, y, v, pl, colour, li, hyps, taps);
-    }
+    for (int i = 0; i < 8; ++i) new_plane[i] = flag[i] ? d.planes[positions[i]] : make_float4(0, 0, 0, 0);
+    // the candidate cost matrix: k_weak_cand's rows, read into a local array in
+    // one pass (independent loads; read where they are used, each would wait on
+    // its own load); a repeated candidate takes the row of the first one at its
+    // pixel; rows without a candidate are {2, 0, ...} and 0 as the reference
+    // initialises them (DPE.cu:1694)
+    float cost_array[8][32];
     for (int i = 0; i < 8; ++i)
-        if (flag[i] && src[i] != i)
-            for (int v = 1; v < d.N; ++v) cost_array[i][v - 1] = cost_array[src[i]][v - 1];
+        for (int j = 0; j < 32; ++j) cost_array[i][j] = 0.0f;
+    cost_array[0][0] = 2.0f;
+    {
+        const int n = d.weak_colour_count[colour];
+        for (int i = 0; i < 8; ++i) {
+            if (!flag[i]) continue;
+            const float *row = d.weak_cost + (size_t)src[i] * NV * n + li;
+            for (int v = 0; v < NV; ++v) cost_array[i][v] = row[(size_t)v * n];
+        }
+    }
+    auto cost = [&](int i, int v) -> float { return cost_array[i][v]; };
     float priors[32];
     for (int j = 0; j < NV; ++j) priors[j] = 0.0f;
     for (int i = 0; i < 8; ++i) {
@@ -2288,7 +2343,6 @@ __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int 
     Rng rng(d.prm.seed, p, ph_weak(iter), d.prm.call_index);
     uint8_t vwl[kMaxImages];
     uint32_t tsel = 0;
-    auto cost = [&](int i, int v) -> float { return cost_array[i][v]; };
     const float wn = joint_view_selection(d, iter, cost, priors, rng, vwl, &tsel);
     float final_costs[8];
     for (int i = 0; i < 8; ++i) {
@@ -2297,11 +2351,11 @@ __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int 
             if (vwl[j] > 0) {
                 if (geom) {
                     if (flag[i])
-                        fc += vwl[j] * (cost_array[i][j] + gf * geom_cost(d, x, y, j + 1, d.planes[positions[i]]));
+                        fc += vwl[j] * (cost(i, j) + gf * geom_cost(d, x, y, j + 1, d.planes[positions[i]]));
                     else
-                        fc += vwl[j] * (cost_array[i][j] + gf * 3.0f);
+                        fc += vwl[j] * (cost(i, j) + gf * 3.0f);
                 } else {
-                    fc += vwl[j] * cost_array[i][j];
+                    fc += vwl[j] * cost(i, j);
                 }
             }
         }
@@ -2407,7 +2461,7 @@ __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int 
     d.costs[p] = c2 / wn;
 }
 
-__global__ __launch_bounds__(256) void k_weak_update(Dev d, int colour, int iter) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_weak_update(Dev d, int colour, int iter) {
     unsigned long long hyps = 0, taps = 0;
     weak_update_pixel(d, colour, iter, hyps, taps);
     count_add(d.counters, kFamWeak, hyps, taps);
@@ -2779,6 +2833,8 @@ void launch_ransac(const Dev &d, int iter, hipStream_t s) {
 void launch_weak_half(const Dev &d, int colour, int iter, hipStream_t s) {
     if (d.weak_colour_count[colour] > 0) {
         hipLaunchKernelGGL(k_weak_patch, grid1(d.weak_colour_count[colour], 256), dim3(256), 0, s, d, colour);
+        hipLaunchKernelGGL(k_weak_cand, grid1(d.weak_colour_count[colour], kLanes * kWeakCandWaves),
+                           dim3(kLanes * kWeakCandWaves), 0, s, d, colour);
         hipLaunchKernelGGL(k_weak_update, grid1(d.weak_colour_count[colour], 64), dim3(64), 0, s, d, colour, iter);
     }
 }

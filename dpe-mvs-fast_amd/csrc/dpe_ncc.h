@@ -98,12 +98,18 @@ __device__ __forceinline__ float sample_src_any(const Dev &d, int v, float u, fl
 // instructions where floor, subtract and a float cell offset took 6); the
 // texel differences t.y - t.x, t.w - t.z are taken in packed fp16, exact on the
 // quarter grid, so the sample equals sample_src's bit for bit.
-// cell0 = cell (0, 0) of the view: wave-uniform where the view is (the
-// gather then takes a scalar base and a 32-bit offset per tap).
-__device__ __forceinline__ float sample_fast(const char *__restrict__ cell0, uint32_t row_cells, float u, float w) {
+// base = cell (0, 0) of view 0 (wave-uniform), voff = the view's byte offset
+// from it: all views' quads lie within 4 GiB of base (checked at set-up), so
+// the gather takes a scalar base and one 32-bit offset per tap even where the
+// view varies across the wave.
+__device__ __forceinline__ float sample_fast(const char *__restrict__ base, uint32_t voff, uint32_t row_cells, float u,
+                                             float w) {
     const float a = __builtin_amdgcn_fractf(u), b = __builtin_amdgcn_fractf(w);
-    const uint32_t cell = __umul24((uint32_t)w, row_cells) + (uint32_t)u;
-    const uint2 h = *(const uint2 *)(cell0 + cell * 8u);
+    // one v_mad_u32_u24 for the cell index (the compiler otherwise folds voff
+    // into the sum and spends a third instruction on the address)
+    uint32_t cell;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(cell) : "v"((uint32_t)w), "s"(row_cells), "v"((uint32_t)u));
+    const uint2 h = *(const uint2 *)(base + (voff + cell * 8u));
     const half2_t p = __builtin_bit_cast(half2_t, h.x);  // (t.x, t.z)
     const half2_t q = __builtin_bit_cast(half2_t, h.y);  // (t.y, t.w)
     const half2_t dd = q - p;
@@ -219,8 +225,9 @@ __device__ __forceinline__ void fill_shared_patch(const Dev &d, bool valid, int 
 template <int NT, bool REFINED, int FMT, bool FAST = false>
 __device__ __forceinline__ void shared_window_sums(const Dev &d, const SharedPatch<NT> &S, int lane, int x, int y,
                                                    int v, const float H[9], float &s_s, float &s_ss, float &s_rs) {
-    // FAST: cell (0, 0) of the fp16 quads and the cells per row
-    const char *cell0 = (const char *)(d.quads_h + (size_t)v * d.qs) + (size_t)(d.W + 2) * 8;
+    // FAST: cell (0, 0) of view 0's fp16 quads, view v's byte offset from it, the cells per row
+    const char *base = (const char *)d.quads_h + (size_t)(d.W + 2) * 8;
+    const uint32_t voff = (uint32_t)v * (uint32_t)d.qs * 8u;
     const uint32_t row_cells = (uint32_t)(d.W + 1);
 #pragma unroll 1
     for (int ki = 0; ki < NT; ++ki) {
@@ -230,7 +237,7 @@ __device__ __forceinline__ void shared_window_sums(const Dev &d, const SharedPat
         for (int kj = 0; kj < NT; ++kj) {
             const int ry = y + d.tap_off0 + kj * d.tap_inc;
             const float2 sp = REFINED ? project_refined(H, rx, ry) : project(H, rx, ry);
-            const float src = FAST ? sample_fast(cell0, row_cells, sp.x, sp.y) : sample_src<FMT>(d, v, sp.x, sp.y);
+            const float src = FAST ? sample_fast(base, voff, row_cells, sp.x, sp.y) : sample_src<FMT>(d, v, sp.x, sp.y);
             const float2 e = S.wwr[ki * NT + kj][lane];
             r_s = fmaf(e.x, src, r_s);
             r_ss = fmaf(e.x * src, src, r_ss);

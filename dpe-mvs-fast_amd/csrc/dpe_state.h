@@ -100,6 +100,7 @@ struct Dev {
     // reference halves of the weak windows per colour-list entry (k_weak_patch)
     float *weak_wpre[2];    // [kWeakPatchTaps][count] bilateral weight of every tap
     float *weak_wsum[2];    // [3 * kNeighbourNum][count]
+    float *weak_cost;       // [8][NV][count] candidate cost rows of the launched colour (k_weak_cand)
     // GenNeighbours constants (DPE.cu:2151-2157, host double math)
     float gn_cos, gn_sin, gn_thr;
     int gn_shift;
