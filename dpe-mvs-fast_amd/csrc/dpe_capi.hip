@@ -82,7 +82,7 @@ enum BufId {
     B_F_VIEWS, B_F_DEPTH, B_F_NORMAL, B_F_STATES, B_F_COLOR, B_F_MASK, B_F_CLAIM, B_F_USED0, B_F_USED1, B_F_ACC0,
     B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_POSTPLANES, B_WEAKLIST, B_ADJ_WEIGHTS,
     B_ADJ_GRADS, B_QUADS_H, B_FLAG, B_WPRE, B_WSUM, B_VC64, B_WEAKCOUNTS, B_WEAKTEMP, B_NOWROW, B_CURCOST,
-    B_CURVALID, B_LASTCAND, B_WEAKCOST, B_COUNT
+    B_CURVALID, B_LASTCAND, B_WEAKCOST, B_WEAKST, B_WEAKTERMS, B_COUNT
 };
 
 struct Fail {
@@ -570,8 +570,10 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
         d.weak_wpre[1] = wpre + (size_t)kWeakPatchTaps * cc[0];
         d.weak_wsum[0] = wsum;
         d.weak_wsum[1] = wsum + (size_t)3 * kNeighbourNum * cc[0];
-        d.weak_cost = (float *)buffer(ctx, B_WEAKCOST,
-                                      sizeof(float) * 8 * (N - 1) * ((size_t)std::max(cc[0], cc[1]) + 1));
+        const size_t nmax = (size_t)std::max(cc[0], cc[1]) + 1;
+        d.weak_cost = (float *)buffer(ctx, B_WEAKCOST, sizeof(float) * 8 * (N - 1) * nmax);
+        d.weak_st = (WeakState *)buffer(ctx, B_WEAKST, sizeof(WeakState) * nmax);
+        d.weak_terms = (float *)buffer(ctx, B_WEAKTERMS, sizeof(float) * kWeakTermSlots * (N - 1) * nmax);
     }
     d.nmap = nmap;
     // state buffers

@@ -2294,26 +2294,51 @@ __global__ __launch_bounds__(kLanes * kWeakCandWaves) void k_weak_cand(Dev d, in
     count_add(d.counters, kFamWeak, hyps, taps);
 }
 
-// CheckerboardPropagationWeak + PlaneHypothesisRefinementWeak, DPE.cu:1672-1866, 1124-1216
-__device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int iter, unsigned long long &hyps,
-                                                  unsigned long long &taps) {
-    // XCD-contiguous ranges of the tile-ordered weak list: the windows of
-    // nearby weak pixels share source cache lines (13 % faster at 1600x1200)
-    const int li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+// CheckerboardPropagationWeak + PlaneHypothesisRefinementWeak (DPE.cu:1672-1866,
+// 1124-1216), split where the windows are evaluated so that every window is a
+// lane's own item (a weak pixel's window counts vary with its candidates,
+// selected views and fitted plane, and one lane per pixel left half the lanes
+// idle):
+//   k_weak_cand    candidate cost rows (above)
+//   k_weak_select  view selection over the candidates, their final costs, argmin
+//   k_weak_terms   window terms of the current and the fitted plane
+//   k_weak_mid     current cost, candidate and fitted-plane acceptance, the 5
+//                  refinement hypotheses
+//   k_weak_terms   window terms of the hypotheses
+//   k_weak_final   hypothesis acceptance, write-back, final cost (NCC Old)
+// Every value is computed by the same operations in the same order as in one
+// pass over the pixel; the sums of the terms run over the views in order.
+
+// One thread per entry of the colour's weak list (XCD-contiguous ranges of the
+// tile-ordered list, so that nearby weak pixels share source cache lines);
+// false for entries past the list or not WEAK.
+__device__ __forceinline__ bool weak_entry(const Dev &d, int colour, int &li, int &p, int &x, int &y) {
+    li = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (li >= d.weak_colour_count[colour]) return false;
+    p = d.weak_colour_list[colour][li];
+    x = p % d.W;
+    y = p / d.W;
+    return d.weak[p] == DPE_WEAK;
+}
+
+// Multi-hypothesis joint view selection and the candidates' final costs
+// (DPE.cu:1713-1807).
+__global__ __launch_bounds__(64) void k_weak_select(Dev d, int colour, int iter) {
+    int li, p, x, y;
+    const bool act = weak_entry(d, colour, li, p, x, y);
     if (li >= d.weak_colour_count[colour]) return;
+    WeakState &st = d.weak_st[li];
+    if (!act) {
+        st.flags = 0;
+        return;
+    }
     const int W = d.W;
-    const int p = d.weak_colour_list[colour][li];
-    const int x = p % W, y = p / W;
-    if (d.weak[p] != DPE_WEAK) return;
     const int NV = d.NV;
-    const dpe_camera &cam = d.cams[0];
     const bool geom = d.prm.geom_consistency != 0;
     const float gf = d.prm.geom_factor;
     bool flag[8];
     int positions[8], src[8];
-    float4 new_plane[8];
     weak_candidates(d, p, positions, flag, src);
-    for (int i = 0; i < 8; ++i) new_plane[i] = flag[i] ? d.planes[positions[i]] : make_float4(0, 0, 0, 0);
     // the candidate cost matrix: k_weak_cand's rows, read into a local array in
     // one pass (independent loads; read where they are used, each would wait on
     // its own load); a repeated candidate takes the row of the first one at its
@@ -2367,61 +2392,151 @@ __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int 
         for (int i = 1; i < 8; ++i)
             if (final_costs[i] <= mc) { mc = final_costs[i]; mi = i; }
     }
-    const float4 cur = d.planes[p];
-    // Views of weight 0 add exactly 0 to this sum and to the final one below:
-    // the costs are finite (clamped to [0, 2] and [0, 3]) and the sums start
-    // at +0, so those views are not evaluated.
-    float cost_now = 0.0f;
-    for (int v = 1; v < d.N; ++v) {
-        if (vwl[v - 1] == 0) continue;
-        const float c = ncc_new_pre(d, x, y, v, cur, colour, li, hyps, taps);
-        if (geom) cost_now += vwl[v - 1] * (c + gf * geom_cost(d, x, y, v, cur));
-        else cost_now += vwl[v - 1] * c;
+    // the view weights (the reference's selection writes them in place,
+    // DPE.cu:1761-1771)
+    uint8_t *vwo = d.vw + (size_t)p * kMaxImages;
+    for (int i = 0; i < kMaxImages; ++i) vwo[i] = vwl[i];
+    const float4 fit = d.fit[p];
+    uint32_t fl = kWeakActive;
+    if (flag[mi]) fl |= kWeakCand;
+    if (!(fit.x == 0 && fit.y == 0 && fit.z == 0)) fl |= kWeakFit;
+    st.cand = flag[mi] ? d.planes[positions[mi]] : make_float4(0, 0, 0, 0);
+    st.fc_mi = final_costs[mi];
+    st.wn = wn;
+    st.tsel = tsel;
+    st.rng_n = rng.n;
+    st.flags = fl;
+}
+
+// Window terms vw_v * (cost_v + geom_factor * geometric cost_v) (or vw_v * cost_v)
+// of the weak entries' planes over their selected views: slots 0 (the current
+// plane) and 1 (the fitted plane, where there is one) in PHASE 0, the 5
+// refinement hypotheses (where the entry draws them) in PHASE 1.  A wave takes
+// 64 consecutive entries and deals their (view, slot) items, entry by entry, to
+// its lanes 64 at a time; term (slot, v) of entry li goes to
+// weak_terms[(slot * NV + v - 1) * n + li].
+template <int PHASE>
+__global__ __launch_bounds__(kLanes * kWeakCandWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_weak_terms(Dev d, int colour) {
+    __shared__ int pref[kWeakCandWaves][kLanes + 1];
+    __shared__ uint32_t vmask[kWeakCandWaves][kLanes];
+    __shared__ uint8_t nslot[kWeakCandWaves][kLanes];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = threadIdx.x / kLanes;
+    const int n = d.weak_colour_count[colour];
+    const int *list = d.weak_colour_list[colour];
+    const int li0 = (xcd_block(blockIdx.x, gridDim.x) * kWeakCandWaves + wave) * kLanes;
+    const int li = li0 + lane;
+    const int NV = d.NV;
+    uint32_t views = 0;
+    int ns = 0;
+    if (li < n) {
+        const uint32_t fl = d.weak_st[li].flags;
+        if (fl & kWeakActive) {
+            const uint8_t *vw = d.vw + (size_t)list[li] * kMaxImages;
+            for (int v = 0; v < NV; ++v)
+                if (vw[v] > 0) views |= 1u << v;
+            ns = PHASE == 0 ? ((fl & kWeakFit) ? 2 : 1) : ((fl & kWeakRandom) ? 5 : 0);
+        }
     }
+    int incl = ns * __popc(views);
+    for (int o = 1; o < kLanes; o <<= 1) {
+        const int t = __shfl_up(incl, o, kLanes);
+        if (lane >= o) incl += t;
+    }
+    pref[wave][lane + 1] = incl;
+    if (lane == 0) pref[wave][0] = 0;
+    vmask[wave][lane] = views;
+    nslot[wave][lane] = (uint8_t)ns;
+    const int total = __shfl(incl, kLanes - 1, kLanes);
+    wave_sync();
+    unsigned long long hyps = 0, taps = 0;
+    const bool geom = d.prm.geom_consistency != 0;
+    const float gf = d.prm.geom_factor;
+    for (int base = 0; base < total; base += kLanes) {
+        const int k = base + lane;
+        if (k < total) {
+            // the entry: the last lane whose items start at or before k
+            int lo = 0, hi = kLanes - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pref[wave][mid] <= k) lo = mid;
+                else hi = mid - 1;
+            }
+            const int j = k - pref[wave][lo];
+            const int nsl = nslot[wave][lo];
+            const int vi = j / nsl;
+            const int slot = (PHASE == 0 ? 0 : 2) + (j - vi * nsl);
+            uint32_t vm = vmask[wave][lo];
+            for (int t = 0; t < vi; ++t) vm &= vm - 1;
+            const int v = __builtin_ctz(vm) + 1;
+            const int lk = li0 + lo;
+            const int p = list[lk];
+            const int x = p % d.W, y = p / d.W;
+            const float4 pl = slot == 0 ? d.planes[p] : (slot == 1 ? d.fit[p] : d.weak_st[lk].hyp[slot - 2]);
+            const float c = ncc_new_pre(d, x, y, v, pl, colour, lk, hyps, taps);
+            const uint8_t w = d.vw[(size_t)p * kMaxImages + v - 1];
+            const float term = geom ? w * (c + gf * geom_cost(d, x, y, v, pl)) : w * c;
+            d.weak_terms[((size_t)slot * NV + v - 1) * n + lk] = term;
+        }
+    }
+    count_add(d.counters, kFamWeak, hyps, taps);
+}
+
+// sum of an entry's terms of one slot over its selected views, in view order
+__device__ __forceinline__ float weak_term_sum(const Dev &d, int n, int li, int slot, const uint8_t *vwl) {
+    const float *t = d.weak_terms + (size_t)slot * d.NV * n + li;
+    float s = 0.0f;
+    for (int v = 0; v < d.NV; ++v)
+        if (vwl[v] > 0) s += t[(size_t)v * n];
+    return s;
+}
+
+// The current plane's cost, the candidate's acceptance (DPE.cu:1809-1834), the
+// fitted plane's (PlaneHypothesisRefinementWeak, DPE.cu:1143-1172) and the
+// refinement hypotheses (DPE.cu:1174-1192).
+__global__ __launch_bounds__(64) void k_weak_mid(Dev d, int colour, int iter) {
+    int li, p, x, y;
+    if (!weak_entry(d, colour, li, p, x, y)) return;
+    WeakState &st = d.weak_st[li];
+    const uint32_t fl = st.flags;
+    const int n = d.weak_colour_count[colour];
+    const dpe_camera &cam = d.cams[0];
+    const float wn = st.wn;
+    uint8_t vwl[kMaxImages];
+    const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
+    for (int i = 0; i < kMaxImages; ++i) vwl[i] = vw[i];
+    const float4 cur = d.planes[p];
+    float cost_now = weak_term_sum(d, n, li, 0, vwl);
     cost_now /= wn;
     d.costs[p] = cost_now;
     float depth_now = depth_from_plane(cam, cur, x, y);
     float4 plane_now = cur;
-    if (flag[mi]) {
-        const float db = depth_from_plane(cam, new_plane[mi], x, y);
-        if (db >= d.prm.depth_min && db <= d.prm.depth_max && final_costs[mi] < cost_now) {
+    if (fl & kWeakCand) {
+        const float db = depth_from_plane(cam, st.cand, x, y);
+        if (db >= d.prm.depth_min && db <= d.prm.depth_max && st.fc_mi < cost_now) {
             depth_now = db;
-            plane_now = new_plane[mi];
-            cost_now = final_costs[mi];
-            d.selected[p] = tsel;
+            plane_now = st.cand;
+            cost_now = st.fc_mi;
+            d.selected[p] = st.tsel;
         }
     }
-    uint8_t *vwo = d.vw + (size_t)p * kMaxImages;
-    for (int i = 0; i < kMaxImages; ++i) vwo[i] = vwl[i];
-    // PlaneHypothesisRefinementWeak
     const float dmin = d.prm.depth_min, dmax = d.prm.depth_max;
-    auto weighted = [&](const float4 &tp) -> float {
-        float tc = 0.0f;
-        for (int v = 1; v < d.N; ++v) {
-            if (vwl[v - 1] > 0) {
-                const float c = ncc_new_pre(d, x, y, v, tp, colour, li, hyps, taps);
-                if (geom) tc += vwl[v - 1] * (c + gf * geom_cost(d, x, y, v, tp));
-                else tc += vwl[v - 1] * c;
-            }
-        }
-        return tc / wn;
-    };
     bool do_random = true;
-    {
+    if (!(fl & kWeakFit)) {
+        do_random = false;
+    } else {
         const float4 fit = d.fit[p];
-        if (fit.x == 0 && fit.y == 0 && fit.z == 0) {
-            do_random = false;
-        } else {
-            const float tc = weighted(fit);
-            const float db = depth_from_plane(cam, fit, x, y);
-            if (db >= dmin && db <= dmax && tc < cost_now) {
-                depth_now = db;
-                plane_now = fit;
-                cost_now = tc;
-            }
+        const float tc = weak_term_sum(d, n, li, 1, vwl) / wn;
+        const float db = depth_from_plane(cam, fit, x, y);
+        if (db >= dmin && db <= dmax && tc < cost_now) {
+            depth_now = db;
+            plane_now = fit;
+            cost_now = tc;
         }
     }
     if (do_random) {
+        Rng rng(d.prm.seed, p, ph_weak(iter), d.prm.call_index);
+        rng.n = st.rng_n;
         const float depth_rand = rng.uniform() * (dmax - dmin) + dmin;
         const float4 plane_rand = random_normal(cam, x, y, rng, depth_now);
         const float dminp = (1 - 0.02f) * depth_now;
@@ -2431,39 +2546,63 @@ __device__ __forceinline__ void weak_update_pixel(const Dev &d, int colour, int 
         const float depths[5] = {depth_rand, depth_now, depth_rand, depth_now, depth_pert};
         const float4 normals[5] = {plane_now, plane_rand, plane_rand, plane_pert, plane_now};
         for (int i = 0; i < 5; ++i) {
-            const float4 n = normals[i];
-            const float4 tp = make_float4(n.x, n.y, n.z, distance_to_origin(cam, x, y, depths[i], n));
-            const float tc = weighted(tp);
-            const float db = depth_from_plane(cam, tp, x, y);
-            if (db >= dmin && db <= dmax && tc < cost_now) {
-                depth_now = db;
-                plane_now = tp;
-                cost_now = tc;
+            const float4 nn = normals[i];
+            st.hyp[i] = make_float4(nn.x, nn.y, nn.z, distance_to_origin(cam, x, y, depths[i], nn));
+        }
+        st.flags = fl | kWeakRandom;
+    }
+    st.plane_now = plane_now;
+    st.depth_now = depth_now;
+    st.cost_now = cost_now;
+}
+
+// The hypotheses' acceptance (DPE.cu:1193-1213), the write-back
+// (DPE.cu:1838-1847) and the final cost with the Old NCC (DPE.cu:1849-1865).
+__global__ __launch_bounds__(64) void k_weak_final(Dev d, int colour) {
+    int li, p, x, y;
+    const bool act = weak_entry(d, colour, li, p, x, y);
+    unsigned long long hyps = 0, taps = 0;
+    if (act) {
+        const WeakState &st = d.weak_st[li];
+        const int n = d.weak_colour_count[colour];
+        const dpe_camera &cam = d.cams[0];
+        const float wn = st.wn;
+        uint8_t vwl[kMaxImages];
+        const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
+        for (int i = 0; i < kMaxImages; ++i) vwl[i] = vw[i];
+        const float dmin = d.prm.depth_min, dmax = d.prm.depth_max;
+        float depth_now = st.depth_now, cost_now = st.cost_now;
+        float4 plane_now = st.plane_now;
+        if (st.flags & kWeakRandom) {
+            for (int i = 0; i < 5; ++i) {
+                const float4 tp = st.hyp[i];
+                const float tc = weak_term_sum(d, n, li, 2 + i, vwl) / wn;
+                const float db = depth_from_plane(cam, tp, x, y);
+                if (db >= dmin && db <= dmax && tc < cost_now) {
+                    depth_now = db;
+                    plane_now = tp;
+                    cost_now = tc;
+                }
             }
         }
-    }
-    float4 final_plane = cur;
-    if (d.prm.state == DPE_REFINE_INIT) {
-        if ((double)cost_now < (double)d.costs[p] - 0.1) {
+        (void)depth_now;
+        float4 final_plane = d.planes[p];
+        if (d.prm.state == DPE_REFINE_INIT) {
+            if ((double)cost_now < (double)d.costs[p] - 0.1) {
+                d.costs[p] = cost_now;
+                d.planes[p] = plane_now;
+                final_plane = plane_now;
+            }
+        } else {
             d.costs[p] = cost_now;
             d.planes[p] = plane_now;
             final_plane = plane_now;
         }
-    } else {
-        d.costs[p] = cost_now;
-        d.planes[p] = plane_now;
-        final_plane = plane_now;
+        float c2 = 0.0f;
+        for (int v = 1; v < d.N; ++v)
+            if (vwl[v - 1] > 0) c2 += vwl[v - 1] * ncc_old_direct(d, x, y, v, final_plane, hyps, taps);
+        d.costs[p] = c2 / wn;
     }
-    // final cost with the Old NCC (DPE.cu:1849-1865)
-    float c2 = 0.0f;
-    for (int v = 1; v < d.N; ++v)
-        if (vwl[v - 1] > 0) c2 += vwl[v - 1] * ncc_old_direct(d, x, y, v, final_plane, hyps, taps);
-    d.costs[p] = c2 / wn;
-}
-
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_weak_update(Dev d, int colour, int iter) {
-    unsigned long long hyps = 0, taps = 0;
-    weak_update_pixel(d, colour, iter, hyps, taps);
     count_add(d.counters, kFamWeak, hyps, taps);
 }
 
@@ -2835,7 +2974,14 @@ void launch_weak_half(const Dev &d, int colour, int iter, hipStream_t s) {
         hipLaunchKernelGGL(k_weak_patch, grid1(d.weak_colour_count[colour], 256), dim3(256), 0, s, d, colour);
         hipLaunchKernelGGL(k_weak_cand, grid1(d.weak_colour_count[colour], kLanes * kWeakCandWaves),
                            dim3(kLanes * kWeakCandWaves), 0, s, d, colour);
-        hipLaunchKernelGGL(k_weak_update, grid1(d.weak_colour_count[colour], 64), dim3(64), 0, s, d, colour, iter);
+        const int n = d.weak_colour_count[colour];
+        hipLaunchKernelGGL(k_weak_select, grid1(n, 64), dim3(64), 0, s, d, colour, iter);
+        hipLaunchKernelGGL(k_weak_terms<0>, grid1(n, kLanes * kWeakCandWaves), dim3(kLanes * kWeakCandWaves), 0, s,
+                           d, colour);
+        hipLaunchKernelGGL(k_weak_mid, grid1(n, 64), dim3(64), 0, s, d, colour, iter);
+        hipLaunchKernelGGL(k_weak_terms<1>, grid1(n, kLanes * kWeakCandWaves), dim3(kLanes * kWeakCandWaves), 0, s,
+                           d, colour);
+        hipLaunchKernelGGL(k_weak_final, grid1(n, 64), dim3(64), 0, s, d, colour);
     }
 }
 void launch_depth_and_normal(const Dev &d, hipStream_t s) {

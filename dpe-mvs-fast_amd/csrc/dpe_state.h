@@ -36,6 +36,24 @@ struct NccTile {
     int lds_pad = 0;
 };
 
+// Per-entry state of a weak half-iteration between its kernels (k_weak_select
+// -> k_weak_mid -> k_weak_final), one per entry of the colour's weak list.
+struct WeakState {
+    float4 cand;       // the argmin candidate's plane
+    float4 plane_now;  // best plane so far (k_weak_mid)
+    float4 hyp[5];     // refinement hypotheses (k_weak_mid)
+    float fc_mi;       // the argmin candidate's final cost
+    float wn;          // view weight norm
+    float depth_now, cost_now;
+    uint32_t tsel;     // views selected with the candidates
+    uint32_t rng_n;    // draws taken from the entry's random stream
+    uint32_t flags;    // kWeakActive | kWeakCand | kWeakFit | kWeakRandom
+    uint32_t pad;
+};
+constexpr uint32_t kWeakActive = 1, kWeakCand = 2, kWeakFit = 4, kWeakRandom = 8;
+// slots of the weak window terms: the current plane, the fitted plane, the 5 hypotheses
+constexpr int kWeakTermSlots = 7;
+
 struct Dev {
     int W, H, N, NV, P;
     int half_rows, Wh, Q;   // colour indexing
@@ -101,6 +119,8 @@ struct Dev {
     float *weak_wpre[2];    // [kWeakPatchTaps][count] bilateral weight of every tap
     float *weak_wsum[2];    // [3 * kNeighbourNum][count]
     float *weak_cost;       // [8][NV][count] candidate cost rows of the launched colour (k_weak_cand)
+    WeakState *weak_st;     // [count] of the launched colour
+    float *weak_terms;      // [kWeakTermSlots][NV][count] weighted window terms (k_weak_terms)
     // GenNeighbours constants (DPE.cu:2151-2157, host double math)
     float gn_cos, gn_sin, gn_thr;
     int gn_shift;
