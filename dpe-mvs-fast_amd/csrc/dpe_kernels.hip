@@ -1285,7 +1285,7 @@ __device__ float joint_view_selection(const Dev &d, int iter, CostFn cost, const
 
 // View selection, candidate adoption and refinement hypotheses of
 // CheckerboardPropagationStrong (DPE.cu:1551-1658, 1069-1099).
-__global__ void k_strong_select(Dev d, int colour, int iter) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_strong_select(Dev d, int colour, int iter) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= d.Q) return;
     const int Q = d.Q;
