@@ -31,7 +31,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # peak is 157.3e12 / 2 / 2 / 64 = 614.5e9 wave instructions per second (256 CUs x
 # 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction).
 VALU_ISSUE_PEAK = 157.3e12 / 2 / 2 / 64
-FAMILY_NAMES = ["k_ncc_full<6>", "k_ncc_weighted<6,REFINE>", "k_ncc_weighted<6,POST>", "k_weak_update"]
+# the WEAK family is the weak half-iteration: k_weak_patch, k_weak_cand, k_weak_select,
+# k_weak_terms (twice), k_weak_mid, k_weak_final
+FAMILY_NAMES = ["k_ncc_full<6>", "k_ncc_weighted<6,REFINE>", "k_ncc_weighted<6,POST>", "weak half-iteration"]
 # HBM traffic of one k_ncc_full<6> launch on this workload, from two rocprofv3
 # --pmc passes (FETCH_SIZE, WRITE_SIZE) over `python bench.py --no-cpu-baseline`,
 # each scaled by the counter's reading for a 256 MiB device copy

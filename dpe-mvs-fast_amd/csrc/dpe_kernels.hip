@@ -1686,7 +1686,7 @@ __device__ __forceinline__ bool pixel_valid(float f) { return __builtin_bit_cast
 // GenNeighbours, DPE.cu:2107-2467
 // cache_words: 32-bit words per lane of the dynamic LDS pair table
 // (edge_pair_bit; word w of lane l at [w * 64 + l]), 0 = no table.
-__global__ __launch_bounds__(256) void k_gen_neighbours(Dev d, int cache_words) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_gen_neighbours(Dev d, int cache_words) {
     extern __shared__ uint32_t edge_cache[];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.weak_count) return;
@@ -2042,7 +2042,7 @@ __global__ void k_fit_copy(Dev d) {
     if (d.weak[p] != DPE_WEAK) d.fit[p] = d.planes[p];
 }
 
-__global__ void k_ransac(Dev d, int iter) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_ransac(Dev d, int iter) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.weak_count) return;
     const int p = d.weak_list[i];

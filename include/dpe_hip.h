@@ -137,7 +137,8 @@ typedef struct dpe_stats {
     int64_t ncc_hyps;         /* all families */
     int64_t ncc_taps;         /* all families */
     int32_t ncc_launches;     /* launches of families FULL, REFINE and POST */
-    float family_ms[DPE_NCC_FAMILIES];   /* the WEAK family's kernel also does the view selection */
+    float family_ms[DPE_NCC_FAMILIES];   /* the WEAK family's time is its whole half-iteration (view selection and
+                                            acceptance included); family_launches counts half-iterations */
     int64_t family_hyps[DPE_NCC_FAMILIES];
     int64_t family_taps[DPE_NCC_FAMILIES];
     int32_t family_launches[DPE_NCC_FAMILIES];
