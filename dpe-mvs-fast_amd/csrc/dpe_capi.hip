@@ -82,7 +82,7 @@ enum BufId {
     B_F_VIEWS, B_F_DEPTH, B_F_NORMAL, B_F_STATES, B_F_COLOR, B_F_MASK, B_F_CLAIM, B_F_USED0, B_F_USED1, B_F_ACC0,
     B_F_ACC1, B_F_FLAGS, B_F_INDEX, B_F_TEMP, B_F_POINTS, B_F_CHANGED, B_POSTPLANES, B_WEAKLIST, B_ADJ_WEIGHTS,
     B_ADJ_GRADS, B_QUADS_H, B_FLAG, B_WPRE, B_WSUM, B_VC64, B_WEAKCOUNTS, B_WEAKTEMP, B_NOWROW, B_CURCOST,
-    B_CURVALID, B_LASTCAND, B_WEAKCOST, B_WEAKST, B_WEAKTERMS, B_COUNT
+    B_CURVALID, B_LASTCAND, B_WEAKCOST, B_WEAKST, B_WEAKTERMS, B_CANDALIAS, B_COUNT
 };
 
 struct Fail {
@@ -625,6 +625,7 @@ dpe_status process_impl(dpe_ctx *ctx, const dpe_params *prm, const dpe_form *for
     }
     // half-iteration and post-pass scratch
     d.cand_pos = (int *)buffer(ctx, B_CANDPOS, sizeof(int) * kStrongSlots * Q);
+    d.cand_alias = (uint8_t *)buffer(ctx, B_CANDALIAS, (size_t)kStrongSlots * Q);
     // one cost matrix per colour: a candidate repeating its slot's last plane keeps
     // the row from the colour's previous half-iteration (k_ncc_full)
     float *cost_mat =

@@ -522,6 +522,17 @@ __device__ __forceinline__ bool repeated_row(float4 *__restrict__ last, size_t a
     return false;
 }
 
+// A strong candidate whose plane repeats another row of its pixel's
+// half-iteration (cand_alias, k_strong_candidates) is read from that row and
+// not evaluated; its own row is then stale, so its last-plane record is
+// cleared (all-ones bits: none).  Returns true for such a slot.
+__device__ __forceinline__ bool aliased_row(const Dev &d, float4 *__restrict__ last, int slot, int Qn, int q) {
+    if (d.cand_alias[(size_t)slot * Qn + q] == slot) return false;
+    const float none = __uint_as_float(0xFFFFFFFFu);
+    last[(size_t)slot * Qn + q] = make_float4(none, none, none, none);
+    return true;
+}
+
 // Per-view cost matrix out[(slot*NV + v-1)*Qn + q] of the plane of every
 // (pixel, slot): plane_src[cand_pos[slot*Qn+q]] if cand_pos is given (a -1
 // entry skips the slot), else plane_src[slot*Qn + q].  With view_mask (indexed
@@ -582,6 +593,7 @@ __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_full(Dev d, int colour, 
             if (pos >= 0) {
                 const float4 pl = cand_pos ? plane_src[pos] : plane_src[(size_t)slot * Qn + q];
                 need = !(last && slot > 0 && repeated_row(last, (size_t)slot * Qn + q, pl));
+                if (need && last && slot > 0) need = !aliased_row(d, last, slot, Qn, q);
                 if (need && cached && slot == 0) {
                     const uint32_t known = mask & d.cur_valid[p];
                     for (uint32_t b = known; b; b &= b - 1) {
@@ -639,8 +651,10 @@ __global__ __launch_bounds__(256) void k_ncc_full_direct(Dev d, int colour, int 
         } else {
             pl = plane_src[(size_t)slot * Qn + q];
         }
-        // a candidate repeating its slot's last plane keeps its row (k_ncc_full)
+        // a candidate repeating its slot's last plane keeps its row, one repeating
+        // another row of the pixel reads that one (k_ncc_full)
         if (active && last && slot > 0) active = !repeated_row(last, (size_t)slot * Qn + q, pl);
+        if (active && last && slot > 0) active = !aliased_row(d, last, slot, Qn, q);
     }
     unsigned long long hyps = 0, taps = 0;
     if (active) {
@@ -1222,6 +1236,42 @@ __global__ void k_strong_candidates(Dev d, int colour, int iter) {
         }
         for (int i = 0; i < 8; ++i) cp[(size_t)(1 + i) * Q] = pos[i];
     }
+    // Candidate rows that repeat a plane of this pixel's half-iteration: a row
+    // is a pure function of (pixel, plane, view), so a slot whose plane equals,
+    // bit for bit, the current plane's or that of a slot before it (in the
+    // order the candidates are taken: the scanned ones, then the fixed-step
+    // ones; far before near without edges) reads that row; cand_alias holds
+    // the row each slot's costs are read from (the slot itself otherwise).
+    {
+        const float4 cur = d.planes[p];
+        uint8_t *al = d.cand_alias + q;
+        al[0] = 0;
+        constexpr int order_plain[8] = {2, 4, 6, 8, 1, 3, 5, 7};
+        float4 own[kStrongSlots - 1];
+        bool is_own[kStrongSlots - 1];
+#pragma unroll
+        for (int k = 0; k < kStrongSlots - 1; ++k) {
+            is_own[k] = false;
+            own[k] = make_float4(0, 0, 0, 0);
+            if (k >= nslot - 1) continue;
+            const int slot = d.prm.use_edge ? 1 + k : order_plain[k];
+            const int pos = cp[(size_t)slot * Q];
+            int a = slot;
+            if (pos >= 0) {
+                const float4 pl = d.planes[pos];
+                if (same_plane(pl, cur)) {
+                    a = 0;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < k; ++j)
+                        if (a == slot && is_own[j] && same_plane(own[j], pl)) a = d.prm.use_edge ? 1 + j : order_plain[j];
+                }
+                own[k] = pl;
+                is_own[k] = a == slot;
+            }
+            al[(size_t)slot * Q] = (uint8_t)a;
+        }
+    }
     const long nb[4] = {(long)p - W, (long)p + W, (long)p - 1, (long)p + 1};
     for (int i = 0; i < 4; ++i) d.nsel[(size_t)i * Q + q] = (nb[i] >= 0 && nb[i] < (long)d.P) ? d.selected[nb[i]] : 0u;
 }
@@ -1296,7 +1346,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
     const int NV = d.NV;
     const dpe_camera &cam = d.cams[0];
     const float *cm = d.cost_mat;
-    // chosen slot of every candidate row (-1: no candidate, the row stays {2,0,..} / 0)
+    // chosen slot of every candidate row (-1: no candidate, the row stays {2,0,..} / 0);
+    // a slot's costs are read from the row cand_alias names (the same plane's)
+    const uint8_t *alias = d.cand_alias + q;
     int row_slot[8];
     for (int i = 0; i < 8; ++i) row_slot[i] = d.cand_pos[(size_t)(1 + i) * Q + q] >= 0 ? 1 + i : -1;
     if (d.prm.use_edge && !d.edge[p]) {
@@ -1305,10 +1357,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
         for (int i = 0; i < 8; ++i) {
             if (d.cand_pos[(size_t)(9 + i) * Q + q] < 0) continue;
             const bool had = row_slot[i] >= 0;
+            const int r0 = had ? alias[(size_t)row_slot[i] * Q] : 0, r1 = alias[(size_t)(9 + i) * Q];
             int good0 = 0, bad0 = 0, good1 = 0, bad1 = 0;
             for (int j = 0; j < NV; ++j) {
-                const float v0 = had ? cm[((size_t)row_slot[i] * NV + j) * Q + q] : ((i == 0 && j == 0) ? 2.0f : 0.0f);
-                const float v1 = cm[((size_t)(9 + i) * NV + j) * Q + q];
+                const float v0 = had ? cm[((size_t)r0 * NV + j) * Q + q] : ((i == 0 && j == 0) ? 2.0f : 0.0f);
+                const float v1 = cm[((size_t)r1 * NV + j) * Q + q];
                 if (v0 < good_thr) good0++;
                 if (v0 > bad_thr) bad0++;
                 if (v1 < good_thr) good1++;
@@ -1317,8 +1370,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
             if (!had || good1 > good0 || (good1 == good0 && bad1 < bad0)) row_slot[i] = 9 + i;
         }
     }
+    int row_read[8];
+    for (int i = 0; i < 8; ++i) row_read[i] = row_slot[i] >= 0 ? (int)alias[(size_t)row_slot[i] * Q] : -1;
     auto cost = [&](int i, int v) -> float {
-        const int s = row_slot[i];
+        const int s = row_read[i];
         if (s < 0) return (i == 0 && v == 0) ? 2.0f : 0.0f;
         return cm[((size_t)s * NV + v) * Q + q];
     };
@@ -1361,7 +1416,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
         if (db >= d.prm.depth_min && db <= d.prm.depth_max && final_costs[mi] < cost_now) {
             depth_now = db;
             plane_now = cand;
-            now_row = row_slot[mi];
+            now_row = row_read[mi];
             cost_now = final_costs[mi];
             d.selected[p] = tsel;
         }

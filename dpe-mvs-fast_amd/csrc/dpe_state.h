@@ -90,6 +90,7 @@ struct Dev {
     const int *label;
     // strong half-iteration scratch
     int *cand_pos;
+    uint8_t *cand_alias;  // [17][Q] cost_mat row holding each slot's plane (k_strong_candidates)
     float *cost_mat;      // the launched colour's [17][NV][Q] rows
     float4 *last_cand;    // [2][17][Q] plane of each slot's last evaluated row (all-ones: none)
     uint32_t *nsel;

@@ -890,13 +890,26 @@ bool same_plane(const F4 &a, const F4 &b) { return std::memcmp(&a, &b, sizeof(F4
 constexpr int kCandSlots = 17;
 // NCC family of candidate slot `slot`'s row at `center` with plane pl (see
 // Ctx::last_cand); records pl as the slot's last evaluated plane.
+// A candidate whose plane equals, bit for bit, the pixel's current plane or the
+// plane of a candidate taken before it in this half-iteration (one that did not
+// itself repeat an earlier plane) gives the same costs as that row; the GPU
+// reads that row instead of evaluating (and clears the slot's last-plane record,
+// its own row then being stale), so it counts as dead work here too.
+// tl_own holds this pixel's candidate planes that repeat no earlier one
+// (StrongCandidates resets it).
+thread_local int tl_own_n = 0;
+thread_local F4 tl_own[16];
 int candidate_family(const Ctx &C, int slot, int center, const F4 &pl) {
     F4 &l = C.last_cand[(size_t)slot * C.P + center];
     F4 none;
     std::memset(&none, 0xFF, sizeof(F4));
     const bool repeat = same_plane(l, pl) && !same_plane(l, none);
-    l = pl;
-    return repeat ? kFamDead : kFamFull;
+    bool alias = same_plane(pl, C.planes[center]);
+    for (int k = 0; k < tl_own_n; ++k) alias = alias || same_plane(tl_own[k], pl);
+    if (!alias && tl_own_n < 16) tl_own[tl_own_n++] = pl;
+    if (repeat) return kFamDead;
+    l = alias ? none : pl;
+    return alias ? kFamDead : kFamFull;
 }
 
 // RandomInitialization, DPE.cu:1039-1067
@@ -1067,6 +1080,7 @@ void RefinementWeak(Ctx &C, F4 *plane, float *depth, float *cost, Rng &rng, cons
 // the snapshot buffers.  flag/positions as in the reference.
 void StrongCandidates(const Ctx &C, int px, int py, int iter, bool flag[8], int positions[8],
                       float cost_array[8][32]) {
+    tl_own_n = 0;
     const int W = C.W, H = C.H;
     const int center = py * W + px;
     const std::vector<float> &costs = C.snap_costs;
