@@ -2607,7 +2607,6 @@ __global__ __launch_bounds__(64) void k_weak_mid(Dev d, int colour, int iter) {
         st.flags = fl | kWeakRandom;
     }
     st.plane_now = plane_now;
-    st.depth_now = depth_now;
     st.cost_now = cost_now;
 }
 
@@ -2626,7 +2625,7 @@ __global__ __launch_bounds__(64) void k_weak_final(Dev d, int colour) {
         const uint8_t *vw = d.vw + (size_t)p * kMaxImages;
         for (int i = 0; i < kMaxImages; ++i) vwl[i] = vw[i];
         const float dmin = d.prm.depth_min, dmax = d.prm.depth_max;
-        float depth_now = st.depth_now, cost_now = st.cost_now;
+        float cost_now = st.cost_now;
         float4 plane_now = st.plane_now;
         if (st.flags & kWeakRandom) {
             for (int i = 0; i < 5; ++i) {
@@ -2634,13 +2633,11 @@ __global__ __launch_bounds__(64) void k_weak_final(Dev d, int colour) {
                 const float tc = weak_term_sum(d, n, li, 2 + i, vwl) / wn;
                 const float db = depth_from_plane(cam, tp, x, y);
                 if (db >= dmin && db <= dmax && tc < cost_now) {
-                    depth_now = db;
                     plane_now = tp;
                     cost_now = tc;
                 }
             }
         }
-        (void)depth_now;
         float4 final_plane = d.planes[p];
         if (d.prm.state == DPE_REFINE_INIT) {
             if ((double)cost_now < (double)d.costs[p] - 0.1) {

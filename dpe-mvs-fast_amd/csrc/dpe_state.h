@@ -44,11 +44,11 @@ struct WeakState {
     float4 hyp[5];     // refinement hypotheses (k_weak_mid)
     float fc_mi;       // the argmin candidate's final cost
     float wn;          // view weight norm
-    float depth_now, cost_now;
+    float cost_now;    // best cost after the candidate and the fitted plane (k_weak_mid)
     uint32_t tsel;     // views selected with the candidates
     uint32_t rng_n;    // draws taken from the entry's random stream
     uint32_t flags;    // kWeakActive | kWeakCand | kWeakFit | kWeakRandom
-    uint32_t pad;
+    uint32_t pad[2];
 };
 constexpr uint32_t kWeakActive = 1, kWeakCand = 2, kWeakFit = 4, kWeakRandom = 8;
 // slots of the weak window terms: the current plane, the fitted plane, the 5 hypotheses
