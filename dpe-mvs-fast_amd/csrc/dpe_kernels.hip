@@ -608,12 +608,43 @@ __global__ __launch_bounds__(kLanes * WAVES) void k_ncc_full(Dev d, int colour, 
         if (need) items[total + lanes_below(b)] = (uint16_t)((k << 6) | lane);
         total += __popcll(b);
     }
-    wave_sync();
-    for (int base = 0; base < total; base += kLanes) {
+    // With item lists, the workgroup's lists are dealt to its waves 64 items at a
+    // time (item i of the concatenation to wave (i / 64) % WAVES): a wave working
+    // through its own list would end on a partial pass, and the waves' lists
+    // differ in length (wave 0 takes one slot more when WAVES does not divide
+    // the slot count).  All waves hold the same 64 pixels, so a wave reads an
+    // item's pixel from its own lane `src` as before.
+    __shared__ int list_first[WAVES + 1];
+    int n_items = total;
+    if (!dense) {
+        if (lane == 0) list_first[wave + 1] = total;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            list_first[0] = 0;
+            for (int w = 1; w <= WAVES; ++w) list_first[w] += list_first[w - 1];
+        }
+        __syncthreads();
+        n_items = list_first[WAVES];
+    } else {
+        wave_sync();
+    }
+    for (int base = dense ? 0 : wave * kLanes; base < n_items; base += dense ? kLanes : WAVES * kLanes) {
         const int i = base + lane;
-        const bool act = dense ? valid : i < total;
-        const uint32_t e = dense ? (uint32_t)i : act ? items[i] : 0u;
-        const int src = (int)(e & 63u), slot = split ? (int)(e >> 6) : wave + WAVES * (int)(e >> 6);
+        bool act;
+        uint32_t e;
+        int sw = wave;  // the wave whose list holds the item
+        if (dense) {
+            act = valid;
+            e = (uint32_t)i;
+        } else {
+            act = i < n_items;
+            sw = 0;
+            for (int w = 1; w < WAVES; ++w)
+                if (i >= list_first[w]) sw = w;
+            const uint16_t *lst = (const uint16_t *)((char *)full_lists + (size_t)sw * full_list_bytes(nslot, WAVES));
+            e = act ? lst[i - list_first[sw]] : 0u;
+        }
+        const int src = (int)(e & 63u), slot = split ? (int)(e >> 6) : sw + WAVES * (int)(e >> 6);
         const int ix = __shfl(x, src, kLanes), iy = __shfl(y, src, kLanes);
         const int iq = blockIdx.x * kLanes + src;
         uint32_t views = __shfl(mask, src, kLanes) & wave_views;
