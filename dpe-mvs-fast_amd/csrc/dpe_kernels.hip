@@ -2361,14 +2361,28 @@ __global__ __launch_bounds__(kLanes * kWeakCandWaves) void k_weak_cand(Dev d, in
     }
     uint16_t *it = items[wave];
     for (uint32_t m = mask; m; m &= m - 1) it[first++] = (uint16_t)((lane << 3) | __builtin_ctz(m));
-    wave_sync();
+    // the waves' lists, concatenated in entry order, are dealt to the waves 64
+    // items at a time, so that no wave ends on a partial pass of its own
+    __shared__ int list_first[kWeakCandWaves + 1];
+    if (lane == 0) list_first[wave + 1] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        list_first[0] = 0;
+        for (int w = 1; w <= kWeakCandWaves; ++w) list_first[w] += list_first[w - 1];
+    }
+    __syncthreads();
+    const int n_items = list_first[kWeakCandWaves];
+    const int block_li0 = xcd_block(blockIdx.x, gridDim.x) * kWeakCandWaves * kLanes;
     unsigned long long hyps = 0, taps = 0;
     const int NV = d.NV;
-    for (int base = 0; base < total; base += kLanes) {
+    for (int base = wave * kLanes; base < n_items; base += kWeakCandWaves * kLanes) {
         const int k = base + lane;
-        if (k < total) {
-            const uint32_t e = it[k];
-            const int lk = li0 + (int)(e >> 3), i = (int)(e & 7u);
+        if (k < n_items) {
+            int sw = 0;
+            for (int w = 1; w < kWeakCandWaves; ++w)
+                if (k >= list_first[w]) sw = w;
+            const uint32_t e = items[sw][k - list_first[sw]];
+            const int lk = block_li0 + sw * kLanes + (int)(e >> 3), i = (int)(e & 7u);
             const int p = list[lk];
             const int x = p % d.W, y = p / d.W;
             const short2 np = neighbour_point(d, p, i + 1);
