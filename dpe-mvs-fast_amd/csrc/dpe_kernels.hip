@@ -1307,14 +1307,36 @@ __global__ void k_strong_candidates(Dev d, int colour, int iter) {
     for (int i = 0; i < 4; ++i) d.nsel[(size_t)i * Q + q] = (nb[i] >= 0 && nb[i] < (long)d.P) ? d.selected[nb[i]] : 0u;
 }
 
+// Per-view sample counts of the joint view selection (each at most 15: 15
+// samples), 4 bits per view in two 64-bit words.  Read with a shift, so a
+// count indexed by a runtime view stays in registers (a byte array indexed so
+// lives in private memory).
+struct ViewCounts {
+    uint64_t lo = 0, hi = 0;
+    __device__ __forceinline__ int operator[](int v) const {
+        return (int)((v < 16 ? lo >> (4 * v) : hi >> (4 * (v - 16))) & 15u);
+    }
+    __device__ __forceinline__ void add(int v) {
+        if (v < 16) lo += 1ull << (4 * v);
+        else hi += 1ull << (4 * (v - 16));
+    }
+};
+
 // Multi-hypothesis joint view selection (DPE.cu:1551-1631), shared by the
 // strong select kernel and the weak propagation.  `cost(i, v)` reads row i of
-// the 8 x NV candidate cost matrix.
-template <class CostFn>
-__device__ float joint_view_selection(const Dev &d, int iter, CostFn cost, const float *priors, Rng &rng,
-                                      uint8_t *vwl, uint32_t *tsel_out) {
+// the 8 x NV candidate cost matrix, `prior(v)` gives view v's prior.  The
+// sampling probabilities live in a fixed-size register array: a runtime view
+// writes its entry through a select over all entries, and the CDF and the
+// sampling scans are unrolled over every view with a guard, so no array is
+// indexed by a runtime value (the same operations in the same order).
+template <class CostFn, class PriorFn>
+__device__ float joint_view_selection(const Dev &d, int iter, CostFn cost, PriorFn prior, Rng &rng, ViewCounts &vc,
+                                      uint32_t *tsel_out) {
+    constexpr int kV = kMaxImages - 1;
     const int NV = d.NV;
-    float probs[32];
+    float probs[kV];
+#pragma unroll
+    for (int k = 0; k < kV; ++k) probs[k] = 0.0f;
     const float thr = (float)(0.8 * (double)d_expf((float)(iter * iter) / (-90.0f)));
     for (int i = 0; i < NV; i++) {
         float count = 0;
@@ -1331,33 +1353,40 @@ __device__ float joint_view_selection(const Dev &d, int iter, CostFn cost, const
         float pr = 0.0f;
         if (count > 2 && count_false < 3) pr = tmpw / count;
         else if (count_false < 3) pr = d_expf(thr * thr / (-0.32f));
-        probs[i] = pr * priors[i];
+        const float val = pr * prior(i);
+#pragma unroll
+        for (int k = 0; k < kV; ++k) probs[k] = k == i ? val : probs[k];
     }
     float sum = 0.0f;  // TransformPDFToCDF, DPE.cu:297-311
-    for (int i = 0; i < NV; ++i) sum += probs[i];
+#pragma unroll
+    for (int k = 0; k < kV; ++k)
+        if (k < NV) sum += probs[k];
     const float inv = 1.0f / sum;
     float cum = 0.0f;
-    for (int i = 0; i < NV; ++i) {
-        const float pr = probs[i] * inv;
-        cum += pr;
-        probs[i] = cum;
+#pragma unroll
+    for (int k = 0; k < kV; ++k) {
+        if (k < NV) {
+            const float pr = probs[k] * inv;
+            cum += pr;
+            probs[k] = cum;
+        }
     }
-    for (int i = 0; i < kMaxImages; ++i) vwl[i] = 0;
+    vc = ViewCounts{};
     for (int s = 0; s < 15; ++s) {
         const float rp = rng.uniform() - 1.19209290e-07f;
-        for (int id = 0; id < NV; ++id) {
-            if (probs[id] > rp) {
-                vwl[id] += 1;
-                break;
-            }
-        }
+        // the first view whose CDF value exceeds the draw (none: no count)
+        int sel = -1;
+#pragma unroll
+        for (int k = kV - 1; k >= 0; --k)
+            if (k < NV && probs[k] > rp) sel = k;
+        if (sel >= 0) vc.add(sel);
     }
     uint32_t tsel = 0;
     float wn = 0;
     for (int i = 0; i < NV; ++i) {
-        if (vwl[i] > 0) {
+        if (vc[i] > 0) {
             tsel |= (1u << i);
-            wn += vwl[i];
+            wn += vc[i];
         }
     }
     *tsel_out = tsel;
@@ -1408,18 +1437,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
         if (s < 0) return (i == 0 && v == 0) ? 2.0f : 0.0f;
         return cm[((size_t)s * NV + v) * Q + q];
     };
-    float priors[32];
-    for (int j = 0; j < NV; ++j) priors[j] = 0.0f;
+    // view priors from the neighbours' selected views (each view's sum in neighbour order)
+    uint32_t nsv[4];
+    bool nuse[4];
     for (int i = 0; i < 4; ++i) {
-        if (row_slot[2 * i] >= 0) {
-            const uint32_t sv = d.nsel[(size_t)i * Q + q];
-            for (int j = 0; j < NV; ++j) priors[j] += isSet(sv, j) == 1 ? 0.9f : 0.1f;
-        }
+        nuse[i] = row_slot[2 * i] >= 0;
+        nsv[i] = nuse[i] ? d.nsel[(size_t)i * Q + q] : 0u;
     }
+    auto prior = [&](int j) -> float {
+        float pj = 0.0f;
+        for (int i = 0; i < 4; ++i)
+            if (nuse[i]) pj += isSet(nsv[i], j) == 1 ? 0.9f : 0.1f;
+        return pj;
+    };
     Rng rng(d.prm.seed, p, ph_strong(iter), d.prm.call_index);
-    uint8_t vwl[kMaxImages];
+    ViewCounts vwl;
     uint32_t tsel = 0;
-    const float wn = joint_view_selection(d, iter, cost, priors, rng, vwl, &tsel);
+    const float wn = joint_view_selection(d, iter, cost, prior, rng, vwl, &tsel);
     float final_costs[8];
     for (int i = 0; i < 8; ++i) {
         float fc = 0.0f;
@@ -1453,7 +1487,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
         }
     }
     uint8_t *vwo = d.vw + (size_t)p * kMaxImages;
-    for (int i = 0; i < kMaxImages; ++i) vwo[i] = vwl[i];
+    for (int i = 0; i < kMaxImages; ++i) vwo[i] = (uint8_t)vwl[i];
     // hypotheses of PlaneHypothesisRefinementStrong (DPE.cu:1080-1104)
     const float dmin = d.prm.depth_min, dmax = d.prm.depth_max;
     const float depth_rand = rng.uniform() * (dmax - dmin) + dmin;
@@ -2457,18 +2491,24 @@ __global__ __launch_bounds__(64) void k_weak_select(Dev d, int colour, int iter)
         }
     }
     auto cost = [&](int i, int v) -> float { return cost_array[i][v]; };
-    float priors[32];
-    for (int j = 0; j < NV; ++j) priors[j] = 0.0f;
+    // view priors from the GenNeighbours points' selected views (each view's sum in point order)
+    uint32_t nsv[8];
+    bool nuse[8];
     for (int i = 0; i < 8; ++i) {
         const short2 np = neighbour_point(d, p, i + 1);
-        if (np.x == -1 || np.y == -1) continue;
-        const uint32_t sv = d.selected[np.x + np.y * W];
-        for (int j = 0; j < NV; ++j) priors[j] += isSet(sv, j) == 1 ? 0.9f : 0.1f;
+        nuse[i] = !(np.x == -1 || np.y == -1);
+        nsv[i] = nuse[i] ? d.selected[np.x + np.y * W] : 0u;
     }
+    auto prior = [&](int j) -> float {
+        float pj = 0.0f;
+        for (int i = 0; i < 8; ++i)
+            if (nuse[i]) pj += isSet(nsv[i], j) == 1 ? 0.9f : 0.1f;
+        return pj;
+    };
     Rng rng(d.prm.seed, p, ph_weak(iter), d.prm.call_index);
-    uint8_t vwl[kMaxImages];
+    ViewCounts vwl;
     uint32_t tsel = 0;
-    const float wn = joint_view_selection(d, iter, cost, priors, rng, vwl, &tsel);
+    const float wn = joint_view_selection(d, iter, cost, prior, rng, vwl, &tsel);
     float final_costs[8];
     for (int i = 0; i < 8; ++i) {
         float fc = 0.0f;
@@ -2495,7 +2535,7 @@ __global__ __launch_bounds__(64) void k_weak_select(Dev d, int colour, int iter)
     // the view weights (the reference's selection writes them in place,
     // DPE.cu:1761-1771)
     uint8_t *vwo = d.vw + (size_t)p * kMaxImages;
-    for (int i = 0; i < kMaxImages; ++i) vwo[i] = vwl[i];
+    for (int i = 0; i < kMaxImages; ++i) vwo[i] = (uint8_t)vwl[i];
     const float4 fit = d.fit[p];
     uint32_t fl = kWeakActive;
     if (flag[mi]) fl |= kWeakCand;
