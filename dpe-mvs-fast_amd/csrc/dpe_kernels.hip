@@ -1731,27 +1731,36 @@ __global__ void k_gen_edge_inform(Dev d) {
 // the ring radius r* = min over STRONG q of max(|dx|, |dy|) comes from a
 // row-wise distance map g (k_row_strong_distance), then only ring r* is scanned
 // in the reference's order.  Cost O(r*) per pixel instead of O(r*^2).
+// g[y][x] = min(127, |x - x'|) over the STRONG pixels x' of row y (127 when
+// none lies within 126 pixels): one thread per pixel, scanning outwards from x
+// (neighbouring lanes read neighbouring bytes).
 __global__ void k_row_strong_distance(Dev d, int16_t *__restrict__ g) {
-    const int y = blockIdx.x * blockDim.x + threadIdx.x;
-    if (y >= d.H) return;
-    const int W = d.W;
-    const uint8_t *wk = d.weak + (size_t)y * W;
-    int16_t *gr = g + (size_t)y * W;
-    int last = -1000;
-    for (int x = 0; x < W; ++x) {
-        if (wk[x] == DPE_STRONG) last = x;
-        gr[x] = (int16_t)min(x - last, 127);
-    }
-    last = 1000000;
-    for (int x = W - 1; x >= 0; --x) {
-        if (wk[x] == DPE_STRONG) last = x;
-        gr[x] = (int16_t)min((int)gr[x], min(last - x, 127));
-    }
-}
-
-__global__ void k_find_nearest_strong(Dev d, const int16_t *__restrict__ g) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.P) return;
+    const int W = d.W;
+    const int x = p % W;
+    const uint8_t *wk = d.weak + (size_t)(p - x);
+    int best = 127;
+    for (int k = 0; k < 127; ++k) {
+        if ((x - k >= 0 && wk[x - k] == DPE_STRONG) || (x + k < W && wk[x + k] == DPE_STRONG)) {
+            best = k;
+            break;
+        }
+    }
+    g[p] = (int16_t)best;
+}
+
+// Pixels that are not WEAK have no nearest strong point: (-1, -1).
+__global__ void k_nearest_clear(Dev d) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < d.P && d.weak[p] != DPE_WEAK) d.nearest[p] = make_short2(-1, -1);
+}
+
+// One thread per entry of the WEAK list (the other pixels were cleared).
+__global__ void k_find_nearest_strong(Dev d, const int16_t *__restrict__ g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.weak_count) return;
+    const int p = d.weak_list[i];
     const int W = d.W, H = d.H;
     const int x = p % W, y = p / W;
     short2 result = make_short2(-1, -1);
@@ -2986,8 +2995,10 @@ void launch_gen_edge_inform(const Dev &d, hipStream_t s) {
 }
 void launch_find_nearest_strong(const Dev &d, hipStream_t s) {
     int16_t *g = (int16_t *)d.pcost;  // the post-pass scratch is free before the iterations
-    hipLaunchKernelGGL(k_row_strong_distance, grid1(d.H, 64), dim3(64), 0, s, d, g);
-    hipLaunchKernelGGL(k_find_nearest_strong, grid1(d.P), dim3(256), 0, s, d, (const int16_t *)g);
+    hipLaunchKernelGGL(k_row_strong_distance, grid1(d.P), dim3(256), 0, s, d, g);
+    hipLaunchKernelGGL(k_nearest_clear, grid1(d.P), dim3(256), 0, s, d);
+    if (d.weak_count > 0)
+        hipLaunchKernelGGL(k_find_nearest_strong, grid1(d.weak_count), dim3(256), 0, s, d, (const int16_t *)g);
 }
 void launch_gen_neighbours(const Dev &d, hipStream_t s) {
     if (d.weak_count > 0) {
