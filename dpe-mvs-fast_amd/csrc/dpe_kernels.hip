@@ -1700,7 +1700,20 @@ __global__ void k_gen_edge_inform(Dev d) {
         }
         d.complexity[p] = (float)(1.0f / (1.0f + d_exp(-25.0 * ((double)density - 0.35))));
     }
-    if (d.prm.use_label && d.weak[p] == DPE_WEAK) {
+}
+
+// GenEdgeInform's label bounds of the WEAK pixels (DPE.cu:2564-2594), one thread
+// per entry of the WEAK list: their walks are long and one thread per pixel
+// left the lanes of the other pixels idle.
+__global__ void k_label_bounds(Dev d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.weak_count) return;
+    const int p = d.weak_list[i];
+    const int W = d.W, H = d.H;
+    const int x = p % W, y = p / W;
+    const int dirx[8] = {0, 0, -1, 1, -1, 1, -1, 1};
+    const int diry[8] = {-1, 1, 0, 0, -1, 1, 1, -1};
+    if (d.weak[p] == DPE_WEAK) {
         short2 *lb = d.label_bound + (size_t)d.nmap[p] * 8;
         const int cl = d.label[p];
         if (cl > 0) {
@@ -2991,7 +3004,9 @@ hipError_t launch_weak_lists(const uint8_t *weak, int W, int H, int half_rows, i
     return hipGetLastError();
 }
 void launch_gen_edge_inform(const Dev &d, hipStream_t s) {
-    hipLaunchKernelGGL(k_gen_edge_inform, grid1(d.P), dim3(256), 0, s, d);
+    if (d.prm.use_edge) hipLaunchKernelGGL(k_gen_edge_inform, grid1(d.P), dim3(256), 0, s, d);
+    if (d.prm.use_label && d.weak_count > 0)
+        hipLaunchKernelGGL(k_label_bounds, grid1(d.weak_count), dim3(256), 0, s, d);
 }
 void launch_find_nearest_strong(const Dev &d, hipStream_t s) {
     int16_t *g = (int16_t *)d.pcost;  // the post-pass scratch is free before the iterations
