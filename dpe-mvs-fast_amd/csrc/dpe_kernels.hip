@@ -1455,18 +1455,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
     uint32_t tsel = 0;
     const float wn = joint_view_selection(d, iter, cost, prior, rng, vwl, &tsel);
     float final_costs[8];
+#pragma unroll
     for (int i = 0; i < 8; ++i) {
         float fc = 0.0f;
         for (int j = 0; j < NV; ++j)
             if (vwl[j] > 0) fc += vwl[j] * cost(i, j);
         final_costs[i] = fc / wn;
     }
-    int mi = 0;  // FindMinCostIndex, DPE.cu:50-61
-    {
-        float mc = final_costs[0];
-        for (int i = 1; i < 8; ++i)
-            if (final_costs[i] <= mc) { mc = final_costs[i]; mi = i; }
-    }
+    // FindMinCostIndex, DPE.cu:50-61; the minimum's slot and row are carried
+    // along (unrolled), so no array is indexed by the runtime minimum
+    float fc_mi = final_costs[0];
+    int slot_mi = row_slot[0], read_mi = row_read[0];
+#pragma unroll
+    for (int i = 1; i < 8; ++i)
+        if (final_costs[i] <= fc_mi) {
+            fc_mi = final_costs[i];
+            slot_mi = row_slot[i];
+            read_mi = row_read[i];
+        }
     float cost_now = 0.0f;
     for (int i = 0; i < NV; ++i) cost_now += vwl[i] * cm[((size_t)0 * NV + i) * Q + q];
     cost_now /= wn;
@@ -1475,14 +1481,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
     float depth_now = depth_from_plane(cam, cur, x, y);
     float4 plane_now = cur;
     int now_row = 0;
-    if (row_slot[mi] >= 0) {
-        const float4 cand = d.planes[d.cand_pos[(size_t)row_slot[mi] * Q + q]];
+    if (slot_mi >= 0) {
+        const float4 cand = d.planes[d.cand_pos[(size_t)slot_mi * Q + q]];
         const float db = depth_from_plane(cam, cand, x, y);
-        if (db >= d.prm.depth_min && db <= d.prm.depth_max && final_costs[mi] < cost_now) {
+        if (db >= d.prm.depth_min && db <= d.prm.depth_max && fc_mi < cost_now) {
             depth_now = db;
             plane_now = cand;
-            now_row = row_read[mi];
-            cost_now = final_costs[mi];
+            now_row = read_mi;
+            cost_now = fc_mi;
             d.selected[p] = tsel;
         }
     }
@@ -1654,8 +1660,27 @@ __device__ bool point_in_triangle(short2 A, short2 B, short2 C, int px, int py) 
     return t1 * t2 >= 0 && t1 * t3 >= 0;
 }
 
+// GenEdgeInform's (2r + 1)^2 window counts (DPE.cu:2527-2557), taken in two
+// passes: rc[p] = the edge pixels and rc[P + p] the label-0 pixels among
+// (x - r .. x + r, y) inside the image; the window's counts are the sums of
+// 2r + 1 such row counts (integers: the same counts as the cell-by-cell loop).
+__global__ void k_edge_row_counts(Dev d, uint16_t *__restrict__ rc) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    const int W = d.W, x = p % W, r = d.prm.strong_radius;
+    const int x0 = max(x - r, 0), x1 = min(x + r, W - 1);
+    int e = 0, b = 0;
+    for (int nx = x0; nx <= x1; ++nx) {
+        const int q = p - x + nx;
+        e += d.edge[q] != 0;
+        if (d.prm.use_label) b += d.label[q] == 0;
+    }
+    rc[p] = (uint16_t)e;
+    rc[(size_t)d.P + p] = (uint16_t)b;
+}
+
 // GenEdgeInform, DPE.cu:2487-2595
-__global__ void k_gen_edge_inform(Dev d) {
+__global__ void k_gen_edge_inform(Dev d, const uint16_t *__restrict__ rc) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.P) return;
     const int W = d.W, H = d.H;
@@ -1678,26 +1703,18 @@ __global__ void k_gen_edge_inform(Dev d) {
             }
             en[i] = r;
         }
+        // the window's in-image pixels, edge pixels and label-0 pixels: sums of
+        // the row counts over the window's rows
         const int radius = d.prm.strong_radius;
-        int edge_pix = 0, tot_pix = 0;
-        for (int i = -radius; i <= radius; i++)
-            for (int j = -radius; j <= radius; j++) {
-                const int nx = x + i, ny = y + j;
-                if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
-                if (d.edge[ny * W + nx]) edge_pix++;
-                tot_pix++;
-            }
-        float density = 1.0f * edge_pix / tot_pix;
-        if (d.prm.use_label) {
-            int bound_pix = 0;
-            for (int i = -radius; i <= radius; i++)
-                for (int j = -radius; j <= radius; j++) {
-                    const int nx = x + i, ny = y + j;
-                    if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
-                    if (d.label[ny * W + nx] == 0) bound_pix++;
-                }
-            density = fmaxf(density, (float)(bound_pix / tot_pix));
+        const int y0 = max(y - radius, 0), y1 = min(y + radius, H - 1);
+        const int tot_pix = max(0, min(x + radius, W - 1) - max(x - radius, 0) + 1) * max(0, y1 - y0 + 1);
+        int edge_pix = 0, bound_pix = 0;
+        for (int ny = y0; ny <= y1; ++ny) {
+            edge_pix += rc[(size_t)ny * W + x];
+            bound_pix += rc[(size_t)d.P + (size_t)ny * W + x];
         }
+        float density = 1.0f * edge_pix / tot_pix;
+        if (d.prm.use_label) density = fmaxf(density, (float)(bound_pix / tot_pix));
         d.complexity[p] = (float)(1.0f / (1.0f + d_exp(-25.0 * ((double)density - 0.35))));
     }
 }
@@ -3004,7 +3021,12 @@ hipError_t launch_weak_lists(const uint8_t *weak, int W, int H, int half_rows, i
     return hipGetLastError();
 }
 void launch_gen_edge_inform(const Dev &d, hipStream_t s) {
-    if (d.prm.use_edge) hipLaunchKernelGGL(k_gen_edge_inform, grid1(d.P), dim3(256), 0, s, d);
+    if (d.prm.use_edge) {
+        // the row counts (2 x P 16-bit words) in the post-pass scratch, free before the iterations
+        uint16_t *rc = (uint16_t *)d.pcost;
+        hipLaunchKernelGGL(k_edge_row_counts, grid1(d.P), dim3(256), 0, s, d, rc);
+        hipLaunchKernelGGL(k_gen_edge_inform, grid1(d.P), dim3(256), 0, s, d, (const uint16_t *)rc);
+    }
     if (d.prm.use_label && d.weak_count > 0)
         hipLaunchKernelGGL(k_label_bounds, grid1(d.weak_count), dim3(256), 0, s, d);
 }
