@@ -40,7 +40,7 @@ FAMILY_NAMES = ["k_ncc_full<6>", "k_ncc_weighted<6,REFINE>", "k_ncc_weighted<6,P
 # (`python bench.py --copy-calibration`: FETCH_SIZE x2.000, WRITE_SIZE x1.000);
 # the runs and the arithmetic are in profiles/ncc_kernels_1600x1200_20views_hbm_traffic.md.
 # Counters cannot be read inside a timed run.
-NCC_FULL_TRAFFIC_BYTES = 6.8673e9
+NCC_FULL_TRAFFIC_BYTES = 6.8665e9
 # VALU wave instructions of one k_ncc_full<6> launch on this workload (SQ_INSTS_VALU
 # summed over the step's launches / launches, one rocprofv3 --pmc pass over
 # `python bench.py --no-cpu-baseline`; profiles/ncc_kernels_1600x1200_20views_valu_issue.md).
